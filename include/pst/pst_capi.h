@@ -1,0 +1,195 @@
+/*
+ * C ABI of libpst_hip.so: the MI355X (gfx950) static priority search tree.
+ *
+ * The reference (arraylistnumstring/static-priority-search-tree) exposes its
+ * hot path as header-only C++ class templates; there is no FFI. This header
+ * is the flat boundary those templates are re-implemented over (see
+ * include/pst/static-pst-gpu.h, static-pst-gpu-arr.h and
+ * interval-parallel-search.h for the drop-in C++ classes). Each entry point
+ * names the reference member it replaces (paths relative to the reference).
+ *
+ * Conventions
+ *  - All pointers named *_d are device pointers on the current HIP device.
+ *  - Point arrays use the reference's AoS PointStruct<T, IDType, num_IDs>
+ *    layout {T dim1_val; T dim2_val; [IDType id;]} with C struct padding
+ *    (helpers/point-struct.h:25-33, 107-117); pst_point_size() gives its size.
+ *  - Bounds and search values are passed as pointers to ONE host value of
+ *    the tree's data type.
+ *  - Every function returns PST_OK (0) or an error code; pst_last_error()
+ *    then returns a message in the reference's style, "Error in <what> on
+ *    device <i> (1-indexed) of <n>: <HIP error string>"
+ *    (helpers/gpu-err-chk.h:9-13).
+ *  - Results of the blocking searches are returned as a device array the
+ *    library allocates with hipMalloc, of num_elems * sizeof(RetType) bytes
+ *    as the reference does (pst/static-pst-gpu.tu:563); release it with
+ *    pst_free() or hipFree(). Only the first *num_res_elems entries are
+ *    valid; their order is unspecified.
+ *  - An empty tree prints "Tree is empty; nothing to search" and returns
+ *    *num_res_elems = 0, *res_arr_d = NULL (pst/static-pst-gpu.tu:504-510).
+ *  - Handles are not shared between threads without external locking; each
+ *    handle owns its own device counters (the reference's single global
+ *    counter, helpers/dev-symbols.h:6, made searches non re-entrant).
+ */
+#ifndef PST_CAPI_H
+#define PST_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* hipStream_t-compatible opaque stream handle (NULL = default stream). */
+typedef struct ihipStream_t* pst_stream_t;
+
+enum pst_status {
+  PST_OK = 0,
+  PST_ERR_INVALID = 1,  /* bad argument or unsupported type combination */
+  PST_ERR_HIP = 2,      /* a HIP runtime call failed */
+  PST_ERR_NAN = 3,      /* a dimension (or floating ID) is NaN: no order */
+  PST_ERR_CAPACITY = 4, /* results exceeded the caller's buffer (counted only) */
+  PST_ERR_INTERNAL = 5
+};
+
+/* Data type codes: T in {F32, F64, I32, I64} (the reference's float,
+ * double, int, long: testers/pst-test-info-struct.h:72-119); IDType in all
+ * six or PST_NO_ID (num_IDs = 0) (pst-test-info-struct.h:234-284). */
+enum pst_dtype {
+  PST_F32 = 0,
+  PST_F64 = 1,
+  PST_I32 = 2,
+  PST_I64 = 3,
+  PST_U32 = 4,
+  PST_U64 = 5,
+  PST_NO_ID = -1
+};
+
+/* RetType: IDType or the whole PointStruct (pst/static-pst-gpu.h:103-128). */
+enum pst_ret { PST_RET_ID = 0, PST_RET_POINT = 1 };
+
+/* Query kinds; bounds are inclusive.
+ *  THREE: min_dim1 <= dim1 <= max_dim1 && dim2 >= min_dim2
+ *  LEFT : dim1 <= max_dim1 && dim2 >= min_dim2   (min_dim1 ignored)
+ *  RIGHT: dim1 >= min_dim1 && dim2 >= min_dim2   (max_dim1 ignored) */
+enum pst_query_kind { PST_THREE_SIDED = 0, PST_TWO_SIDED_LEFT = 1, PST_TWO_SIDED_RIGHT = 2 };
+
+/* Batched query records (device arrays), one struct per data type. */
+typedef struct { float min_dim1, max_dim1, min_dim2; uint32_t kind; } pst_query_f32;
+typedef struct { double min_dim1, max_dim1, min_dim2; uint32_t kind; } pst_query_f64;
+typedef struct { int32_t min_dim1, max_dim1, min_dim2; uint32_t kind; } pst_query_i32;
+typedef struct { int64_t min_dim1, max_dim1, min_dim2; uint32_t kind; } pst_query_i64;
+
+typedef struct pst_tree pst_tree;     /* StaticPSTGPU */
+typedef struct pst_forest pst_forest; /* StaticPSTGPUArr */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* pst_last_error(void);
+/* sizeof(PointStruct<T, IDType, num_IDs>) for the given codes (0 if invalid). */
+size_t pst_point_size(int dtype, int id_dtype);
+/* Releases a result array returned by a blocking search. */
+int pst_free(void* res_arr_d);
+
+/* ------------------------------------------------------------------ */
+/* StaticPSTGPU (pst/static-pst-gpu.h:72-477)                          */
+/* ------------------------------------------------------------------ */
+
+/* Replaces StaticPSTGPU::StaticPSTGPU(pt_arr_d, num_elems, warp_multiplier,
+ * dev_ind, num_devs, dev_props) (pst/static-pst-gpu.h:76-78,
+ * pst/static-pst-gpu.tu:19-437). pt_arr_d is read during construction only
+ * and not retained. warp_multiplier is accepted for signature parity; the
+ * launch geometry is the library's own (wave64). Blocking. */
+int pst_gpu_create(const void* pt_arr_d, size_t num_elems, int dtype, int id_dtype,
+                   int warp_multiplier, int dev_ind, int num_devs, pst_stream_t stream,
+                   pst_tree** out);
+/* Replaces ~StaticPSTGPU (pst/static-pst-gpu.h:80-88). */
+int pst_gpu_destroy(pst_tree* tree);
+/* Getters (pst/static-pst-gpu.h:96-98) and sizes. */
+size_t pst_gpu_num_elems(const pst_tree* tree);
+int pst_gpu_dev_ind(const pst_tree* tree);
+int pst_gpu_num_devs(const pst_tree* tree);
+
+/* Replaces StaticPSTGPU::threeSidedSearch(num_res_elems, res_arr_d,
+ * min_dim1_val, max_dim1_val, min_dim2_val, warp_multiplier)
+ * (pst/static-pst-gpu.h:108-110, pst/static-pst-gpu.tu:489-539). Blocking. */
+int pst_gpu_three_sided_search(pst_tree* tree, int ret, size_t* num_res_elems, void** res_arr_d,
+                               const void* min_dim1_val, const void* max_dim1_val,
+                               const void* min_dim2_val);
+/* Replaces StaticPSTGPU::twoSidedLeftSearch(num_res_elems, res_arr_d,
+ * max_dim1_val, min_dim2_val, warp_multiplier)
+ * (pst/static-pst-gpu.h:117-119, pst/static-pst-gpu.tu:541-591). Blocking. */
+int pst_gpu_two_sided_left_search(pst_tree* tree, int ret, size_t* num_res_elems, void** res_arr_d,
+                                  const void* max_dim1_val, const void* min_dim2_val);
+/* Replaces StaticPSTGPU::twoSidedRightSearch(num_res_elems, res_arr_d,
+ * min_dim1_val, min_dim2_val, warp_multiplier)
+ * (pst/static-pst-gpu.h:126-128, pst/static-pst-gpu.tu:593-642). Blocking. */
+int pst_gpu_two_sided_right_search(pst_tree* tree, int ret, size_t* num_res_elems, void** res_arr_d,
+                                   const void* min_dim1_val, const void* min_dim2_val);
+
+/* One search into a caller-owned device buffer of `capacity` records
+ * (no allocation). Blocking; *num_res_elems is the full result count, and
+ * PST_ERR_CAPACITY is returned (with the first `capacity` results written)
+ * when it exceeds capacity. No counterpart in the reference: the
+ * preallocated-output form of the three searches above. */
+int pst_gpu_search_into(pst_tree* tree, int kind, int ret, const void* min_dim1_val,
+                        const void* max_dim1_val, const void* min_dim2_val, void* out_d,
+                        size_t capacity, size_t* num_res_elems, pst_stream_t stream);
+
+/* Replaces StaticPSTGPU::calcGlobalMemNeeded(num_elems)
+ * (pst/static-pst-gpu.h:132, pst/static-pst-gpu.tu:646-671): bytes of
+ * device memory construction and one blocking search need. */
+size_t pst_gpu_calc_global_mem_needed(size_t num_elems, int dtype, int id_dtype);
+
+/* The tree in the reference's memory layout, copied to host memory:
+ * dim1[S] dim2[S] med[S] (T), ids[S] (IDType, offset rounded up to its
+ * alignment), bitcodes[S] (HAS_LEFT 0x2 | HAS_RIGHT 0x1), S = 2^ceil(log2(n+1))-1
+ * (pst/static-pst-gpu.h:316-365). offsets6 receives the byte offsets of the
+ * five arrays and S. Used by StaticPSTGPU::print and parity tests. */
+int pst_gpu_export_tree(pst_tree* tree, void* host_blob, size_t blob_bytes, size_t* offsets6);
+size_t pst_gpu_export_bytes(const pst_tree* tree);
+
+/* ---- Batched searches (no counterpart in the reference) ----
+ * queries_d: device array of pst_query_<dtype>. Count pass then write pass:
+ *   pst_gpu_batch_count   counts_d[q] = number of results of query q
+ *   pst_gpu_batch_write   writes query q's results at out_d + offsets_d[q]
+ *                          (records of RetType), at most caps_d[q] of them
+ *                          (caps_d may be NULL: no limit beyond offsets).
+ * Both are asynchronous on `stream` and allocate nothing after the first
+ * call with a given batch size (scratch is kept by the handle). */
+int pst_gpu_batch_count(pst_tree* tree, const void* queries_d, size_t num_queries,
+                        unsigned long long* counts_d, pst_stream_t stream);
+int pst_gpu_batch_write(pst_tree* tree, const void* queries_d, size_t num_queries, int ret,
+                        const unsigned long long* offsets_d, const unsigned long long* caps_d,
+                        void* out_d, pst_stream_t stream);
+/* Tuning knobs of the search rounds (0 keeps the default). */
+int pst_gpu_set_search_tuning(pst_tree* tree, int rounds, int budget_small, int budget_large,
+                              int waves_per_cu);
+
+/* ------------------------------------------------------------------ */
+/* Interval parallel search (interval-parallel-search/                 */
+/* interval-parallel-search.h:23-66)                                   */
+/* ------------------------------------------------------------------ */
+
+/* Replaces intervalParallelSearch<T, PointStructTemplate, IDType, num_IDs,
+ * RetType>(pt_arr_d, num_elems, res_arr_d, num_res_elems, search_val,
+ * dev_ind, num_devs, warp_size, num_thread_blocks, threads_per_block):
+ * reports points with dim1 <= search_val <= dim2. warp_size and
+ * threads_per_block are accepted for signature parity; num_thread_blocks =
+ * 0 lets the library size the grid. Blocking; allocates
+ * num_elems * sizeof(RetType) for the result like the reference. */
+int pst_interval_parallel_search(const void* pt_arr_d, size_t num_elems, int dtype, int id_dtype,
+                                 int ret, void** res_arr_d, size_t* num_res_elems,
+                                 const void* search_val, int dev_ind, int num_devs, int warp_size,
+                                 unsigned num_thread_blocks, unsigned threads_per_block);
+/* Same filter into a caller buffer; asynchronous. counter_d (one
+ * unsigned long long, zeroed by the call) receives the hit count. */
+int pst_interval_parallel_search_into(const void* pt_arr_d, size_t num_elems, int dtype,
+                                      int id_dtype, int ret, const void* search_val, void* out_d,
+                                      size_t capacity, unsigned long long* counter_d,
+                                      pst_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PST_CAPI_H */

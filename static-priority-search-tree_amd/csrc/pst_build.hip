@@ -1,0 +1,597 @@
+// Tree construction for StaticPSTGPU on gfx950.
+//
+// Replaces the reference's construction (pst/static-pst-gpu.tu:19-437:
+// index init, two thrust::sorts, then populateTree, a one-block kernel that
+// builds nodes thread-per-node and relaunches itself through dynamic
+// parallelism, pst/static-pst-gpu-populate-tree.tu:10-158) with rank arrays
+// and level-synchronous passes:
+//
+//   1. key2 ranks: r2[i] = position of point i in the key2-descending order
+//      (key2 = (dim2, id, index), the reference's compareDim2 order).
+//   2. A = r2 listed in key1-ascending order (key1 = (dim1, id, index)).
+//      From here on construction only compares 32-bit ranks.
+//   3. Level t holds 2^t segments of A, one per slot, in slot order; their
+//      sizes and offsets follow from n alone (pst_common.hpp). Per segment,
+//      the node is the minimum rank (max key2); med is the dim1 of the
+//      floor(m/2)-th remaining point; removing the node from A (a coalesced
+//      compaction) yields level t+1, whose left/right segments are the
+//      first floor(m/2) / the remaining points.
+//   4. Once every segment fits in LDS (<= kFinishTile points), one
+//      workgroup finishes its whole subtree in LDS.
+// The node chosen, the median rule and the child split are those of the
+// reference's constructNode (pst/static-pst-gpu.tu:706-806 and
+// pst/static-pst-cpu-iter.tpp:488-598); the tree is bit-identical to the
+// reference's CPU iterative tree on the same input.
+#include <algorithm>
+#include <vector>
+
+#include "pst_internal.hpp"
+
+namespace pst {
+
+namespace {
+
+constexpr int kLevelBlock = 256;
+constexpr int kLevelElems = 32;
+constexpr uint32_t kLevelTile = kLevelBlock * kLevelElems;  // 8192 points
+constexpr int kFinishBlock = 256;
+constexpr uint32_t kFinishTile = 2048;  // max points per LDS subtree
+
+template <typename I>
+struct IdStore {
+  using type = typename std::conditional<sizeof(I) == 8, uint64_t, uint32_t>::type;
+};
+template <>
+struct IdStore<NoId> {
+  using type = NoId;
+};
+
+// ---------------------------------------------------------------------------
+// Key kernels. Keys are order-preserving unsigned integers; for the
+// descending key2 order the keys are complemented and the input visited
+// back to front, so that a stable ascending sort also orders equal keys by
+// descending index, as the reference's comparisonTiebreaker does for
+// compareDim2(...) > 0.
+// ---------------------------------------------------------------------------
+template <typename T, typename I, typename K, bool DIM2, bool DESC, bool WITH_ID>
+__global__ void k_keys_single(const Point<T, I>* __restrict__ pts, uint32_t n, K* __restrict__ keys,
+                              uint32_t* __restrict__ vals, const uint32_t* __restrict__ r2,
+                              int* __restrict__ nan_flag) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t i = DESC ? n - 1 - j : j;
+  const Point<T, I> p = pts[i];
+  const T d = DIM2 ? p.dim2_val : p.dim1_val;
+  bool bad = is_nan(d);
+  K k;
+  if constexpr (WITH_ID) {
+    bad |= is_nan(p.id);
+    k = (static_cast<K>(ord_key(d)) << 32) | static_cast<K>(ord_key(p.id));
+  } else {
+    k = static_cast<K>(ord_key(d));
+  }
+  if (bad) *nan_flag = 1;
+  keys[j] = DESC ? ~k : k;
+  vals[j] = r2 ? r2[i] : i;
+}
+
+// Two-pass keys (when dim or id is 8 bytes): pass A orders by id.
+template <typename T, typename I, bool DESC>
+__global__ void k_keys_id(const Point<T, I>* __restrict__ pts, uint32_t n, uint64_t* __restrict__ keys,
+                          uint32_t* __restrict__ vals, int* __restrict__ nan_flag) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t i = DESC ? n - 1 - j : j;
+  const I id = pts[i].id;
+  if (is_nan(id)) *nan_flag = 1;
+  const uint64_t k = static_cast<uint64_t>(ord_key(id));
+  keys[j] = DESC ? ~k : k;
+  vals[j] = i;
+}
+
+// Pass B: orders by dim, visiting points in pass-A order.
+template <typename T, typename I, bool DIM2, bool DESC>
+__global__ void k_keys_dim_perm(const Point<T, I>* __restrict__ pts, uint32_t n,
+                                const uint32_t* __restrict__ perm, uint64_t* __restrict__ keys,
+                                uint32_t* __restrict__ vals, const uint32_t* __restrict__ r2,
+                                int* __restrict__ nan_flag) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t i = perm[j];
+  const T d = DIM2 ? pts[i].dim2_val : pts[i].dim1_val;
+  if (is_nan(d)) *nan_flag = 1;
+  const uint64_t k = static_cast<uint64_t>(ord_key(d));
+  keys[j] = DESC ? ~k : k;
+  vals[j] = r2 ? r2[i] : i;
+}
+
+__global__ void k_scatter_rank(const uint32_t* __restrict__ perm2, uint32_t n, uint32_t* __restrict__ r2) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) r2[perm2[p]] = p;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, size_t count, unsigned long long v) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Level passes over A (all segments of level t are non-empty here).
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t find_segment(const uint32_t* __restrict__ starts, uint32_t nseg, uint32_t pos) {
+  uint32_t lo = 0, hi = nseg;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (starts[mid] <= pos) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Per segment: min over (rank << 32 | local position), i.e. the max-key2
+// point and where it sits in the segment.
+__global__ __launch_bounds__(kLevelBlock) void k_level_argmin(
+    const uint32_t* __restrict__ A, uint32_t n_level, uint32_t nseg,
+    const uint32_t* __restrict__ starts, unsigned long long* __restrict__ segmin) {
+  __shared__ unsigned long long smin[8];
+  __shared__ uint32_t seg0;
+  const uint32_t tile = blockIdx.x * kLevelTile;
+  if (threadIdx.x == 0) seg0 = find_segment(starts, nseg, tile);
+  if (threadIdx.x < 8) smin[threadIdx.x] = ~0ull;
+  __syncthreads();
+  const uint32_t first = tile + threadIdx.x;
+  uint32_t j = seg0;
+  uint32_t next = (j + 1 < nseg) ? starts[j + 1] : n_level;
+  uint32_t st = starts[j];
+  unsigned long long best = ~0ull;
+#pragma unroll 4
+  for (int e = 0; e < kLevelElems; e++) {
+    const uint32_t P = first + e * kLevelBlock;
+    if (P >= n_level) break;
+    while (P >= next) {
+      if (best != ~0ull) atomicMin(&smin[j - seg0], best);
+      best = ~0ull;
+      j++;
+      st = next;
+      next = (j + 1 < nseg) ? starts[j + 1] : n_level;
+    }
+    const unsigned long long key = (static_cast<unsigned long long>(A[P]) << 32) | (P - st);
+    best = key < best ? key : best;
+  }
+  if (best != ~0ull) atomicMin(&smin[j - seg0], best);
+  __syncthreads();
+  if (threadIdx.x < 8 && smin[threadIdx.x] != ~0ull) atomicMin(&segmin[seg0 + threadIdx.x], smin[threadIdx.x]);
+}
+
+template <typename T, typename IS>
+__device__ inline void write_node(KV<T, IS>* kv, DM<T>* dm, uint64_t slot, const Point<T, IS>& p, T med) {
+  KV<T, IS> a;
+  a.dim2 = p.dim2_val;
+  if constexpr (has_id_v<IS>) a.id = p.id;
+  kv[slot] = a;
+  DM<T> b;
+  b.dim1 = p.dim1_val;
+  b.med = med;
+  dm[slot] = b;
+}
+
+// Emits the level's nodes and compacts A into the next level: the point at
+// local position x of segment j (x != q_j) moves to start_j - j + x - [x > q_j].
+template <typename T, typename IS>
+__global__ __launch_bounds__(kLevelBlock) void k_level_emit(
+    const uint32_t* __restrict__ A, uint32_t* __restrict__ Anext, uint32_t n_level, uint32_t t,
+    uint64_t M, const uint32_t* __restrict__ starts, const unsigned long long* __restrict__ segmin,
+    const uint32_t* __restrict__ perm2, const Point<T, IS>* __restrict__ pts, KV<T, IS>* __restrict__ kv,
+    DM<T>* __restrict__ dm) {
+  const uint32_t nseg = 1u << t;
+  __shared__ uint32_t seg0;
+  const uint32_t tile = blockIdx.x * kLevelTile;
+  if (threadIdx.x == 0) seg0 = find_segment(starts, nseg, tile);
+  __syncthreads();
+  const uint32_t first = tile + threadIdx.x;
+  uint32_t j = seg0;
+  uint32_t st = starts[j];
+  uint32_t next = (j + 1 < nseg) ? starts[j + 1] : n_level;
+  unsigned long long mn = segmin[j];
+#pragma unroll 4
+  for (int e = 0; e < kLevelElems; e++) {
+    const uint32_t P = first + e * kLevelBlock;
+    if (P >= n_level) break;
+    while (P >= next) {
+      j++;
+      st = next;
+      next = (j + 1 < nseg) ? starts[j + 1] : n_level;
+      mn = segmin[j];
+    }
+    const uint32_t x = P - st;
+    const uint32_t q = static_cast<uint32_t>(mn);
+    if (x == q) {
+      const uint32_t m = static_cast<uint32_t>(level_size(M, t, j));
+      const uint32_t L = m >> 1;
+      const Point<T, IS> p = pts[perm2[static_cast<uint32_t>(mn >> 32)]];
+      T med = p.dim1_val;
+      if (L >= 1) {
+        const uint32_t pm = q < L ? L : L - 1;
+        med = pts[perm2[A[st + pm]]].dim1_val;
+      }
+      write_node(kv, dm, (1ull << t) - 1 + j, p, med);
+    } else {
+      Anext[st - j + x - (x > q ? 1u : 0u)] = A[P];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Whole subtree in LDS: A-segment of m <= kFinishTile points in bufA.
+// ---------------------------------------------------------------------------
+template <typename T, typename IS, typename PermF>
+__device__ void lds_build(uint32_t* bufA, uint32_t* bufB, unsigned long long* sres,
+                          unsigned long long* wred, uint32_t m, uint64_t root_slot,
+                          const Point<T, IS>* __restrict__ pts, PermF perm, KV<T, IS>* __restrict__ kv,
+                          DM<T>* __restrict__ dm) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nthr = blockDim.x;
+  const uint64_t Mp = static_cast<uint64_t>(m) + 1;
+  for (uint32_t u = 0; (Mp >> u) != 0; u++) {
+    const uint32_t nsub = 1u << u;
+    if (nsub <= nthr) {
+      const uint32_t G = nthr / nsub;
+      const uint32_t k = tid / G, g = tid % G;
+      const int64_t sz = level_size(Mp, u, k);
+      const uint32_t st = static_cast<uint32_t>(sz > 0 ? level_start(Mp, u, k) : 0);
+      unsigned long long best = ~0ull;
+      for (int64_t x = g; x < sz; x += G) {
+        const unsigned long long key = (static_cast<unsigned long long>(bufA[st + x]) << 32) |
+                                       static_cast<unsigned long long>(x);
+        best = key < best ? key : best;
+      }
+      const uint32_t wg = G < kWave ? G : kWave;
+      for (uint32_t off = wg >> 1; off >= 1; off >>= 1) {
+        const unsigned long long o = __shfl_xor(best, off, kWave);
+        best = o < best ? o : best;
+      }
+      if (G > kWave) {  // uniform across the block
+        if ((tid & (kWave - 1)) == 0) wred[tid / kWave] = best;
+        __syncthreads();
+        if (g == 0) {
+          const uint32_t w0 = tid / kWave, nw = G / kWave;
+          for (uint32_t w = 0; w < nw; w++) best = wred[w0 + w] < best ? wred[w0 + w] : best;
+        }
+      }
+      if (g == 0) sres[k] = best;
+      __syncthreads();
+      if (sz > 0) {
+        const unsigned long long mn = sres[k];
+        const uint32_t q = static_cast<uint32_t>(mn);
+        const uint32_t L = static_cast<uint32_t>(sz) >> 1;
+        if (g == 0) {
+          const Point<T, IS> p = pts[perm(static_cast<uint32_t>(mn >> 32))];
+          T med = p.dim1_val;
+          if (L >= 1) med = pts[perm(bufA[st + (q < L ? L : L - 1)])].dim1_val;
+          write_node(kv, dm, (root_slot + 1) * nsub - 1 + k, p, med);
+        }
+        const uint32_t dst0 = static_cast<uint32_t>(sz > 1 ? level_start(Mp, u + 1, 2 * k) : 0);
+        for (int64_t x = g; x < sz; x += G) {
+          if (static_cast<uint32_t>(x) == q) continue;
+          bufB[dst0 + x - (static_cast<uint32_t>(x) > q ? 1 : 0)] = bufA[st + x];
+        }
+      }
+    } else {
+      for (uint32_t k = tid; k < nsub; k += nthr) {
+        const int64_t sz = level_size(Mp, u, k);
+        if (sz <= 0) continue;
+        const uint32_t st = static_cast<uint32_t>(level_start(Mp, u, k));
+        unsigned long long best = ~0ull;
+        for (int64_t x = 0; x < sz; x++) {
+          const unsigned long long key = (static_cast<unsigned long long>(bufA[st + x]) << 32) |
+                                         static_cast<unsigned long long>(x);
+          best = key < best ? key : best;
+        }
+        const uint32_t q = static_cast<uint32_t>(best);
+        const uint32_t L = static_cast<uint32_t>(sz) >> 1;
+        const Point<T, IS> p = pts[perm(static_cast<uint32_t>(best >> 32))];
+        T med = p.dim1_val;
+        if (L >= 1) med = pts[perm(bufA[st + (q < L ? L : L - 1)])].dim1_val;
+        write_node(kv, dm, (root_slot + 1) * nsub - 1 + k, p, med);
+        if (sz > 1) {
+          const uint32_t dst0 = static_cast<uint32_t>(level_start(Mp, u + 1, 2 * k));
+          for (int64_t x = 0; x < sz; x++) {
+            if (static_cast<uint32_t>(x) == q) continue;
+            bufB[dst0 + x - (static_cast<uint32_t>(x) > q ? 1 : 0)] = bufA[st + x];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    uint32_t* tmp = bufA;
+    bufA = bufB;
+    bufB = tmp;
+  }
+}
+
+struct GlobalPerm {
+  const uint32_t* perm2;
+  __device__ uint32_t operator()(uint32_t r) const { return perm2[r]; }
+};
+
+template <typename T, typename IS>
+__global__ __launch_bounds__(kFinishBlock) void k_finish(
+    const uint32_t* __restrict__ A, uint32_t t0, uint64_t M, const uint32_t* __restrict__ starts,
+    const uint32_t* __restrict__ perm2, const Point<T, IS>* __restrict__ pts, KV<T, IS>* __restrict__ kv,
+    DM<T>* __restrict__ dm) {
+  __shared__ uint32_t bufA[kFinishTile];
+  __shared__ uint32_t bufB[kFinishTile];
+  __shared__ unsigned long long sres[kFinishTile];
+  __shared__ unsigned long long wred[kFinishBlock / kWave];
+  const uint32_t j = blockIdx.x;
+  const int64_t m = level_size(M, t0, j);
+  if (m <= 0) return;
+  const uint32_t st = starts[j];
+  for (uint32_t x = threadIdx.x; x < m; x += blockDim.x) bufA[x] = A[st + x];
+  __syncthreads();
+  lds_build<T, IS>(bufA, bufB, sres, wred, static_cast<uint32_t>(m), (1ull << t0) - 1 + j, pts,
+                   GlobalPerm{perm2}, kv, dm);
+}
+
+// Reference-layout export: dim1[S] dim2[S] med[S] ids[S] bitcodes[S].
+template <typename T, typename IS>
+__global__ void k_export(const KV<T, IS>* __restrict__ kv, const DM<T>* __restrict__ dm, uint64_t S,
+                         uint64_t M, unsigned char* __restrict__ blob, size_t o1, size_t o2, size_t om,
+                         size_t oi, size_t ob) {
+  const uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const int64_t m = subtree_size(M, s);
+  T d1 = T(0), d2 = T(0), md = T(0);
+  unsigned char bc = 0;
+  if (m >= 1) {
+    d1 = dm[s].dim1;
+    md = dm[s].med;
+    d2 = kv[s].dim2;
+    bc = static_cast<unsigned char>((m >= 2 ? 2 : 0) | (m >= 3 ? 1 : 0));
+  }
+  reinterpret_cast<T*>(blob + o1)[s] = d1;
+  reinterpret_cast<T*>(blob + o2)[s] = d2;
+  reinterpret_cast<T*>(blob + om)[s] = md;
+  if constexpr (has_id_v<IS>) {
+    IS id = 0;
+    if (m >= 1) id = kv[s].id;
+    reinterpret_cast<IS*>(blob + oi)[s] = id;
+  }
+  blob[ob + s] = bc;
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
+  template <typename U>
+  U* as() const {
+    return static_cast<U*>(p);
+  }
+};
+
+inline unsigned grid_for(size_t n, unsigned block) { return static_cast<unsigned>((n + block - 1) / block); }
+
+template <typename T, typename I>
+Status build_typed(const void* pts_v, size_t n64, void* kv_v, void* dm_v, hipStream_t s, BuildTimes* times) {
+  using IS = typename IdStore<I>::type;
+  Status st;
+  if (n64 == 0) return st;
+  if (n64 >= (1ull << 32) - 1) {
+    st.code = 1;
+    st.msg = "Error: StaticPSTGPU supports fewer than 2^32 - 1 points";
+    return st;
+  }
+  const uint32_t n = static_cast<uint32_t>(n64);
+  const uint64_t M = n64 + 1;
+  const auto* pts = static_cast<const Point<T, I>*>(pts_v);
+  const auto* ptsS = static_cast<const Point<T, IS>*>(pts_v);
+  auto* kv = static_cast<KV<T, IS>*>(kv_v);
+  auto* dm = static_cast<DM<T>*>(dm_v);
+  constexpr bool kHasId = has_id_v<I>;
+  constexpr bool kSingle = !kHasId ? true : (sizeof(T) == 4 && sizeof(I) == 4);
+  using K = typename std::conditional<(!kHasId && sizeof(T) == 4), uint32_t, uint64_t>::type;
+
+  hipEvent_t ev[5] = {};
+  if (times)
+    for (auto& e : ev) hipEventCreate(&e);
+  auto mark = [&](int k) {
+    if (times) hipEventRecord(ev[k], s);
+  };
+
+  DevBuf kbuf[2], vbuf[2], perm2b, r2b, flag, tmpb;
+  hipError_t e;
+#define PST_TRY(expr, what)                         \
+  do {                                              \
+    e = (expr);                                     \
+    if (e != hipSuccess) return hip_status(e, what); \
+  } while (0)
+  PST_TRY(kbuf[0].alloc(n64 * sizeof(K)), "Error in allocating construction key array");
+  PST_TRY(kbuf[1].alloc(n64 * sizeof(K)), "Error in allocating construction key array");
+  PST_TRY(vbuf[0].alloc(n64 * 4), "Error in allocating construction index array");
+  PST_TRY(vbuf[1].alloc(n64 * 4), "Error in allocating construction index array");
+  PST_TRY(perm2b.alloc(n64 * 4), "Error in allocating dimension-2 order array");
+  PST_TRY(r2b.alloc(n64 * 4), "Error in allocating dimension-2 rank array");
+  PST_TRY(flag.alloc(sizeof(int)), "Error in allocating construction flag");
+  PST_TRY(hipMemsetAsync(flag.p, 0, sizeof(int), s), "Error in clearing construction flag");
+
+  size_t tmp_bytes = 0;
+  if constexpr (sizeof(K) == 4)
+    PST_TRY(radix_sort_u32(nullptr, tmp_bytes, nullptr, nullptr, nullptr, nullptr, n, s),
+            "Error in sizing radix sort storage");
+  else
+    PST_TRY(radix_sort_u64(nullptr, tmp_bytes, nullptr, nullptr, nullptr, nullptr, n, s),
+            "Error in sizing radix sort storage");
+  PST_TRY(tmpb.alloc(tmp_bytes), "Error in allocating radix sort storage");
+  auto sortK = [&](int from) -> hipError_t {
+    size_t tb = tmp_bytes;
+    if constexpr (sizeof(K) == 4)
+      return radix_sort_u32(tmpb.p, tb, kbuf[from].as<uint32_t>(), kbuf[1 - from].as<uint32_t>(),
+                            vbuf[from].as<uint32_t>(), vbuf[1 - from].as<uint32_t>(), n, s);
+    else
+      return radix_sort_u64(tmpb.p, tb, kbuf[from].as<uint64_t>(), kbuf[1 - from].as<uint64_t>(),
+                            vbuf[from].as<uint32_t>(), vbuf[1 - from].as<uint32_t>(), n, s);
+  };
+  const unsigned B = 256;
+  const unsigned G = grid_for(n, B);
+  int* nan_flag = flag.as<int>();
+  mark(0);
+  // ---- key2 descending -> perm2, r2
+  if constexpr (kSingle) {
+    k_keys_single<T, I, K, true, true, kHasId>
+        <<<G, B, 0, s>>>(pts, n, kbuf[0].as<K>(), vbuf[0].as<uint32_t>(), nullptr, nan_flag);
+    PST_TRY(sortK(0), "Error in sorting by dimension 2");
+    PST_TRY(hipMemcpyAsync(perm2b.p, vbuf[1].p, n64 * 4, hipMemcpyDeviceToDevice, s),
+            "Error in copying dimension-2 order");
+  } else {
+    if constexpr (kHasId) {
+      k_keys_id<T, I, true><<<G, B, 0, s>>>(pts, n, kbuf[0].as<uint64_t>(), vbuf[0].as<uint32_t>(), nan_flag);
+      PST_TRY(sortK(0), "Error in sorting by ID");
+      k_keys_dim_perm<T, I, true, true><<<G, B, 0, s>>>(pts, n, vbuf[1].as<uint32_t>(), kbuf[0].as<uint64_t>(),
+                                                       vbuf[0].as<uint32_t>(), nullptr, nan_flag);
+      PST_TRY(sortK(0), "Error in sorting by dimension 2");
+      PST_TRY(hipMemcpyAsync(perm2b.p, vbuf[1].p, n64 * 4, hipMemcpyDeviceToDevice, s),
+              "Error in copying dimension-2 order");
+    }
+  }
+  k_scatter_rank<<<G, B, 0, s>>>(perm2b.as<uint32_t>(), n, r2b.as<uint32_t>());
+  // ---- key1 ascending, carrying r2 -> A
+  if constexpr (kSingle) {
+    k_keys_single<T, I, K, false, false, kHasId><<<G, B, 0, s>>>(pts, n, kbuf[0].as<K>(), vbuf[0].as<uint32_t>(),
+                                                               r2b.as<uint32_t>(), nan_flag);
+    PST_TRY(sortK(0), "Error in sorting by dimension 1");
+  } else {
+    if constexpr (kHasId) {
+      k_keys_id<T, I, false><<<G, B, 0, s>>>(pts, n, kbuf[0].as<uint64_t>(), vbuf[0].as<uint32_t>(), nan_flag);
+      PST_TRY(sortK(0), "Error in sorting by ID");
+      k_keys_dim_perm<T, I, false, false><<<G, B, 0, s>>>(pts, n, vbuf[1].as<uint32_t>(), kbuf[0].as<uint64_t>(),
+                                                         vbuf[0].as<uint32_t>(), r2b.as<uint32_t>(), nan_flag);
+      PST_TRY(sortK(0), "Error in sorting by dimension 1");
+    }
+  }
+  mark(1);
+  // A now in vbuf[1]; vbuf[0] is free for the next level.
+  uint32_t* Acur = vbuf[1].as<uint32_t>();
+  uint32_t* Anxt = vbuf[0].as<uint32_t>();
+
+  // ---- level schedule
+  uint32_t t0 = 0;
+  while (true) {
+    const uint64_t base = M >> t0;
+    const uint64_t r = M & ((1ull << t0) - 1ull);
+    const uint64_t mx = base - 1 + (r ? 1 : 0);
+    if (mx <= kFinishTile) break;
+    t0++;
+  }
+  // segment starts of levels 0..t0, slot-indexed (level t at offset 2^t - 1)
+  const size_t nstart = (1ull << (t0 + 1)) - 1;
+  std::vector<uint32_t> hstarts(nstart);
+  for (uint32_t t = 0; t <= t0; t++)
+    for (uint32_t j = 0; j < (1u << t); j++)
+      hstarts[(1u << t) - 1 + j] = static_cast<uint32_t>(std::max<int64_t>(0, level_start(M, t, j)));
+  DevBuf startsb, segminb;
+  PST_TRY(startsb.alloc(nstart * 4), "Error in allocating level tables");
+  PST_TRY(hipMemcpyAsync(startsb.p, hstarts.data(), nstart * 4, hipMemcpyHostToDevice, s),
+          "Error in copying level tables");
+  const size_t nmin = t0 ? (1ull << t0) - 1 : 1;
+  PST_TRY(segminb.alloc(nmin * 8), "Error in allocating level minima");
+  k_fill_u64<<<grid_for(nmin, B), B, 0, s>>>(segminb.as<unsigned long long>(), nmin, ~0ull);
+  const uint32_t* starts = startsb.as<uint32_t>();
+  unsigned long long* segmin = segminb.as<unsigned long long>();
+  for (uint32_t t = 0; t < t0; t++) {
+    const uint32_t n_level = n - ((1u << t) - 1);
+    const unsigned tiles = grid_for(n_level, kLevelTile);
+    k_level_argmin<<<tiles, kLevelBlock, 0, s>>>(Acur, n_level, 1u << t, starts + (1u << t) - 1,
+                                                 segmin + (1u << t) - 1);
+    k_level_emit<T, IS><<<tiles, kLevelBlock, 0, s>>>(Acur, Anxt, n_level, t, M, starts + (1u << t) - 1,
+                                                      segmin + (1u << t) - 1, perm2b.as<uint32_t>(), ptsS,
+                                                      kv, dm);
+    std::swap(Acur, Anxt);
+  }
+  mark(2);
+  k_finish<T, IS><<<1u << t0, kFinishBlock, 0, s>>>(Acur, t0, M, starts + (1u << t0) - 1,
+                                                    perm2b.as<uint32_t>(), ptsS, kv, dm);
+  mark(3);
+  PST_TRY(hipGetLastError(), "Error in launching construction kernels");
+  int hflag = 0;
+  PST_TRY(hipMemcpyAsync(&hflag, nan_flag, sizeof(int), hipMemcpyDeviceToHost, s),
+          "Error in reading construction flag");
+  PST_TRY(hipStreamSynchronize(s), "Error in synchronising after construction");
+  if (times) {
+    hipEventElapsedTime(&times->sort_ms, ev[0], ev[1]);
+    hipEventElapsedTime(&times->levels_ms, ev[1], ev[2]);
+    hipEventElapsedTime(&times->finish_ms, ev[2], ev[3]);
+    for (auto& x : ev) hipEventDestroy(x);
+  }
+  if (hflag) {
+    st.code = 3;
+    st.msg = "Error: input contains NaN values, which have no order";
+  }
+#undef PST_TRY
+  return st;
+}
+
+template <typename T, typename IS>
+Status export_typed(size_t n, const void* kv, const void* dm, void* blob, size_t o1, size_t o2, size_t om,
+                    size_t oi, size_t ob, hipStream_t s) {
+  if (n == 0) return Status{};
+  uint64_t S = 1;
+  while (S <= n) S <<= 1;
+  S -= 1;
+  k_export<T, IS><<<grid_for(S, 256), 256, 0, s>>>(static_cast<const KV<T, IS>*>(kv),
+                                                    static_cast<const DM<T>*>(dm), S, n + 1,
+                                                    static_cast<unsigned char*>(blob), o1, o2, om, oi, ob);
+  return hip_status(hipGetLastError(), "Error in launching tree export");
+}
+
+template <typename F>
+Status with_t(int dt, F&& f) {
+  switch (dt) {
+    case F32: return f(float{});
+    case F64: return f(double{});
+    case I32: return f(int32_t{});
+    case I64: return f(int64_t{});
+  }
+  return Status{1, "Error: unsupported data type code " + std::to_string(dt)};
+}
+template <typename F>
+Status with_id(int idt, F&& f) {
+  switch (idt) {
+    case NO_ID: return f(NoId{});
+    case F32: return f(float{});
+    case F64: return f(double{});
+    case I32: return f(int32_t{});
+    case I64: return f(int64_t{});
+    case U32: return f(uint32_t{});
+    case U64: return f(uint64_t{});
+  }
+  return Status{1, "Error: unsupported ID type code " + std::to_string(idt)};
+}
+
+}  // namespace
+
+Status build_tree(int dtype, int id_dtype, const void* pts_d, size_t n, void* kv_d, void* dm_d,
+                  hipStream_t stream, BuildTimes* times) {
+  return with_t(dtype, [&](auto t) {
+    return with_id(id_dtype, [&](auto i) {
+      return build_typed<decltype(t), decltype(i)>(pts_d, n, kv_d, dm_d, stream, times);
+    });
+  });
+}
+
+Status export_tree(int dtype, int id_dtype, size_t n, const void* kv_d, const void* dm_d, void* blob_d,
+                   size_t o1, size_t o2, size_t om, size_t oi, size_t ob, hipStream_t stream) {
+  return with_t(dtype, [&](auto t) {
+    return with_id(id_dtype, [&](auto i) {
+      using IS = typename IdStore<decltype(i)>::type;
+      return export_typed<decltype(t), IS>(n, kv_d, dm_d, blob_d, o1, o2, om, oi, ob, stream);
+    });
+  });
+}
+
+}  // namespace pst

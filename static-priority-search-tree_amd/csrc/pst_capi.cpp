@@ -1,0 +1,478 @@
+// extern "C" entry points of libpst_hip.so (declared in include/pst/pst_capi.h).
+#include "pst_capi.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "pst_internal.hpp"
+
+using pst::Status;
+
+struct pst_tree {
+  int dtype = 0, id_dtype = -1;
+  size_t n = 0, S = 0;
+  int dev_ind = 0, num_devs = 1, device = 0;
+  void* kv = nullptr;
+  void* dm = nullptr;
+  pst::SearchScratch sc;
+  int rounds = 4, budget_small = 0, budget_large = 0, waves_per_cu = 0;
+  hipStream_t stream = nullptr;
+  unsigned long long* debug = nullptr;  // per-round search counters (debug builds)
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const Status& s) {
+  g_err = s.msg;
+  return s.code ? s.code : PST_ERR_INTERNAL;
+}
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+std::string dev_suffix(int dev_ind, int num_devs) {
+  return " on device " + std::to_string(dev_ind + 1) + " (1-indexed) of " + std::to_string(num_devs) + ": ";
+}
+
+bool valid_t(int d) { return d >= PST_F32 && d <= PST_I64; }
+bool valid_id(int d) { return d == PST_NO_ID || (d >= PST_F32 && d <= PST_U64); }
+
+size_t id_size(int idt) {
+  switch (idt) {
+    case PST_NO_ID: return 0;
+    case PST_F32: case PST_I32: case PST_U32: return 4;
+    default: return 8;
+  }
+}
+
+size_t slots_for(size_t n) {
+  if (n == 0) return 0;
+  size_t p = 1;
+  while (p <= n) p <<= 1;
+  return p - 1;
+}
+
+// Per-device counter used by the blocking interval search.
+std::mutex g_ctr_mu;
+std::map<int, unsigned long long*> g_ctr;
+unsigned long long* device_counter(int dev) {
+  std::lock_guard<std::mutex> lk(g_ctr_mu);
+  auto it = g_ctr.find(dev);
+  if (it != g_ctr.end()) return it->second;
+  unsigned long long* p = nullptr;
+  if (hipMalloc(&p, sizeof(unsigned long long)) != hipSuccess) return nullptr;
+  g_ctr[dev] = p;
+  return p;
+}
+
+template <typename T>
+void make_query(void* dst, uint32_t kind, const void* a, const void* b, const void* c) {
+  pst::Query<T> q{};
+  if (a) std::memcpy(&q.min_dim1, a, sizeof(T));
+  if (b) std::memcpy(&q.max_dim1, b, sizeof(T));
+  if (c) std::memcpy(&q.min_dim2, c, sizeof(T));
+  q.kind = kind;
+  std::memcpy(dst, &q, sizeof(q));
+}
+
+void fill_query(int dt, void* dst, uint32_t kind, const void* a, const void* b, const void* c) {
+  switch (dt) {
+    case PST_F32: make_query<float>(dst, kind, a, b, c); break;
+    case PST_F64: make_query<double>(dst, kind, a, b, c); break;
+    case PST_I32: make_query<int32_t>(dst, kind, a, b, c); break;
+    default: make_query<int64_t>(dst, kind, a, b, c); break;
+  }
+}
+
+pst::SearchArgs base_args(pst_tree* t) {
+  pst::SearchArgs a{};
+  a.dtype = t->dtype;
+  a.id_dtype = t->id_dtype;
+  a.n = t->n;
+  a.kv = t->kv;
+  a.dm = t->dm;
+  a.rounds = t->rounds;
+  a.budget_small = t->budget_small;
+  a.budget_large = t->budget_large;
+  a.waves_per_cu = t->waves_per_cu;
+  a.debug = t->debug;
+  return a;
+}
+
+// One query, results written to out (capacity cap), blocking; *num = count.
+int single_search(pst_tree* t, int kind, int ret, const void* a, const void* b, const void* c, void* out,
+                  size_t cap, size_t* num, hipStream_t s) {
+  alignas(16) unsigned char qrec[32];
+  fill_query(t->dtype, qrec, static_cast<uint32_t>(kind), a, b, c);
+  pst::SearchArgs args = base_args(t);
+  args.single_query = qrec;
+  args.nq = 1;
+  args.mode = 1;
+  args.ret_point = ret;
+  Status st = pst::scratch_reserve(t->sc, 1, t->dtype, s);
+  if (!st.ok()) return fail(st);
+  args.counts = t->sc.cursor;
+  args.cap_all = cap;
+  args.out = out;
+  st = pst::run_search(args, t->sc, s);
+  if (!st.ok()) return fail(st);
+  unsigned long long host[2] = {0, 0};
+  hipError_t e = hipMemcpyAsync(&host[0], t->sc.cursor, 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&host[1], pst::search_error_flag(t->sc), 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess)
+    return fail(PST_ERR_HIP, "Error in copying global result array final index from device" +
+                                 dev_suffix(t->dev_ind, t->num_devs) + hipGetErrorString(e));
+  if (host[1] & 1ull) return fail(PST_ERR_INTERNAL, "Error: search stack overflow");
+  if (host[1] & 2ull) return fail(PST_ERR_INTERNAL, "Error: search read an invalid work item");
+  *num = host[0];
+  if (host[0] > cap) return fail(PST_ERR_CAPACITY, "Error: search results exceed the output capacity");
+  return PST_OK;
+}
+
+int blocking_search(pst_tree* t, int kind, int ret, size_t* num, void** res, const void* a, const void* b,
+                    const void* c) {
+  if (!t || !num || !res) return fail(PST_ERR_INVALID, "Error: null argument");
+  if (ret != PST_RET_POINT && t->id_dtype == PST_NO_ID)
+    return fail(PST_ERR_INVALID, "Error: IDs requested from a tree built without IDs");
+  if (t->n == 0) {
+    std::printf("Tree is empty; nothing to search\n");
+    *num = 0;
+    *res = nullptr;
+    return PST_OK;
+  }
+  hipSetDevice(t->device);
+  const size_t rec = ret == PST_RET_POINT ? pst_point_size(t->dtype, t->id_dtype) : id_size(t->id_dtype);
+  void* out = nullptr;
+  hipError_t e = hipMalloc(&out, t->n * rec);
+  if (e != hipSuccess)
+    return fail(PST_ERR_HIP, "Error in allocating array to store PointStruct search result" +
+                                 dev_suffix(t->dev_ind, t->num_devs) + hipGetErrorString(e));
+  int rc = single_search(t, kind, ret, a, b, c, out, t->n, num, t->stream);
+  if (rc != PST_OK) {
+    hipFree(out);
+    *res = nullptr;
+    return rc;
+  }
+  *res = out;
+  return PST_OK;
+}
+
+}  // namespace
+
+namespace pst {
+
+Status hip_status(hipError_t e, const std::string& what) {
+  Status s;
+  if (e != hipSuccess) {
+    int dev = 0;
+    hipGetDevice(&dev);
+    s.code = PST_ERR_HIP;
+    s.msg = what + dev_suffix(dev, 1) + hipGetErrorString(e);
+  }
+  return s;
+}
+
+size_t dtype_size(int dt) { return (dt == PST_F64 || dt == PST_I64 || dt == PST_U64) ? 8 : 4; }
+
+size_t kv_elem_size(int dt, int idt) {
+  const size_t a = dtype_size(dt), b = id_size(idt);
+  if (b == 0) return a;
+  const size_t al = a > b ? a : b;
+  size_t off = ((a + b - 1) / b) * b;
+  return ((off + b + al - 1) / al) * al;
+}
+size_t dm_elem_size(int dt) { return 2 * dtype_size(dt); }
+
+size_t point_size(int dt, int idt) {
+  const size_t a = dtype_size(dt), b = id_size(idt);
+  if (b == 0) return 2 * a;
+  const size_t al = a > b ? a : b;
+  const size_t off = ((2 * a + b - 1) / b) * b;
+  return ((off + b + al - 1) / al) * al;
+}
+
+}  // namespace pst
+
+extern "C" {
+
+const char* pst_last_error(void) { return g_err.c_str(); }
+
+size_t pst_point_size(int dtype, int id_dtype) {
+  if (!valid_t(dtype) || !valid_id(id_dtype)) return 0;
+  return pst::point_size(dtype, id_dtype);
+}
+
+int pst_free(void* p) {
+  if (!p) return PST_OK;
+  hipError_t e = hipFree(p);
+  if (e != hipSuccess) return fail(pst::hip_status(e, "Error in freeing search result array"));
+  return PST_OK;
+}
+
+int pst_gpu_create(const void* pt_arr_d, size_t n, int dtype, int id_dtype, int warp_multiplier, int dev_ind,
+                   int num_devs, pst_stream_t stream, pst_tree** out) {
+  (void)warp_multiplier;
+  if (!out) return fail(PST_ERR_INVALID, "Error: null output handle");
+  *out = nullptr;
+  if (!valid_t(dtype) || !valid_id(id_dtype))
+    return fail(PST_ERR_INVALID, "Error: unsupported data/ID type codes " + std::to_string(dtype) + "/" +
+                                     std::to_string(id_dtype));
+  if (n > 0 && !pt_arr_d) return fail(PST_ERR_INVALID, "Error: null point array");
+  // search work items pack a slot into 30 bits: S = 2^ceil(log2(n+1)) - 1 < 2^30
+  if (n >= (1ull << 29)) return fail(PST_ERR_INVALID, "Error: StaticPSTGPU holds fewer than 2^29 points");
+  pst_tree* t = new pst_tree();
+  t->dtype = dtype;
+  t->id_dtype = id_dtype;
+  t->n = n;
+  t->S = slots_for(n);
+  t->dev_ind = dev_ind;
+  t->num_devs = num_devs;
+  t->stream = reinterpret_cast<hipStream_t>(stream);
+  hipGetDevice(&t->device);
+  if (n == 0) {
+    *out = t;
+    return PST_OK;
+  }
+  hipError_t e = hipMalloc(&t->kv, t->S * pst::kv_elem_size(dtype, id_dtype));
+  if (e == hipSuccess) e = hipMalloc(&t->dm, t->S * pst::dm_elem_size(dtype));
+  if (e != hipSuccess) {
+    std::string m = "Error in allocating priority search tree storage array" + dev_suffix(dev_ind, num_devs) +
+                    hipGetErrorString(e);
+    pst_gpu_destroy(t);
+    return fail(PST_ERR_HIP, m);
+  }
+  Status st = pst::build_tree(dtype, id_dtype, pt_arr_d, n, t->kv, t->dm, t->stream, nullptr);
+  if (!st.ok()) {
+    pst_gpu_destroy(t);
+    return fail(st);
+  }
+  *out = t;
+  return PST_OK;
+}
+
+int pst_gpu_destroy(pst_tree* t) {
+  if (!t) return PST_OK;
+  hipError_t e = hipSuccess;
+  if (t->kv) e = hipFree(t->kv);
+  if (t->dm) {
+    hipError_t e2 = hipFree(t->dm);
+    if (e == hipSuccess) e = e2;
+  }
+  pst::scratch_release(t->sc);
+  const int di = t->dev_ind, nd = t->num_devs;
+  delete t;
+  if (e != hipSuccess)
+    return fail(PST_ERR_HIP, "Error in freeing array storing PST" + dev_suffix(di, nd) + hipGetErrorString(e));
+  return PST_OK;
+}
+
+size_t pst_gpu_num_elems(const pst_tree* t) { return t ? t->n : 0; }
+int pst_gpu_dev_ind(const pst_tree* t) { return t ? t->dev_ind : 0; }
+int pst_gpu_num_devs(const pst_tree* t) { return t ? t->num_devs : 0; }
+
+int pst_gpu_three_sided_search(pst_tree* t, int ret, size_t* num, void** res, const void* min1, const void* max1,
+                               const void* min2) {
+  return blocking_search(t, PST_THREE_SIDED, ret, num, res, min1, max1, min2);
+}
+int pst_gpu_two_sided_left_search(pst_tree* t, int ret, size_t* num, void** res, const void* max1,
+                                  const void* min2) {
+  return blocking_search(t, PST_TWO_SIDED_LEFT, ret, num, res, nullptr, max1, min2);
+}
+int pst_gpu_two_sided_right_search(pst_tree* t, int ret, size_t* num, void** res, const void* min1,
+                                   const void* min2) {
+  return blocking_search(t, PST_TWO_SIDED_RIGHT, ret, num, res, min1, nullptr, min2);
+}
+
+int pst_gpu_search_into(pst_tree* t, int kind, int ret, const void* min1, const void* max1, const void* min2,
+                        void* out_d, size_t capacity, size_t* num, pst_stream_t stream) {
+  if (!t || !num) return fail(PST_ERR_INVALID, "Error: null argument");
+  if (kind < 0 || kind > 2) return fail(PST_ERR_INVALID, "Error: unknown query kind");
+  if (ret != PST_RET_POINT && t->id_dtype == PST_NO_ID)
+    return fail(PST_ERR_INVALID, "Error: IDs requested from a tree built without IDs");
+  if (t->n == 0) {
+    *num = 0;
+    return PST_OK;
+  }
+  return single_search(t, kind, ret, min1, max1, min2, out_d, capacity, num,
+                       reinterpret_cast<hipStream_t>(stream));
+}
+
+size_t pst_gpu_calc_global_mem_needed(size_t n, int dtype, int id_dtype) {
+  if (!valid_t(dtype) || !valid_id(id_dtype)) return 0;
+  const size_t S = slots_for(n);
+  const size_t tree = S * (pst::kv_elem_size(dtype, id_dtype) + pst::dm_elem_size(dtype));
+  // construction: two key arrays, two 32-bit value arrays, order and rank
+  // arrays, and radix-sort storage (about one more key array)
+  const size_t key = (pst::dtype_size(dtype) == 4 && id_size(id_dtype) == 0) ? 4 : 8;
+  const size_t construct = n * (3 * key + 4 * 4);
+  const size_t search = n * pst::point_size(dtype, id_dtype) + (2ull << 22) * 8;
+  return tree + (construct > search ? construct : search);
+}
+
+size_t pst_gpu_export_bytes(const pst_tree* t) {
+  if (!t) return 0;
+  const size_t sT = pst::dtype_size(t->dtype), sI = id_size(t->id_dtype);
+  size_t after = 3 * t->S * sT;
+  size_t off_id = sI ? ((after + sI - 1) / sI) * sI : after;
+  size_t off_bits = off_id + t->S * sI;
+  return ((off_bits + t->S + 7) / 8) * 8;
+}
+
+int pst_gpu_export_tree(pst_tree* t, void* host_blob, size_t blob_bytes, size_t* off6) {
+  if (!t || !off6) return fail(PST_ERR_INVALID, "Error: null argument");
+  const size_t sT = pst::dtype_size(t->dtype), sI = id_size(t->id_dtype);
+  const size_t S = t->S;
+  off6[0] = 0;
+  off6[1] = S * sT;
+  off6[2] = 2 * S * sT;
+  const size_t after = 3 * S * sT;
+  off6[3] = sI ? ((after + sI - 1) / sI) * sI : after;
+  off6[4] = off6[3] + S * sI;
+  off6[5] = S;
+  const size_t need = pst_gpu_export_bytes(t);
+  if (blob_bytes < need) return fail(PST_ERR_INVALID, "Error: export buffer too small");
+  if (t->n == 0) return PST_OK;
+  void* d = nullptr;
+  hipError_t e = hipMalloc(&d, need);
+  if (e != hipSuccess) return fail(pst::hip_status(e, "Error in allocating tree export buffer"));
+  Status st = pst::export_tree(t->dtype, t->id_dtype, t->n, t->kv, t->dm, d, off6[0], off6[1], off6[2], off6[3],
+                               off6[4], t->stream);
+  if (st.ok()) st = pst::hip_status(hipMemcpyAsync(host_blob, d, need, hipMemcpyDeviceToHost, t->stream),
+                                    "Error in copying tree export");
+  if (st.ok()) st = pst::hip_status(hipStreamSynchronize(t->stream), "Error in synchronising tree export");
+  hipFree(d);
+  if (!st.ok()) return fail(st);
+  return PST_OK;
+}
+
+int pst_gpu_batch_count(pst_tree* t, const void* queries_d, size_t nq, unsigned long long* counts_d,
+                        pst_stream_t stream) {
+  if (!t) return fail(PST_ERR_INVALID, "Error: null tree");
+  if (nq == 0) return PST_OK;
+  if (nq >= (1ull << 24)) return fail(PST_ERR_INVALID, "Error: batches hold fewer than 2^24 queries");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (t->n == 0) {
+    Status st = pst::hip_status(hipMemsetAsync(counts_d, 0, nq * 8, s), "Error in clearing counts");
+    return st.ok() ? PST_OK : fail(st);
+  }
+  pst::SearchArgs a = base_args(t);
+  a.queries = queries_d;
+  a.nq = nq;
+  a.mode = 0;
+  a.ret_point = 0;
+  a.counts = counts_d;
+  Status st = pst::run_search(a, t->sc, s);
+  return st.ok() ? PST_OK : fail(st);
+}
+
+int pst_gpu_batch_write(pst_tree* t, const void* queries_d, size_t nq, int ret, const unsigned long long* offsets_d,
+                        const unsigned long long* caps_d, void* out_d, pst_stream_t stream) {
+  if (!t) return fail(PST_ERR_INVALID, "Error: null tree");
+  if (nq == 0 || t->n == 0) return PST_OK;
+  if (nq >= (1ull << 24)) return fail(PST_ERR_INVALID, "Error: batches hold fewer than 2^24 queries");
+  if (ret != PST_RET_POINT && t->id_dtype == PST_NO_ID)
+    return fail(PST_ERR_INVALID, "Error: IDs requested from a tree built without IDs");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  Status st = pst::scratch_reserve(t->sc, nq, t->dtype, s);
+  if (!st.ok()) return fail(st);
+  pst::SearchArgs a = base_args(t);
+  a.queries = queries_d;
+  a.nq = nq;
+  a.mode = 1;
+  a.ret_point = ret;
+  a.counts = t->sc.cursor;
+  a.offsets = offsets_d;
+  a.caps = caps_d;
+  a.cap_all = ~0ull;
+  a.out = out_d;
+  st = pst::run_search(a, t->sc, s);
+  return st.ok() ? PST_OK : fail(st);
+}
+
+int pst_gpu_set_search_tuning(pst_tree* t, int rounds, int budget_small, int budget_large, int waves_per_cu) {
+  if (!t) return fail(PST_ERR_INVALID, "Error: null tree");
+  if (rounds > 0) t->rounds = rounds;
+  if (budget_small > 0) t->budget_small = budget_small;
+  if (budget_large > 0) t->budget_large = budget_large;
+  if (waves_per_cu >= 0) t->waves_per_cu = waves_per_cu;
+  return PST_OK;
+}
+
+int pst_interval_parallel_search(const void* pt_arr_d, size_t n, int dtype, int id_dtype, int ret, void** res,
+                                 size_t* num, const void* search_val, int dev_ind, int num_devs, int warp_size,
+                                 unsigned num_thread_blocks, unsigned threads_per_block) {
+  (void)warp_size;
+  (void)threads_per_block;
+  if (!res || !num || !search_val) return fail(PST_ERR_INVALID, "Error: null argument");
+  if (!valid_t(dtype) || !valid_id(id_dtype)) return fail(PST_ERR_INVALID, "Error: unsupported type codes");
+  if (ret != PST_RET_POINT && id_dtype == PST_NO_ID)
+    return fail(PST_ERR_INVALID, "Error: IDs requested from points without IDs");
+  const size_t rec = ret == PST_RET_POINT ? pst::point_size(dtype, id_dtype) : id_size(id_dtype);
+  void* out = nullptr;
+  hipError_t e = hipMalloc(&out, (n ? n : 1) * rec);
+  if (e != hipSuccess)
+    return fail(PST_ERR_HIP, "Error in allocating array to store PointStruct search result" +
+                                 dev_suffix(dev_ind, num_devs) + hipGetErrorString(e));
+  int dev = 0;
+  hipGetDevice(&dev);
+  unsigned long long* ctr = device_counter(dev);
+  if (!ctr) {
+    hipFree(out);
+    return fail(PST_ERR_HIP, "Error in allocating global result array index" + dev_suffix(dev_ind, num_devs));
+  }
+  e = hipMemsetAsync(ctr, 0, 8, nullptr);
+  if (e != hipSuccess) {
+    hipFree(out);
+    return fail(PST_ERR_HIP, "Error in initialising global result array index to 0" +
+                                 dev_suffix(dev_ind, num_devs) + hipGetErrorString(e));
+  }
+  Status st = pst::ips_search(dtype, id_dtype, pt_arr_d, n, ret, search_val, out, n, ctr, num_thread_blocks, nullptr);
+  unsigned long long h = 0;
+  if (st.ok()) st = pst::hip_status(hipMemcpy(&h, ctr, 8, hipMemcpyDeviceToHost),
+                                    "Error in copying global result array final index from device");
+  if (!st.ok()) {
+    hipFree(out);
+    return fail(st);
+  }
+  *num = h;
+  *res = out;
+  return PST_OK;
+}
+
+int pst_interval_parallel_search_into(const void* pt_arr_d, size_t n, int dtype, int id_dtype, int ret,
+                                      const void* search_val, void* out_d, size_t capacity,
+                                      unsigned long long* counter_d, pst_stream_t stream) {
+  if (!valid_t(dtype) || !valid_id(id_dtype) || !counter_d || !search_val)
+    return fail(PST_ERR_INVALID, "Error: invalid argument");
+  if (ret != PST_RET_POINT && id_dtype == PST_NO_ID)
+    return fail(PST_ERR_INVALID, "Error: IDs requested from points without IDs");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  Status st = pst::hip_status(hipMemsetAsync(counter_d, 0, 8, s), "Error in clearing the result counter");
+  if (st.ok()) st = pst::ips_search(dtype, id_dtype, pt_arr_d, n, ret, search_val, out_d, capacity, counter_d, 0, s);
+  return st.ok() ? PST_OK : fail(st);
+}
+
+#ifdef PST_DEBUG_EXPORTS
+// Debug builds only: per-round counters of the search launches.
+int pst_debug_set_counters(pst_tree* t, unsigned long long* dbg) {
+  t->debug = dbg;
+  return PST_OK;
+}
+int pst_debug_get_round_counters(pst_tree* t, unsigned long long* host20) {
+  if (!t->sc.ctr) return PST_ERR_INVALID;
+  return hipMemcpy(host20, t->sc.ctr, 20 * sizeof(unsigned long long), hipMemcpyDeviceToHost) == hipSuccess
+             ? PST_OK
+             : PST_ERR_HIP;
+}
+#endif
+
+}  // extern "C"

@@ -1,0 +1,191 @@
+// Shared definitions for the MI355X (gfx950) static priority search tree.
+//
+// Device-side tree layout (one tree, S = 2^ceil(log2(n+1)) - 1 slots):
+//   kv[S] : { T dim2; IDType id; }   prune key + payload, read by every visit
+//   dm[S] : { T dim1; T med;   }     read only on the O(log n) search paths
+//                                    and when whole points are returned
+// Children of slot s sit at 2s+1 and 2s+2 (pst/gpu-tree-node.h:11-13 of the
+// reference). No bitcode array is stored: the tree shape depends on n only
+// (a node holding m points gives floor(m/2) to its left child and
+// ceil(m/2)-1 to its right child, pst/static-pst-cpu-iter.tpp:512-545), so a
+// node's point count, and hence which children exist, is computed in closed
+// form (subtree_size below). The reference-layout blob with bitcodes is
+// produced on request by the export entry point.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <type_traits>
+
+namespace pst {
+
+// dtype codes of the C-ABI (include/pst/pst_capi.h)
+enum DType : int { F32 = 0, F64 = 1, I32 = 2, I64 = 3, U32 = 4, U64 = 5, NO_ID = -1 };
+
+// Search codes (the reference's SearchCodes, pst/static-pst-gpu.h:408-414).
+enum Code : uint32_t { REPORT_ABOVE = 0, LEFT_SEARCH = 1, RIGHT_SEARCH = 2, THREE_SEARCH = 3 };
+// Query kinds of the C-ABI.
+enum Kind : uint32_t { KIND_THREE = 0, KIND_LEFT = 1, KIND_RIGHT = 2 };
+
+struct NoId {};
+
+template <typename T, typename I>
+struct KV {
+  T dim2;
+  I id;
+};
+template <typename T>
+struct KV<T, NoId> {
+  T dim2;
+};
+template <typename T>
+struct DM {
+  T dim1;
+  T med;
+};
+
+// AoS input/output record with the reference's PointStruct layout
+// (helpers/point-struct.h:25-33, 107-117).
+template <typename T, typename I>
+struct Point {
+  T dim1_val;
+  T dim2_val;
+  I id;
+};
+template <typename T>
+struct Point<T, NoId> {
+  T dim1_val;
+  T dim2_val;
+};
+
+template <typename I>
+constexpr bool has_id_v = !std::is_same<I, NoId>::value;
+
+// Query record for batched searches: bounds in T plus the kind.
+template <typename T>
+struct Query {
+  T min_dim1;
+  T max_dim1;
+  T min_dim2;
+  uint32_t kind;
+};
+
+// ---------------------------------------------------------------------------
+// Order-preserving unsigned keys. IEEE compares treat -0.0 == +0.0, so both
+// map to the key of +0.0 and the tie-break decides, as in the reference's
+// compareDim1/compareDim2 (helpers/point-struct.h:62-68, 154-161).
+// ---------------------------------------------------------------------------
+__host__ __device__ inline uint32_t ord_key(float v) {
+  uint32_t u = __builtin_bit_cast(uint32_t, v);
+  if ((u & 0x7fffffffu) == 0) u = 0;
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ inline uint64_t ord_key(double v) {
+  uint64_t u = __builtin_bit_cast(uint64_t, v);
+  if ((u & 0x7fffffffffffffffull) == 0) u = 0;
+  return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+__host__ __device__ inline uint32_t ord_key(int32_t v) { return static_cast<uint32_t>(v) ^ 0x80000000u; }
+__host__ __device__ inline uint64_t ord_key(int64_t v) {
+  return static_cast<uint64_t>(v) ^ 0x8000000000000000ull;
+}
+__host__ __device__ inline uint32_t ord_key(uint32_t v) { return v; }
+__host__ __device__ inline uint64_t ord_key(uint64_t v) { return v; }
+
+template <typename V>
+__host__ __device__ inline bool is_nan(V v) {
+  if constexpr (std::is_floating_point<V>::value) return v != v;
+  else return false;
+}
+
+// ---------------------------------------------------------------------------
+// Tree shape (data independent). With M = n + 1, the node at depth t and
+// position j (slot 2^t - 1 + j) holds
+//     floor(M / 2^t) - 1 + [brev_t(j) < (M mod 2^t)]
+// points: the recursion m -> (floor(m/2), ceil(m/2)-1) splits M = m+1 into
+// (ceil(M/2), floor(M/2)), i.e. repeated halving with the larger half on the
+// left, whose parts are floor(M/2^t) or that plus one, the larger ones at the
+// bit-reversed positions below M mod 2^t.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline uint32_t brev_bits(uint32_t x, uint32_t t) {
+  if (t == 0) return 0;
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __brev(x) >> (32u - t);
+#else
+  uint32_t r = 0;
+  for (uint32_t b = 0; b < t; b++)
+    if ((x >> b) & 1u) r |= 1u << (t - 1 - b);
+  return r;
+#endif
+}
+
+__host__ __device__ inline uint32_t depth_of(uint64_t slot) {
+  // floor(log2(slot + 1))
+#if defined(__HIP_DEVICE_COMPILE__)
+  return 63u - static_cast<uint32_t>(__clzll(static_cast<long long>(slot + 1)));
+#else
+  return 63u - static_cast<uint32_t>(__builtin_clzll(slot + 1));
+#endif
+}
+
+// Point count of level-t position j in a subtree whose root holds M-1
+// points (negative when the position is past the last level).
+__host__ __device__ inline int64_t level_size(uint64_t M, uint32_t t, uint32_t j) {
+  const int64_t base = static_cast<int64_t>(M >> t);
+  const uint64_t r = M & ((1ull << t) - 1ull);
+  return base - 1 + (static_cast<uint64_t>(brev_bits(j, t)) < r ? 1 : 0);
+}
+
+__host__ __device__ inline int64_t subtree_size(uint64_t M, uint64_t slot) {
+  const uint32_t t = depth_of(slot);
+  const uint32_t j = static_cast<uint32_t>(slot + 1 - (1ull << t));
+  return level_size(M, t, j);
+}
+
+// Number of points held by level-t positions 0..j-1 (their segments are
+// consecutive in key1 order), in O(t): position j' < j is grouped by the
+// highest bit where it differs from j; each group is a set of positions
+// sharing a top prefix, whose bit reversals are an arithmetic progression.
+// Meaningful while all positions before j exist (level_size >= 0).
+__host__ __device__ inline int64_t level_start(uint64_t M, uint32_t t, uint32_t j) {
+  const int64_t base = static_cast<int64_t>(M >> t) - 1;
+  const uint64_t r = M & ((1ull << t) - 1ull);
+  int64_t extra = 0;
+  for (uint32_t b = 0; b < t; b++) {
+    if ((j >> b) & 1u) {
+      const uint32_t prefix = (j >> b) ^ 1u;
+      const uint64_t c = brev_bits(prefix, t - b);
+      if (r > c) {
+        const uint64_t step = 1ull << (t - b);
+        const uint64_t cnt = (r - c + step - 1) / step;
+        const uint64_t cap = 1ull << b;
+        extra += static_cast<int64_t>(cnt < cap ? cnt : cap);
+      }
+    }
+  }
+  return static_cast<int64_t>(j) * base + extra;
+}
+
+// ---------------------------------------------------------------------------
+// Wave64 helpers (gfx950 wavefront = 64 lanes).
+// ---------------------------------------------------------------------------
+constexpr int kWave = 64;
+
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+
+// Exclusive count of set bits of `mask` below this lane.
+__device__ inline uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+__device__ inline uint64_t ballot(bool p) { return __ballot(p); }
+
+template <typename V>
+__device__ inline V shfl(V v, int src) {
+  return __shfl(v, src, kWave);
+}
+
+}  // namespace pst

@@ -1,0 +1,113 @@
+// Internal interfaces between the translation units of libpst_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "pst_common.hpp"
+
+namespace pst {
+
+// ---- sorting (pst_sort.hip) ----
+hipError_t radix_sort_u32(void* tmp, size_t& tmp_bytes, const uint32_t* kin, uint32_t* kout,
+                          const uint32_t* vin, uint32_t* vout, size_t n, hipStream_t s);
+hipError_t radix_sort_u64(void* tmp, size_t& tmp_bytes, const uint64_t* kin, uint64_t* kout,
+                          const uint32_t* vin, uint32_t* vout, size_t n, hipStream_t s);
+
+// Error carrier: a HIP status plus the reference-style context message
+// ("Error in ... on device X (1-indexed) of Y: " + hipGetErrorString).
+struct Status {
+  int code = 0;  // 0 ok, else a PST_ERR_* value
+  std::string msg;
+  bool ok() const { return code == 0; }
+};
+
+// Sizes of one tree's device arrays.
+size_t kv_elem_size(int dtype, int id_dtype);
+size_t dm_elem_size(int dtype);
+size_t point_size(int dtype, int id_dtype);
+size_t dtype_size(int dtype);
+
+// ---- construction (pst_build.hip) ----
+struct BuildTimes {
+  float keys_ms = 0, sort_ms = 0, levels_ms = 0, finish_ms = 0;
+};
+// Builds the tree for n AoS points (device pointer) into kv/dm, which hold
+// S = 2^ceil(log2(n+1)) - 1 slots each. Returns PST_ERR_NAN if a NaN dim
+// (or floating ID) is found.
+Status build_tree(int dtype, int id_dtype, const void* pts_d, size_t n, void* kv_d, void* dm_d,
+                  hipStream_t stream, BuildTimes* times);
+
+// Reference-layout blob (dim1, dim2, med, ids, bitcodes) written to a
+// device buffer, for export / parity checks.
+Status export_tree(int dtype, int id_dtype, size_t n, const void* kv_d, const void* dm_d,
+                   void* blob_d, size_t off_dim1, size_t off_dim2, size_t off_med, size_t off_id,
+                   size_t off_bits, hipStream_t stream);
+
+// ---- forest of shallow trees (pst_forest.hip) ----
+// Tree c covers input points [c*full, min(n, (c+1)*full)); full =
+// 2^ceil(log2(tpb+1)) - 1. Tree c's kv/dm arrays start at slot c*full.
+Status build_forest(int dtype, int id_dtype, const void* pts_d, size_t n, size_t full_slots,
+                    void* kv_d, void* dm_d, hipStream_t stream);
+Status forest_search(int dtype, int id_dtype, size_t n, size_t full_slots, const void* kv_d,
+                     const void* dm_d, int kind, int ret_point, const void* min1,
+                     const void* max1, const void* min2, void* out_d, size_t cap,
+                     unsigned long long* counter_d, hipStream_t stream);
+
+// ---- searches on one deep tree (pst_search.hip) ----
+struct SearchScratch {
+  void* list[2] = {nullptr, nullptr};  // work lists (u64 items)
+  size_t list_cap = 0;                 // items per list
+  unsigned long long* ctr = nullptr;   // round counters
+  void* qbuf = nullptr;                // query records for single searches
+  size_t qbuf_bytes = 0;
+  unsigned long long* cursor = nullptr;  // per-query output cursors
+  size_t cursor_cap = 0;
+  int num_waves = 0;
+};
+Status scratch_reserve(SearchScratch& sc, size_t nq, int dtype, hipStream_t stream);
+void scratch_release(SearchScratch& sc);
+
+// mode 0: count into counts_d[q]; mode 1: write results for query q at
+// out + offsets_d[q] (offsets_d may be null: all offsets 0), at most
+// caps_d[q] of them (caps_d null: cap_all), counting into cursor[q].
+struct SearchArgs {
+  int dtype, id_dtype;
+  size_t n;
+  const void* kv;
+  const void* dm;
+  const void* queries;  // device Query<T>[nq]
+  size_t nq;
+  int mode;             // 0 count, 1 write
+  int ret_point;
+  unsigned long long* counts;        // mode 0 output / mode 1 cursors
+  const unsigned long long* offsets;  // mode 1
+  const unsigned long long* caps;     // mode 1 (nullable)
+  unsigned long long cap_all;
+  void* out;
+  int rounds;           // kernel launches enqueued (>= 1)
+  int budget_small, budget_large;
+  int waves_per_cu;
+  const void* single_query;  // host Query<T> for one query (queries ignored)
+  unsigned long long* debug;  // optional per-round counters (8 per round)
+};
+Status run_search(const SearchArgs& a, SearchScratch& sc, hipStream_t stream);
+unsigned long long* search_error_flag(SearchScratch& sc);
+
+// ---- interval parallel search (pst_ips.hip) ----
+// counter_d must be zero on entry; results past `cap` are counted only.
+Status ips_search(int dtype, int id_dtype, const void* pts_d, size_t n, int ret_point,
+                  const void* search_val, void* out_d, unsigned long long cap,
+                  unsigned long long* counter_d, unsigned num_blocks, hipStream_t stream);
+
+// ---- metacells (pst_metacell.hip) ----
+Status form_metacells(int dtype, int id_dtype, const void* vertices_d, const uint32_t pt_dims[3],
+                      const uint32_t mc_dims[3], void* out_pts_d, hipStream_t stream);
+
+// helpers
+Status hip_status(hipError_t e, const std::string& what);
+
+}  // namespace pst

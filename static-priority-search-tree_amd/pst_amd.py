@@ -1,0 +1,297 @@
+"""Python mirror of the drop-in interface, over libpst_hip.so (ctypes).
+
+The classes keep the reference's names and argument order
+(pst/static-pst-gpu.h:76-132, interval-parallel-search/
+interval-parallel-search.h:29-32): StaticPSTGPU(pt_arr_d, num_elems, ...)
+with threeSidedSearch(min_dim1, max_dim1, min_dim2), twoSidedLeftSearch(
+max_dim1, min_dim2), twoSidedRightSearch(min_dim1, min_dim2),
+calcGlobalMemNeeded(num_elems), and intervalParallelSearch(...). Device
+memory is handled with torch tensors (plumbing only); every search runs in
+the HIP library. The module refuses to work without the built library.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpst_hip.so")
+
+DT = {"f32": 0, "f64": 1, "i32": 2, "i64": 3, "u32": 4, "u64": 5, None: -1}
+NP = {"f32": np.float32, "f64": np.float64, "i32": np.int32, "i64": np.int64,
+      "u32": np.uint32, "u64": np.uint64}
+THREE, LEFT, RIGHT = 0, 1, 2
+RET_ID, RET_POINT = 0, 1
+ERR_CAPACITY = 4
+
+EXPORTED = [
+    "pst_last_error", "pst_point_size", "pst_free",
+    "pst_gpu_create", "pst_gpu_destroy", "pst_gpu_num_elems", "pst_gpu_dev_ind",
+    "pst_gpu_num_devs", "pst_gpu_three_sided_search", "pst_gpu_two_sided_left_search",
+    "pst_gpu_two_sided_right_search", "pst_gpu_search_into", "pst_gpu_calc_global_mem_needed",
+    "pst_gpu_export_tree", "pst_gpu_export_bytes", "pst_gpu_batch_count", "pst_gpu_batch_write",
+    "pst_gpu_set_search_tuning", "pst_interval_parallel_search",
+    "pst_interval_parallel_search_into",
+]
+
+
+def point_dtype(t: str, i: Optional[str]) -> np.dtype:
+    """numpy view of PointStruct<T, IDType, num_IDs> (C layout)."""
+    f = [("dim1", NP[t]), ("dim2", NP[t])]
+    if i is not None:
+        f.append(("id", NP[i]))
+    return np.dtype(f, align=True)
+
+
+def query_dtype(t: str) -> np.dtype:
+    """numpy view of pst_query_<t>."""
+    return np.dtype([("min_dim1", NP[t]), ("max_dim1", NP[t]), ("min_dim2", NP[t]),
+                     ("kind", np.uint32)], align=True)
+
+
+_lib = None
+_hip = None
+
+
+class PSTError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run "
+                               "`python static-priority-search-tree_amd/build.py`")
+        # One HIP runtime per process: when torch is present, load it first so
+        # that the library binds to the runtime torch already carries.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        L = ctypes.CDLL(LIB_PATH)
+        c = ctypes
+        vp, sz, i32 = c.c_void_p, c.c_size_t, c.c_int
+        L.pst_last_error.restype = c.c_char_p
+        L.pst_point_size.restype = sz
+        L.pst_point_size.argtypes = [i32, i32]
+        L.pst_free.argtypes = [vp]
+        L.pst_gpu_create.argtypes = [vp, sz, i32, i32, i32, i32, i32, vp, c.POINTER(vp)]
+        L.pst_gpu_destroy.argtypes = [vp]
+        L.pst_gpu_num_elems.restype = sz
+        L.pst_gpu_num_elems.argtypes = [vp]
+        L.pst_gpu_three_sided_search.argtypes = [vp, i32, c.POINTER(sz), c.POINTER(vp), vp, vp, vp]
+        L.pst_gpu_two_sided_left_search.argtypes = [vp, i32, c.POINTER(sz), c.POINTER(vp), vp, vp]
+        L.pst_gpu_two_sided_right_search.argtypes = [vp, i32, c.POINTER(sz), c.POINTER(vp), vp, vp]
+        L.pst_gpu_search_into.argtypes = [vp, i32, i32, vp, vp, vp, vp, sz, c.POINTER(sz), vp]
+        L.pst_gpu_calc_global_mem_needed.restype = sz
+        L.pst_gpu_calc_global_mem_needed.argtypes = [sz, i32, i32]
+        L.pst_gpu_export_bytes.restype = sz
+        L.pst_gpu_export_bytes.argtypes = [vp]
+        L.pst_gpu_export_tree.argtypes = [vp, vp, sz, c.POINTER(sz)]
+        L.pst_gpu_batch_count.argtypes = [vp, vp, sz, vp, vp]
+        L.pst_gpu_batch_write.argtypes = [vp, vp, sz, i32, vp, vp, vp, vp]
+        L.pst_gpu_set_search_tuning.argtypes = [vp, i32, i32, i32, i32]
+        L.pst_interval_parallel_search.argtypes = [vp, sz, i32, i32, i32, c.POINTER(vp),
+                                                   c.POINTER(sz), vp, i32, i32, i32,
+                                                   c.c_uint, c.c_uint]
+        L.pst_interval_parallel_search_into.argtypes = [vp, sz, i32, i32, i32, vp, vp, sz, vp, vp]
+        _lib = L
+    return _lib
+
+
+def hip():
+    """libamdhip64 for plain memcpy of library-allocated results."""
+    global _hip
+    if _hip is None:
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                _hip = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        _hip.hipDeviceSynchronize.argtypes = []
+    return _hip
+
+
+def _check(rc):
+    if rc != 0:
+        raise PSTError(rc, lib().pst_last_error().decode())
+
+
+def _scalar(v, t):
+    return np.array([v], dtype=NP[t])
+
+
+def _d2h(ptr, nbytes) -> np.ndarray:
+    out = np.empty(max(nbytes, 1), dtype=np.uint8)
+    if nbytes:
+        rc = hip().hipMemcpy(out.ctypes.data, ptr, nbytes, 2)
+        if rc != 0:
+            raise PSTError(2, f"hipMemcpy failed ({rc})")
+    return out[:nbytes]
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def to_device(arr: np.ndarray, device="cuda"):
+    import torch
+    b = np.ascontiguousarray(arr).view(np.uint8).reshape(-1)
+    return torch.from_numpy(b.copy()).to(device)
+
+
+class StaticPSTGPU:
+    """Drop-in for the reference's StaticPSTGPU (pst/static-pst-gpu.h:72)."""
+
+    def __init__(self, pt_arr_d, num_elems: int, dtype: str = "f32", id_dtype: Optional[str] = "u32",
+                 warp_multiplier: int = 1, dev_ind: int = 0, num_devs: int = 1, stream=None):
+        self.t, self.i, self.n = dtype, id_dtype, int(num_elems)
+        h = ctypes.c_void_p()
+        ptr = pt_arr_d.data_ptr() if hasattr(pt_arr_d, "data_ptr") else pt_arr_d
+        _check(lib().pst_gpu_create(ptr, self.n, DT[dtype], DT[id_dtype], warp_multiplier,
+                                    dev_ind, num_devs, _stream_ptr(stream), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            _check(lib().pst_gpu_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- reference-named blocking searches (results copied to host) ----
+    def _rec(self, ret):
+        if ret == RET_POINT or self.i is None:
+            return point_dtype(self.t, self.i), RET_POINT
+        return np.dtype(NP[self.i]), RET_ID
+
+    def _finish(self, num, ptr, ret):
+        dt, rp = self._rec(ret)
+        if not ptr.value:
+            return np.zeros(0, dtype=dt)
+        raw = _d2h(ptr.value, num.value * dt.itemsize)
+        _check(lib().pst_free(ptr.value))
+        return raw.view(dt)
+
+    def threeSidedSearch(self, min_dim1_val, max_dim1_val, min_dim2_val, ret=RET_ID):
+        num, ptr = ctypes.c_size_t(), ctypes.c_void_p()
+        _, rp = self._rec(ret)
+        a, b, c = (_scalar(v, self.t) for v in (min_dim1_val, max_dim1_val, min_dim2_val))
+        _check(lib().pst_gpu_three_sided_search(self.h, rp, ctypes.byref(num), ctypes.byref(ptr),
+                                                a.ctypes.data, b.ctypes.data, c.ctypes.data))
+        return self._finish(num, ptr, rp)
+
+    def twoSidedLeftSearch(self, max_dim1_val, min_dim2_val, ret=RET_ID):
+        num, ptr = ctypes.c_size_t(), ctypes.c_void_p()
+        _, rp = self._rec(ret)
+        b, c = _scalar(max_dim1_val, self.t), _scalar(min_dim2_val, self.t)
+        _check(lib().pst_gpu_two_sided_left_search(self.h, rp, ctypes.byref(num), ctypes.byref(ptr),
+                                                   b.ctypes.data, c.ctypes.data))
+        return self._finish(num, ptr, rp)
+
+    def twoSidedRightSearch(self, min_dim1_val, min_dim2_val, ret=RET_ID):
+        num, ptr = ctypes.c_size_t(), ctypes.c_void_p()
+        _, rp = self._rec(ret)
+        a, c = _scalar(min_dim1_val, self.t), _scalar(min_dim2_val, self.t)
+        _check(lib().pst_gpu_two_sided_right_search(self.h, rp, ctypes.byref(num), ctypes.byref(ptr),
+                                                    a.ctypes.data, c.ctypes.data))
+        return self._finish(num, ptr, rp)
+
+    def search(self, kind, min1=0, max1=0, min2=0, ret=RET_ID):
+        if kind == THREE:
+            return self.threeSidedSearch(min1, max1, min2, ret)
+        if kind == LEFT:
+            return self.twoSidedLeftSearch(max1, min2, ret)
+        return self.twoSidedRightSearch(min1, min2, ret)
+
+    # ---- preallocated output ----
+    def search_into(self, kind, min1, max1, min2, out, capacity, ret=RET_ID, stream=None):
+        """out: device tensor (or pointer) of `capacity` records. Returns count."""
+        num = ctypes.c_size_t()
+        _, rp = self._rec(ret)
+        a, b, c = (_scalar(v, self.t) for v in (min1, max1, min2))
+        ptr = out.data_ptr() if hasattr(out, "data_ptr") else out
+        rc = lib().pst_gpu_search_into(self.h, kind, rp, a.ctypes.data, b.ctypes.data, c.ctypes.data,
+                                       ptr, capacity, ctypes.byref(num), _stream_ptr(stream))
+        if rc not in (0, ERR_CAPACITY):
+            _check(rc)
+        return num.value
+
+    # ---- batched ----
+    def batch_count(self, queries_d, nq, counts_d, stream=None):
+        _check(lib().pst_gpu_batch_count(self.h, queries_d.data_ptr(), nq, counts_d.data_ptr(),
+                                         _stream_ptr(stream)))
+
+    def batch_write(self, queries_d, nq, offsets_d, out_d, ret=RET_ID, caps_d=None, stream=None):
+        _, rp = self._rec(ret)
+        _check(lib().pst_gpu_batch_write(self.h, queries_d.data_ptr(), nq, rp, offsets_d.data_ptr(),
+                                         caps_d.data_ptr() if caps_d is not None else None,
+                                         out_d.data_ptr(), _stream_ptr(stream)))
+
+    def set_tuning(self, rounds=0, budget_small=0, budget_large=0, waves_per_cu=-1):
+        _check(lib().pst_gpu_set_search_tuning(self.h, rounds, budget_small, budget_large, waves_per_cu))
+
+    # ---- layout ----
+    def export_tree(self):
+        """Reference-layout arrays dim1, dim2, med, ids, bits (host numpy)."""
+        nb = lib().pst_gpu_export_bytes(self.h)
+        blob = np.zeros(max(nb, 8), dtype=np.uint8)
+        off = (ctypes.c_size_t * 6)()
+        _check(lib().pst_gpu_export_tree(self.h, blob.ctypes.data, nb, off))
+        S = off[5]
+        tb = np.dtype(NP[self.t]).itemsize
+        d = {"dim1": blob[off[0]:off[0] + S * tb].view(NP[self.t]),
+             "dim2": blob[off[1]:off[1] + S * tb].view(NP[self.t]),
+             "med": blob[off[2]:off[2] + S * tb].view(NP[self.t]),
+             "bits": blob[off[4]:off[4] + S]}
+        if self.i is not None:
+            ib = np.dtype(NP[self.i]).itemsize
+            d["ids"] = blob[off[3]:off[3] + S * ib].view(NP[self.i])
+        return d
+
+    @staticmethod
+    def calcGlobalMemNeeded(num_elems, dtype="f32", id_dtype="u32"):
+        return lib().pst_gpu_calc_global_mem_needed(num_elems, DT[dtype], DT[id_dtype])
+
+
+def intervalParallelSearch(pt_arr_d, num_elems, search_val, dtype="f32", id_dtype="u32",
+                           ret=RET_ID, dev_ind=0, num_devs=1, warp_size=64,
+                           num_thread_blocks=0, threads_per_block=256):
+    """Drop-in for the reference's intervalParallelSearch (results to host)."""
+    if id_dtype is None:
+        ret = RET_POINT
+    num, ptr = ctypes.c_size_t(), ctypes.c_void_p()
+    v = _scalar(search_val, dtype)
+    p = pt_arr_d.data_ptr() if hasattr(pt_arr_d, "data_ptr") else pt_arr_d
+    _check(lib().pst_interval_parallel_search(p, num_elems, DT[dtype], DT[id_dtype], ret,
+                                              ctypes.byref(ptr), ctypes.byref(num), v.ctypes.data,
+                                              dev_ind, num_devs, warp_size, num_thread_blocks,
+                                              threads_per_block))
+    dt = point_dtype(dtype, id_dtype) if ret == RET_POINT else np.dtype(NP[id_dtype])
+    raw = _d2h(ptr.value, num.value * dt.itemsize) if ptr.value else np.zeros(0, np.uint8)
+    if ptr.value:
+        _check(lib().pst_free(ptr.value))
+    return raw.view(dt)
+
+
+def ips_into(pt_arr_d, num_elems, search_val, out_d, capacity, counter_d, dtype="f32",
+             id_dtype="u32", ret=RET_ID, stream=None):
+    v = _scalar(search_val, dtype)
+    _check(lib().pst_interval_parallel_search_into(pt_arr_d.data_ptr(), num_elems, DT[dtype],
+                                                   DT[id_dtype], ret, v.ctypes.data,
+                                                   out_d.data_ptr(), capacity,
+                                                   counter_d.data_ptr(), _stream_ptr(stream)))

@@ -1,0 +1,65 @@
+"""CPU checks of the boundary: libpst_hip.so loads and exports every symbol
+include/pst/pst_capi.h declares; host-side size helpers agree with the
+reference's struct layouts. No device calls."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pst", "pst_capi.h")
+
+
+def declared():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(pst_[a-z0-9_]+)\s*\(", txt)))
+
+
+@pytest.fixture(scope="module")
+def L():
+    import pst_amd
+    return pst_amd.lib()
+
+
+def test_header_declares_the_mirror_list():
+    import pst_amd
+    assert sorted(pst_amd.EXPORTED) == declared()
+
+
+def test_library_exports_every_declared_symbol(L):
+    missing = [s for s in declared() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_point_size_matches_reference_layout(L):
+    import oracle as O
+    import pst_amd
+    for t in ("f32", "f64", "i32", "i64"):
+        for i in (None, "f32", "f64", "i32", "i64", "u32", "u64"):
+            assert L.pst_point_size(pst_amd.DT[t], pst_amd.DT[i]) == O.point_dtype(t, i).itemsize
+
+
+def test_query_record_layout():
+    import pst_amd
+    assert pst_amd.query_dtype("f32").itemsize == 16
+    assert pst_amd.query_dtype("f64").itemsize == 32
+    assert pst_amd.query_dtype("i32").itemsize == 16
+    assert pst_amd.query_dtype("i64").itemsize == 32
+
+
+def test_calc_global_mem_needed_covers_tree(L):
+    import pst_amd
+    n = 1 << 20
+    need = pst_amd.StaticPSTGPU.calcGlobalMemNeeded(n, "f32", "u32")
+    S = (1 << 21) - 1
+    assert need >= S * 16 + n * 12
+
+
+def test_invalid_type_codes_rejected_without_device(L):
+    h = ctypes.c_void_p()
+    rc = L.pst_gpu_create(None, 0, 9, 0, 1, 0, 1, None, ctypes.byref(h))
+    assert rc == 1
+    assert b"unsupported" in L.pst_last_error()
