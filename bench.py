@@ -1,0 +1,266 @@
+#!/usr/bin/env python
+"""Benchmark of the static priority search tree hot path on one MI355X.
+
+Workload (N = 1): a 2^26-point tree of float32 intervals with uint32 IDs
+(dim1 ~ U[0,1), dim2 = min(dim1 + Exp(0.02), 1), id = index); one step =
+one batch of 2048 queries: 1024 two-sided stabbing queries (the
+reference's twoSidedLeftSearch(q, q): dim1 <= q <= dim2, q ~ U[0,1)) and
+1024 three-sided queries (threeSidedSearch(a, a + w, c), a, c ~ U[0,1),
+w = exp(U[ln 1e-4, ln 0.5])), run through the batched path (count pass,
+device prefix sum, write pass; IDs returned). Reported beside it, outside
+the timed steps: the plain one-launch-per-query path (preallocated
+output), interval parallel search on the same data, two-sided right
+queries, tree construction, and the CPU baseline (the oracle's stack
+walks, single thread, on the same tree).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "static-priority-search-tree_amd")
+sys.path.insert(0, PKG)
+
+HBM_PEAK_GBPS = 8000.0
+BYTES_PER_ID = 21  # 3 x f32 (dim1, dim2, med) + u32 id + 1 B bitcode read, u32 written
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--log2n", type=int, default=26)
+    ap.add_argument("--nq2", type=int, default=1024)
+    ap.add_argument("--nq3", type=int, default=1024)
+    ap.add_argument("--plain", type=int, default=256, help="queries timed on the plain path")
+    ap.add_argument("--ips", type=int, default=64, help="queries timed for interval parallel search")
+    ap.add_argument("--cpu-sample", type=int, default=96, help="queries in the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by replicating the tree or splitting points; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import pst_amd as P
+    import pst_data as D
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    n = 1 << args.log2n
+    log(f"[bench] generating {n} points")
+    pts = D.interval_points(n, seed=0)
+    d_pts = P.to_device(pts, dev)
+    torch.cuda.synchronize()
+
+    # ---- construction
+    log("[bench] building tree")
+    t0 = time.perf_counter()
+    tree = P.StaticPSTGPU(d_pts, n, "f32", "u32")
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    S = (1 << (args.log2n + 1)) - 1
+    build_bytes = n * 12 + S * 17
+    log(f"[bench] build {build_ms:.1f} ms")
+
+    # ---- queries
+    qs = D.float_query_mix(args.nq3, args.nq2, seed=0)
+    qs["kind"][:args.nq2] = 1  # two-sided left stabbing (q, q)
+    nq = len(qs)
+    d_q = P.to_device(qs, dev)
+    counts = torch.zeros(nq, dtype=torch.int64, device=dev)
+    offsets = torch.zeros(nq + 1, dtype=torch.int64, device=dev)
+    # size the output from one count pass
+    tree.batch_count(d_q, nq, counts, stream=stream)
+    torch.cuda.synchronize()
+    total = int(counts.sum().item())
+    log(f"[bench] batch of {nq}: {total} reported IDs")
+    out = torch.empty(total + 1, dtype=torch.int32, device=dev)
+
+    # sanity (outside timing): batch counts equal brute-force counts for a few queries
+    host_counts = counts.cpu().numpy()
+    ok = True
+    d1, d2 = pts["dim1"], pts["dim2"]
+    for k in (0, 1, args.nq2, args.nq2 + 1):
+        a, b, c, kind = qs["min_dim1"][k], qs["max_dim1"][k], qs["min_dim2"][k], int(qs["kind"][k])
+        if kind == 1:
+            want = int(np.count_nonzero((d1 <= b) & (d2 >= c)))
+        else:
+            want = int(np.count_nonzero((d1 >= a) & (d1 <= b) & (d2 >= c)))
+        ok &= want == int(host_counts[k])
+
+    # ---- timed steps: batched count + scan + write
+    for _ in range(args.warmup):
+        tree.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tree.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ok &= int(offsets[nq].item()) == total
+    ms_per_step = elapsed * 1e3 / args.steps
+    qps = nq * args.steps / elapsed
+    ids_per_s = total * args.steps / elapsed
+
+    # ---- per-pass timing with events on the launch stream (roofline figure)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    t_count = t_write = 0.0
+    reps = max(1, args.steps)
+    for _ in range(reps):
+        ev[0].record(stream)
+        tree.batch_count(d_q, nq, counts, stream=stream)
+        ev[1].record(stream)
+        tree.batch_write(d_q, nq, offsets, out, caps_d=counts, stream=stream)
+        ev[2].record(stream)
+        torch.cuda.synchronize()
+        t_count += ev[0].elapsed_time(ev[1])
+        t_write += ev[1].elapsed_time(ev[2])
+    t_count /= reps
+    t_write /= reps
+    alg_bytes = total * BYTES_PER_ID
+    achieved = alg_bytes / (t_write * 1e-3) / 1e9
+    log(f"[bench] count pass {t_count:.2f} ms, write pass {t_write:.2f} ms, "
+        f"write-pass alg {achieved:.0f} GB/s")
+
+    # ---- plain path: one blocking search per query, preallocated output
+    np_plain = min(args.plain, nq)
+    sel = np.concatenate([np.arange(np_plain // 2), args.nq2 + np.arange(np_plain - np_plain // 2)])
+    cap = int(host_counts[sel].max()) + 1
+    pout = torch.empty(cap, dtype=torch.int32, device=dev)
+    for k in sel[:4]:
+        tree.search_into(int(qs["kind"][k]), qs["min_dim1"][k], qs["max_dim1"][k], qs["min_dim2"][k], pout, cap)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_ids = 0
+    for k in sel:
+        got = tree.search_into(int(qs["kind"][k]), qs["min_dim1"][k], qs["max_dim1"][k],
+                               qs["min_dim2"][k], pout, cap)
+        plain_ids += got
+        ok &= got == int(host_counts[k])
+    plain_s = time.perf_counter() - t0
+    plain_qps = len(sel) / plain_s
+    log(f"[bench] plain path {plain_qps:.0f} q/s")
+
+    # ---- two-sided right (q, q): outputs average n/2
+    right_q = D.stabbing_queries(8, seed=5)
+    rq = np.zeros(len(right_q), P.query_dtype("f32"))
+    rq["min_dim1"] = right_q
+    rq["min_dim2"] = right_q
+    rq["kind"] = 2
+    d_rq = P.to_device(rq, dev)
+    rc = torch.zeros(len(rq), dtype=torch.int64, device=dev)
+    ro = torch.zeros(len(rq) + 1, dtype=torch.int64, device=dev)
+    tree.batch_count(d_rq, len(rq), rc, stream=stream)
+    torch.cuda.synchronize()
+    rtotal = int(rc.sum().item())
+    rout = torch.empty(rtotal + 1, dtype=torch.int32, device=dev)
+    tree.batch_search(d_rq, len(rq), rc, ro, rout, rtotal + 1, stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tree.batch_search(d_rq, len(rq), rc, ro, rout, rtotal + 1, stream=stream)
+    torch.cuda.synchronize()
+    right_s = time.perf_counter() - t0
+    del rout
+
+    # ---- interval parallel search on the same data (stabbing queries)
+    nips = min(args.ips, args.nq2)
+    ips_out = torch.empty(int(host_counts[:nips].max()) + 1, dtype=torch.int32, device=dev)
+    ips_ctr = torch.zeros(1, dtype=torch.int64, device=dev)
+    P.ips_into(d_pts, n, qs["max_dim1"][0], ips_out, ips_out.numel(), ips_ctr, stream=stream)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for k in range(nips):
+        P.ips_into(d_pts, n, qs["max_dim1"][k], ips_out, ips_out.numel(), ips_ctr, stream=stream)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ips_ms = ev0.elapsed_time(ev1) / nips
+    ok &= int(ips_ctr.item()) == int(host_counts[nips - 1])
+    ips_gbps = (n * 12 + host_counts[:nips].mean() * 4) / (ips_ms * 1e-3) / 1e9
+    log(f"[bench] IPS {1e3 / ips_ms:.0f} q/s ({ips_gbps:.0f} GB/s)")
+
+    # ---- CPU baseline: the oracle's walks on the same tree (exported)
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O  # CPU checker, used here only as the timed CPU baseline
+        ex_bytes = P.lib().pst_gpu_export_bytes(tree.h)
+        tr = O.empty_tree("f32", "u32", n)
+        assert tr.blob.nbytes >= ex_bytes
+        import ctypes
+        off = (ctypes.c_size_t * 6)()
+        P._check(P.lib().pst_gpu_export_tree(tree.h, tr.blob.ctypes.data, ex_bytes, off))
+        ns = min(args.cpu_sample, nq)
+        csel = np.concatenate([np.arange(ns // 2), args.nq2 + np.arange(ns - ns // 2)])
+        kinds = np.where(qs["kind"][csel] == 1, 1, 0)
+        tot, secs = O.time_queries(tr, kinds, qs["min_dim1"][csel], qs["max_dim1"][csel],
+                                   qs["min_dim2"][csel], threads=1)
+        ok &= tot == int(host_counts[csel].sum())
+        cpu = {"value": round(ns / secs, 3), "unit": "queries/s", "cores": 1, "kind": "port",
+               "sample": f"{ns} of the batch's queries ({ns // 2} two-sided stabbing, "
+                         f"{ns - ns // 2} three-sided) on the same 2^{args.log2n}-point tree, "
+                         f"oracle stack walk (g++ -O2), one thread, {secs:.1f} s, "
+                         f"{tot} IDs"}
+        log(f"[bench] CPU {ns / secs:.2f} q/s over {secs:.1f} s")
+        del tr
+
+    tree.close()
+    res = {
+        "metric": "queries/s + reported IDs/s (GB/s vs HBM peak), 2^26-pt PST; p50/p99 latency",
+        "value": round(qps, 2),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SplitMix64 generator, pst_data.py)",
+        "config": {
+            "workload": f"batched search on a 2^{args.log2n}-point f32/u32 StaticPSTGPU: "
+                        f"{args.nq2} twoSidedLeftSearch(q, q) + {args.nq3} threeSidedSearch per step, "
+                        "count + scan + write, IDs returned",
+            "points": n, "queries_per_step": nq, "reported_ids_per_step": total,
+        },
+        "reported_ids_per_s": round(ids_per_s, 1),
+        "alg_GBps_step": round(ids_per_s * BYTES_PER_ID / 1e9, 1),
+        "count_pass_ms": round(t_count, 3),
+        "write_pass_ms": round(t_write, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "kernel": "k_search_round (write pass: all rounds of one pass)"},
+        "cpu_baseline": cpu,
+        "plain_per_query_qps": round(plain_qps, 2),
+        "batched_over_plain": round(qps / plain_qps, 2),
+        "right_qq_qps": round(len(rq) / right_s, 2),
+        "right_qq_ids_per_query": rtotal // len(rq),
+        "ips_qps": round(1e3 / ips_ms, 2),
+        "ips_GBps": round(float(ips_gbps), 1),
+        "build_ms": round(build_ms, 2),
+        "build_GBps_io_floor": round(build_bytes / (build_ms * 1e-3) / 1e9, 1),
+        "results_checked": bool(ok),
+    }
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -161,6 +161,15 @@ int pst_gpu_batch_count(pst_tree* tree, const void* queries_d, size_t num_querie
 int pst_gpu_batch_write(pst_tree* tree, const void* queries_d, size_t num_queries, int ret,
                         const unsigned long long* offsets_d, const unsigned long long* caps_d,
                         void* out_d, pst_stream_t stream);
+/* Count, scan and write in one asynchronous call: counts_d[q] (nq
+ * entries) and offsets_d (nq + 1 entries, offsets_d[nq] = total) are
+ * written on the device; results go to out_d, of which only the first
+ * out_capacity records are written (compare offsets_d[nq] with it). No host
+ * synchronisation and no allocation after the first call for a batch size,
+ * so the call can be captured into a HIP graph. */
+int pst_gpu_batch_search(pst_tree* tree, const void* queries_d, size_t num_queries, int ret,
+                         unsigned long long* counts_d, unsigned long long* offsets_d, void* out_d,
+                         size_t out_capacity, pst_stream_t stream);
 /* Tuning knobs of the search rounds (0 keeps the default). */
 int pst_gpu_set_search_tuning(pst_tree* tree, int rounds, int budget_small, int budget_large,
                               int waves_per_cu);

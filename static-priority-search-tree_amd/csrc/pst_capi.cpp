@@ -145,6 +145,7 @@ int blocking_search(pst_tree* t, int kind, int ret, size_t* num, void** res, con
     return fail(PST_ERR_INVALID, "Error: IDs requested from a tree built without IDs");
   if (t->n == 0) {
     std::printf("Tree is empty; nothing to search\n");
+    std::fflush(stdout);
     *num = 0;
     *res = nullptr;
     return PST_OK;
@@ -394,6 +395,36 @@ int pst_gpu_batch_write(pst_tree* t, const void* queries_d, size_t nq, int ret, 
   a.caps = caps_d;
   a.cap_all = ~0ull;
   a.out = out_d;
+  st = pst::run_search(a, t->sc, s);
+  return st.ok() ? PST_OK : fail(st);
+}
+
+int pst_gpu_batch_search(pst_tree* t, const void* queries_d, size_t nq, int ret, unsigned long long* counts_d,
+                         unsigned long long* offsets_d, void* out_d, size_t out_capacity, pst_stream_t stream) {
+  if (!t || !counts_d || !offsets_d) return fail(PST_ERR_INVALID, "Error: null argument");
+  if (nq == 0) return PST_OK;
+  if (nq >= (1ull << 24)) return fail(PST_ERR_INVALID, "Error: batches hold fewer than 2^24 queries");
+  if (ret != PST_RET_POINT && t->id_dtype == PST_NO_ID)
+    return fail(PST_ERR_INVALID, "Error: IDs requested from a tree built without IDs");
+  int rc = pst_gpu_batch_count(t, queries_d, nq, counts_d, stream);
+  if (rc != PST_OK) return rc;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  Status st = pst::exclusive_scan(counts_d, offsets_d, nq, s);
+  if (!st.ok()) return fail(st);
+  if (t->n == 0) return PST_OK;
+  st = pst::scratch_reserve(t->sc, nq, t->dtype, s);
+  if (!st.ok()) return fail(st);
+  pst::SearchArgs a = base_args(t);
+  a.queries = queries_d;
+  a.nq = nq;
+  a.mode = 1;
+  a.ret_point = ret;
+  a.counts = t->sc.cursor;
+  a.offsets = offsets_d;
+  a.caps = counts_d;
+  a.cap_all = ~0ull;
+  a.out = out_d;
+  a.out_total = out_capacity;
   st = pst::run_search(a, t->sc, s);
   return st.ok() ? PST_OK : fail(st);
 }

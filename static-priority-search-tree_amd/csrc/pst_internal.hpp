@@ -93,9 +93,12 @@ struct SearchArgs {
   int waves_per_cu;
   const void* single_query;  // host Query<T> for one query (queries ignored)
   unsigned long long* debug;  // optional per-round counters (8 per round)
+  unsigned long long out_total;  // records in `out` (0: unlimited)
 };
 Status run_search(const SearchArgs& a, SearchScratch& sc, hipStream_t stream);
 unsigned long long* search_error_flag(SearchScratch& sc);
+// offsets[q] = sum of in[0..q), offsets[n] = total (one workgroup).
+Status exclusive_scan(const unsigned long long* in, unsigned long long* offsets, size_t n, hipStream_t s);
 
 // ---- interval parallel search (pst_ips.hip) ----
 // counter_d must be zero on entry; results past `cap` are counted only.

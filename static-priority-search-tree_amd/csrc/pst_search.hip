@@ -52,6 +52,7 @@ struct Params {
   const unsigned long long* offsets;
   const unsigned long long* caps;
   unsigned long long cap_all;
+  unsigned long long out_total;  // records the output buffer holds
   void* out;
   unsigned long long* err;  // bit 0: stack would overflow; bit 1: invalid work item
   unsigned long long* dbg;  // optional per-round counters (nullable)
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
         b = __shfl(b, 0, kWave);
         Rec* out = static_cast<Rec*>(P.out) + off_q;
         for (uint32_t x = lane; x < nstage; x += kWave)
-          if (b + x < cap_q) out[b + x] = stage[x];
+          if (b + x < cap_q && off_q + b + x < P.out_total) out[b + x] = stage[x];
         nstage = 0;
         __builtin_amdgcn_wave_barrier();
       }
@@ -303,9 +304,51 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
   }
 }
 
-struct DevInfo {
-  int cus = 0;
-};
+// Exclusive prefix sum of per-query counts: offsets[q] = sum of counts
+// before q, offsets[nq] = total. One workgroup of 1024 threads walks the
+// array in chunks of 4096 (batches hold fewer than 2^24 queries).
+constexpr int kScanBlock = 1024;
+__global__ __launch_bounds__(kScanBlock) void k_exclusive_scan(const unsigned long long* __restrict__ in,
+                                                               unsigned long long* __restrict__ out,
+                                                               uint32_t n) {
+  __shared__ unsigned long long wsum[kScanBlock / kWave];
+  __shared__ unsigned long long carry;
+  const uint32_t tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < n; base += 4 * kScanBlock) {
+    unsigned long long v[4], loc = 0;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const uint32_t i = base + tid * 4 + e;
+      v[e] = i < n ? in[i] : 0;
+      loc += v[e];
+    }
+    unsigned long long incl = loc;
+    for (int off = 1; off < kWave; off <<= 1) {
+      const unsigned long long o = __shfl_up(incl, off, kWave);
+      if (static_cast<int>(lane) >= off) incl += o;
+    }
+    if (lane == kWave - 1) wsum[wave] = incl;
+    __syncthreads();
+    unsigned long long wbase = 0, total = 0;
+    for (uint32_t w = 0; w < kScanBlock / kWave; w++) {
+      wbase += w < wave ? wsum[w] : 0;
+      total += wsum[w];
+    }
+    unsigned long long run = carry + wbase + incl - loc;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const uint32_t i = base + tid * 4 + e;
+      if (i < n) out[i] = run;
+      run += v[e];
+    }
+    __syncthreads();
+    if (tid == 0) carry += total;
+    __syncthreads();
+  }
+  if (tid == 0) out[n] = carry;
+}
 
 template <typename T, typename IS, bool RP, bool W>
 Status launch_rounds(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
@@ -323,6 +366,7 @@ Status launch_rounds(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
   P.offsets = a.offsets;
   P.caps = a.caps;
   P.cap_all = a.cap_all;
+  P.out_total = a.out_total ? a.out_total : ~0ull;
   P.out = a.out;
   P.err = sc.ctr + 2 * kMaxRounds + 2;
   P.dbg = a.debug;
@@ -457,5 +501,10 @@ Status run_search(const SearchArgs& a_in, SearchScratch& sc, hipStream_t s) {
 }
 
 unsigned long long* search_error_flag(SearchScratch& sc) { return sc.ctr + 2 * kMaxRounds + 2; }
+
+Status exclusive_scan(const unsigned long long* in, unsigned long long* out, size_t n, hipStream_t s) {
+  k_exclusive_scan<<<1, kScanBlock, 0, s>>>(in, out, static_cast<uint32_t>(n));
+  return hip_status(hipGetLastError(), "Error in launching offset scan");
+}
 
 }  // namespace pst

@@ -33,7 +33,7 @@ EXPORTED = [
     "pst_gpu_num_devs", "pst_gpu_three_sided_search", "pst_gpu_two_sided_left_search",
     "pst_gpu_two_sided_right_search", "pst_gpu_search_into", "pst_gpu_calc_global_mem_needed",
     "pst_gpu_export_tree", "pst_gpu_export_bytes", "pst_gpu_batch_count", "pst_gpu_batch_write",
-    "pst_gpu_set_search_tuning", "pst_interval_parallel_search",
+    "pst_gpu_batch_search", "pst_gpu_set_search_tuning", "pst_interval_parallel_search",
     "pst_interval_parallel_search_into",
 ]
 
@@ -96,6 +96,7 @@ def lib():
         L.pst_gpu_export_tree.argtypes = [vp, vp, sz, c.POINTER(sz)]
         L.pst_gpu_batch_count.argtypes = [vp, vp, sz, vp, vp]
         L.pst_gpu_batch_write.argtypes = [vp, vp, sz, i32, vp, vp, vp, vp]
+        L.pst_gpu_batch_search.argtypes = [vp, vp, sz, i32, vp, vp, vp, sz, vp]
         L.pst_gpu_set_search_tuning.argtypes = [vp, i32, i32, i32, i32]
         L.pst_interval_parallel_search.argtypes = [vp, sz, i32, i32, i32, c.POINTER(vp),
                                                    c.POINTER(sz), vp, i32, i32, i32,
@@ -241,6 +242,14 @@ class StaticPSTGPU:
         _check(lib().pst_gpu_batch_write(self.h, queries_d.data_ptr(), nq, rp, offsets_d.data_ptr(),
                                          caps_d.data_ptr() if caps_d is not None else None,
                                          out_d.data_ptr(), _stream_ptr(stream)))
+
+    def batch_search(self, queries_d, nq, counts_d, offsets_d, out_d, out_capacity, ret=RET_ID,
+                     stream=None):
+        """Count + scan + write, asynchronous; offsets_d has nq + 1 entries."""
+        _, rp = self._rec(ret)
+        _check(lib().pst_gpu_batch_search(self.h, queries_d.data_ptr(), nq, rp, counts_d.data_ptr(),
+                                          offsets_d.data_ptr(), out_d.data_ptr(), out_capacity,
+                                          _stream_ptr(stream)))
 
     def set_tuning(self, rounds=0, budget_small=0, budget_large=0, waves_per_cu=-1):
         _check(lib().pst_gpu_set_search_tuning(self.h, rounds, budget_small, budget_large, waves_per_cu))

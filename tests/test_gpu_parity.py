@@ -141,11 +141,11 @@ def test_search_into_capacity_and_preallocated(torch, P):
     tr.close()
 
 
-def test_empty_tree(torch, P, capsys):
+def test_empty_tree(torch, P, capfd):
     tr = P.StaticPSTGPU(0, 0, "f32", "u32")
     r = tr.twoSidedLeftSearch(0.5, 0.5)
     assert len(r) == 0
-    assert "Tree is empty; nothing to search" in capsys.readouterr().out
+    assert "Tree is empty; nothing to search" in capfd.readouterr().out
     tr.close()
 
 
