@@ -45,8 +45,12 @@ def main():
     ap.add_argument("--nq3", type=int, default=1024)
     ap.add_argument("--plain", type=int, default=256, help="queries timed on the plain path")
     ap.add_argument("--ips", type=int, default=64, help="queries timed for interval parallel search")
-    ap.add_argument("--cpu-sample", type=int, default=96, help="queries in the CPU baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=512, help="queries in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sort-queries", type=int, default=1,
+                    help="hand the batch to the library ordered by min_dim1/max_dim1 (locality)")
+    ap.add_argument("--profile-mode", action="store_true",
+                    help="build + one batched step + a 1 GiB copy (for counter scaling), no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -82,6 +86,11 @@ def main():
     # ---- queries
     qs = D.float_query_mix(args.nq3, args.nq2, seed=0)
     qs["kind"][:args.nq2] = 1  # two-sided left stabbing (q, q)
+    if args.sort_queries:
+        # order each kind by the bound that fixes where in dim1 its search runs
+        for lo, hi in ((0, args.nq2), (args.nq2, args.nq2 + args.nq3)):
+            key = qs["max_dim1"][lo:hi] if lo == 0 else qs["min_dim1"][lo:hi]
+            qs[lo:hi] = qs[lo:hi][np.argsort(key, kind="stable")]
     nq = len(qs)
     d_q = P.to_device(qs, dev)
     counts = torch.zeros(nq, dtype=torch.int64, device=dev)
@@ -104,6 +113,18 @@ def main():
         else:
             want = int(np.count_nonzero((d1 >= a) & (d1 <= b) & (d2 >= c)))
         ok &= want == int(host_counts[k])
+
+    if args.profile_mode:
+        tree.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
+        torch.cuda.synchronize()
+        # plain copy of a known size: the scale for HBM byte counters
+        x = torch.ones(1 << 28, dtype=torch.int32, device=dev)
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"[bench] profile mode done: {total} IDs, copy of {x.numel() * 4} bytes")
+        tree.close()
+        return
 
     # ---- timed steps: batched count + scan + write
     for _ in range(args.warmup):

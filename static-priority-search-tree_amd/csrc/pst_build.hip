@@ -165,10 +165,9 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_argmin(
 
 template <typename T, typename IS>
 __device__ inline void write_node(KV<T, IS>* kv, DM<T>* dm, uint64_t slot, const Point<T, IS>& p, T med) {
-  KV<T, IS> a;
-  a.dim2 = p.dim2_val;
-  if constexpr (has_id_v<IS>) a.id = p.id;
-  kv[slot] = a;
+  // children's keys are filled in by k_child_keys once every node exists
+  kv[slot].dim2 = p.dim2_val;
+  if constexpr (has_id_v<IS>) kv[slot].id = p.id;
   DM<T> b;
   b.dim1 = p.dim1_val;
   b.med = med;
@@ -331,6 +330,18 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish(
   __syncthreads();
   lds_build<T, IS>(bufA, bufB, sres, wred, static_cast<uint32_t>(m), (1ull << t0) - 1 + j, pts,
                    GlobalPerm{perm2}, kv, dm);
+}
+
+// Copies each child's dim2 into its parent's record (0 where a child is
+// absent; walks test existence from the shape before using the key).
+template <typename T, typename IS>
+__global__ void k_child_keys(KV<T, IS>* __restrict__ kv, uint64_t S, uint64_t M) {
+  const uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const int64_t m = subtree_size(M, s);
+  if (m < 1) return;
+  kv[s].cdim2[0] = m >= 2 ? kv[2 * s + 1].dim2 : T(0);
+  kv[s].cdim2[1] = m >= 3 ? kv[2 * s + 2].dim2 : T(0);
 }
 
 // Reference-layout export: dim1[S] dim2[S] med[S] ids[S] bitcodes[S].
@@ -516,6 +527,12 @@ Status build_typed(const void* pts_v, size_t n64, void* kv_v, void* dm_v, hipStr
   mark(2);
   k_finish<T, IS><<<1u << t0, kFinishBlock, 0, s>>>(Acur, t0, M, starts + (1u << t0) - 1,
                                                     perm2b.as<uint32_t>(), ptsS, kv, dm);
+  {
+    uint64_t S = 1;
+    while (S <= n64) S <<= 1;
+    S -= 1;
+    k_child_keys<T, IS><<<grid_for(S, 256), 256, 0, s>>>(kv, S, M);
+  }
   mark(3);
   PST_TRY(hipGetLastError(), "Error in launching construction kernels");
   int hflag = 0;

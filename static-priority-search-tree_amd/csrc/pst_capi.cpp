@@ -184,11 +184,12 @@ Status hip_status(hipError_t e, const std::string& what) {
 
 size_t dtype_size(int dt) { return (dt == PST_F64 || dt == PST_I64 || dt == PST_U64) ? 8 : 4; }
 
+// sizeof(KV<T, IdStore>): { T dim2; T child_dim2[2]; [IDType id]; }
 size_t kv_elem_size(int dt, int idt) {
   const size_t a = dtype_size(dt), b = id_size(idt);
-  if (b == 0) return a;
+  if (b == 0) return 3 * a;
   const size_t al = a > b ? a : b;
-  size_t off = ((a + b - 1) / b) * b;
+  const size_t off = ((3 * a + b - 1) / b) * b;
   return ((off + b + al - 1) / al) * al;
 }
 size_t dm_elem_size(int dt) { return 2 * dtype_size(dt); }

@@ -1,9 +1,12 @@
 // Shared definitions for the MI355X (gfx950) static priority search tree.
 //
 // Device-side tree layout (one tree, S = 2^ceil(log2(n+1)) - 1 slots):
-//   kv[S] : { T dim2; IDType id; }   prune key + payload, read by every visit
-//   dm[S] : { T dim1; T med;   }     read only on the O(log n) search paths
-//                                    and when whole points are returned
+//   kv[S] : { T dim2; T child_dim2[2]; IDType id; }
+//           read by every visit: the node's prune key, its children's prune
+//           keys (so a walk pushes only children that survive the dim2
+//           bound and never loads a pruned node) and the payload
+//   dm[S] : { T dim1; T med; }        read only on the O(log n) search paths
+//                                     and when whole points are returned
 // Children of slot s sit at 2s+1 and 2s+2 (pst/gpu-tree-node.h:11-13 of the
 // reference). No bitcode array is stored: the tree shape depends on n only
 // (a node holding m points gives floor(m/2) to its left child and
@@ -34,11 +37,13 @@ struct NoId {};
 template <typename T, typename I>
 struct KV {
   T dim2;
+  T cdim2[2];  // dim2 of the left and right child (unused when absent)
   I id;
 };
 template <typename T>
 struct KV<T, NoId> {
   T dim2;
+  T cdim2[2];
 };
 template <typename T>
 struct DM {

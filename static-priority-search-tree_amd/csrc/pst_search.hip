@@ -24,8 +24,10 @@
 //     remaining stack to a shared list that the next round spreads over all
 //     waves. The last round runs every stack to empty.
 // Whether a child exists follows from n alone (pst_common.hpp), so no
-// bitcode is read; a REPORT_ABOVE node needs no dim1/med load because every
-// node below a REPORT_ABOVE split already satisfies the dim1 bounds.
+// bitcode is read, and each node record carries its children's dim2, so
+// pruned children are never pushed or loaded: every visit is a live node. A
+// REPORT_ABOVE node needs no dim1/med load because every node below a
+// REPORT_ABOVE split already satisfies the dim1 bounds.
 #include <algorithm>
 
 #include "pst_internal.hpp"
@@ -207,8 +209,11 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
         a = P.kv[slot];
         if (RET_POINT || code != REPORT_ABOVE) b = P.dm[slot];
         if (Q.min_dim2 <= a.dim2) {
+          // a child is pushed only if it exists and survives the dim2 bound,
+          // which its parent's record tells without loading the child
           const int64_t m = subtree_size(P.M, slot);
-          const bool hl = m >= 2, hr = m >= 3;
+          const bool hl = m >= 2 && Q.min_dim2 <= a.cdim2[0];
+          const bool hr = m >= 3 && Q.min_dim2 <= a.cdim2[1];
           if (code == REPORT_ABOVE) {
             rep = true;
             cl = hl;

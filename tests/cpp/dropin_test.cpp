@@ -1,0 +1,134 @@
+// Drop-in check: the reference's class/function names and signatures
+// (StaticPSTGPU, intervalParallelSearch) used exactly as its testers use them
+// (testers/pst-tester-functions.tu:513-520, 560-575, 685-736), against a
+// brute-force scan. Prints "DROPIN OK" on success, exits non-zero otherwise.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <random>
+#include <sstream>
+#include <vector>
+
+#include "interval-parallel-search.h"
+#include "pst-point.h"
+#include "static-pst-gpu.h"
+
+static int failures = 0;
+#define EXPECT(c, ...)                \
+  do {                                \
+    if (!(c)) {                       \
+      std::printf("FAIL: " __VA_ARGS__); \
+      std::printf("\n");              \
+      failures++;                     \
+    }                                 \
+  } while (0)
+
+template <typename T, typename P>
+std::vector<P> to_host(P* d, size_t k) {
+  std::vector<P> h(k);
+  if (k) (void)hipMemcpy(h.data(), d, k * sizeof(P), hipMemcpyDeviceToHost);
+  return h;
+}
+
+template <typename T>
+void run_with_ids(size_t n, unsigned seed) {
+  using Pt = PointStruct<T, unsigned, 1>;
+  std::mt19937 rng(seed);
+  std::uniform_int_distribution<int> u(0, 50);
+  std::vector<Pt> pts(n);
+  for (size_t i = 0; i < n; i++) {
+    T a = static_cast<T>(u(rng)), w = static_cast<T>(u(rng) % 7);
+    pts[i] = Pt(a, a + w, static_cast<unsigned>(rng() % 1000));
+  }
+  Pt* d = nullptr;
+  (void)hipMalloc(&d, n * sizeof(Pt));
+  (void)hipMemcpy(d, pts.data(), n * sizeof(Pt), hipMemcpyHostToDevice);
+  hipDeviceProp_t props{};
+  (void)hipGetDeviceProperties(&props, 0);
+  StaticPSTGPU<T, PointStruct, unsigned, 1> tree(d, n, 1, 0, 1, props);
+  for (int q = 0; q < 20; q++) {
+    T a = static_cast<T>(u(rng)), b = a + static_cast<T>(u(rng) % 20), c = static_cast<T>(u(rng));
+    size_t k = 0;
+    unsigned* ids = nullptr;
+    tree.template threeSidedSearch<unsigned>(k, ids, a, b, c);
+    std::vector<unsigned> got = to_host<T>(ids, k), want;
+    (void)hipFree(ids);
+    for (auto& p : pts)
+      if (a <= p.dim1_val && p.dim1_val <= b && c <= p.dim2_val) want.push_back(p.id);
+    std::sort(got.begin(), got.end());
+    std::sort(want.begin(), want.end());
+    EXPECT(got == want, "threeSidedSearch n=%zu q=%d", n, q);
+
+    Pt* res = nullptr;
+    tree.twoSidedLeftSearch(k, res, b, c);  // RetType defaults to the point type
+    std::vector<Pt> gp = to_host<T>(res, k);
+    (void)hipFree(res);
+    size_t wl = 0;
+    for (auto& p : pts) wl += (p.dim1_val <= b && c <= p.dim2_val);
+    EXPECT(gp.size() == wl, "twoSidedLeftSearch count n=%zu", n);
+    for (auto& p : gp) EXPECT(p.dim1_val <= b && c <= p.dim2_val, "twoSidedLeftSearch predicate");
+
+    tree.template twoSidedRightSearch<unsigned>(k, ids, a, c);
+    got = to_host<T>(ids, k);
+    (void)hipFree(ids);
+    want.clear();
+    for (auto& p : pts)
+      if (a <= p.dim1_val && c <= p.dim2_val) want.push_back(p.id);
+    std::sort(got.begin(), got.end());
+    std::sort(want.begin(), want.end());
+    EXPECT(got == want, "twoSidedRightSearch n=%zu", n);
+
+    unsigned* ips = nullptr;
+    intervalParallelSearch<T, PointStruct, unsigned, 1, unsigned>(d, n, ips, k, a, 0, 1, 64, 0, 256);
+    got = to_host<T>(ips, k);
+    (void)hipFree(ips);
+    want.clear();
+    for (auto& p : pts)
+      if (p.dim1_val <= a && a <= p.dim2_val) want.push_back(p.id);
+    std::sort(got.begin(), got.end());
+    std::sort(want.begin(), want.end());
+    EXPECT(got == want, "intervalParallelSearch n=%zu", n);
+  }
+  EXPECT(tree.calcGlobalMemNeeded(n) > 0, "calcGlobalMemNeeded");
+  (void)hipFree(d);
+}
+
+void run_no_ids() {
+  using Pt = PointStruct<double, void, 0>;
+  std::vector<Pt> pts;
+  for (int i = 0; i < 9; i++) pts.emplace_back(0.5 * (i % 4), 0.5 * (i % 4) + 1.0);
+  Pt* d = nullptr;
+  (void)hipMalloc(&d, pts.size() * sizeof(Pt));
+  (void)hipMemcpy(d, pts.data(), pts.size() * sizeof(Pt), hipMemcpyHostToDevice);
+  StaticPSTGPU<double, PointStruct> tree(d, pts.size());
+  size_t k = 0;
+  Pt* res = nullptr;
+  tree.threeSidedSearch(k, res, 0.5, 1.0, 1.2);
+  size_t want = 0;
+  for (auto& p : pts) want += (0.5 <= p.dim1_val && p.dim1_val <= 1.0 && 1.2 <= p.dim2_val);
+  EXPECT(k == want, "no-ID threeSidedSearch count %zu vs %zu", k, want);
+  (void)hipFree(res);
+  std::ostringstream os;
+  os << tree;
+  EXPECT(os.str().find("├─(R)─ ") != std::string::npos, "print draws the tree");
+  (void)hipFree(d);
+  // empty tree: (0, nullptr) and the reference's message
+  StaticPSTGPU<double, PointStruct> empty(nullptr, 0);
+  empty.threeSidedSearch(k, res, 0.0, 1.0, 0.0);
+  EXPECT(k == 0 && res == nullptr, "empty tree");
+}
+
+int main() {
+  run_with_ids<float>(1000, 1);
+  run_with_ids<float>(5000, 2);
+  run_with_ids<int>(3000, 3);
+  run_with_ids<double>(2049, 4);
+  run_no_ids();
+  if (failures) {
+    std::printf("DROPIN FAILED (%d)\n", failures);
+    return 1;
+  }
+  std::printf("DROPIN OK\n");
+  return 0;
+}
