@@ -47,8 +47,11 @@ def main():
     ap.add_argument("--ips", type=int, default=64, help="queries timed for interval parallel search")
     ap.add_argument("--cpu-sample", type=int, default=512, help="queries in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sort-queries", type=int, default=1,
-                    help="hand the batch to the library ordered by min_dim1/max_dim1 (locality)")
+    ap.add_argument("--sort-queries", type=int, default=0,
+                    help="hand the batch to the library pre-ordered by min_dim1/max_dim1 "
+                         "(the library groups queries itself)")
+    ap.add_argument("--ungrouped", type=int, default=1,
+                    help="also time the batch with one walk per query (grouping off)")
     ap.add_argument("--profile-mode", action="store_true",
                     help="build + one batched step + a 1 GiB copy (for counter scaling), no JSON")
     args = ap.parse_args()
@@ -159,6 +162,20 @@ def main():
     achieved = alg_bytes / (t_write * 1e-3) / 1e9
     log(f"[bench] count pass {t_count:.2f} ms, write pass {t_write:.2f} ms, "
         f"write-pass alg {achieved:.0f} GB/s")
+
+    # ---- the same batch with one walk per query (grouping off), for comparison
+    ungrouped_qps = None
+    if args.ungrouped:
+        tree.set_grouping(1)
+        tree.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tree.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
+        torch.cuda.synchronize()
+        ungrouped_qps = nq / (time.perf_counter() - t0)
+        ok &= int(offsets[nq].item()) == total
+        tree.set_grouping(8)
+        log(f"[bench] ungrouped batch {ungrouped_qps:.0f} q/s")
 
     # ---- plain path: one blocking search per query, preallocated output
     np_plain = min(args.plain, nq)
@@ -272,6 +289,7 @@ def main():
         "cpu_baseline": cpu,
         "plain_per_query_qps": round(plain_qps, 2),
         "batched_over_plain": round(qps / plain_qps, 2),
+        "ungrouped_batch_qps": None if ungrouped_qps is None else round(ungrouped_qps, 2),
         "right_qq_qps": round(len(rq) / right_s, 2),
         "right_qq_ids_per_query": rtotal // len(rq),
         "ips_qps": round(1e3 / ips_ms, 2),

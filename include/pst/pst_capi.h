@@ -173,6 +173,11 @@ int pst_gpu_batch_search(pst_tree* tree, const void* queries_d, size_t num_queri
 /* Tuning knobs of the search rounds (0 keeps the default). */
 int pst_gpu_set_search_tuning(pst_tree* tree, int rounds, int budget_small, int budget_large,
                               int waves_per_cu);
+/* Queries per tree walk in batched searches: 8 (default; a batch is sorted
+ * by leading dim1 bound and each walk serves 8 neighbouring queries, with
+ * each query's results exactly those of its own walk) or 1 (one walk per
+ * query). No counterpart in the reference. */
+int pst_gpu_set_query_grouping(pst_tree* tree, int queries_per_walk);
 
 /* ------------------------------------------------------------------ */
 /* Interval parallel search (interval-parallel-search/                 */

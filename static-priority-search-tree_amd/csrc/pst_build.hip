@@ -164,14 +164,13 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_argmin(
 }
 
 template <typename T, typename IS>
-__device__ inline void write_node(KV<T, IS>* kv, DM<T>* dm, uint64_t slot, const Point<T, IS>& p, T med) {
-  // children's keys are filled in by k_child_keys once every node exists
-  kv[slot].dim2 = p.dim2_val;
-  if constexpr (has_id_v<IS>) kv[slot].id = p.id;
-  DM<T> b;
-  b.dim1 = p.dim1_val;
-  b.med = med;
-  dm[slot] = b;
+__device__ inline void write_node(Node<T, IS>* nodes, uint64_t slot, const Point<T, IS>& p, T med) {
+  Node<T, IS> r;
+  r.dim1 = p.dim1_val;
+  r.dim2 = p.dim2_val;
+  r.med = med;
+  if constexpr (has_id_v<IS>) r.id = p.id;
+  nodes[slot] = r;
 }
 
 // Emits the level's nodes and compacts A into the next level: the point at
@@ -180,8 +179,7 @@ template <typename T, typename IS>
 __global__ __launch_bounds__(kLevelBlock) void k_level_emit(
     const uint32_t* __restrict__ A, uint32_t* __restrict__ Anext, uint32_t n_level, uint32_t t,
     uint64_t M, const uint32_t* __restrict__ starts, const unsigned long long* __restrict__ segmin,
-    const uint32_t* __restrict__ perm2, const Point<T, IS>* __restrict__ pts, KV<T, IS>* __restrict__ kv,
-    DM<T>* __restrict__ dm) {
+    const uint32_t* __restrict__ perm2, const Point<T, IS>* __restrict__ pts, Node<T, IS>* __restrict__ nodes) {
   const uint32_t nseg = 1u << t;
   __shared__ uint32_t seg0;
   const uint32_t tile = blockIdx.x * kLevelTile;
@@ -213,7 +211,7 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_emit(
         const uint32_t pm = q < L ? L : L - 1;
         med = pts[perm2[A[st + pm]]].dim1_val;
       }
-      write_node(kv, dm, (1ull << t) - 1 + j, p, med);
+      write_node(nodes, (1ull << t) - 1 + j, p, med);
     } else {
       Anext[st - j + x - (x > q ? 1u : 0u)] = A[P];
     }
@@ -226,8 +224,7 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_emit(
 template <typename T, typename IS, typename PermF>
 __device__ void lds_build(uint32_t* bufA, uint32_t* bufB, unsigned long long* sres,
                           unsigned long long* wred, uint32_t m, uint64_t root_slot,
-                          const Point<T, IS>* __restrict__ pts, PermF perm, KV<T, IS>* __restrict__ kv,
-                          DM<T>* __restrict__ dm) {
+                          const Point<T, IS>* __restrict__ pts, PermF perm, Node<T, IS>* __restrict__ nodes) {
   const uint32_t tid = threadIdx.x;
   const uint32_t nthr = blockDim.x;
   const uint64_t Mp = static_cast<uint64_t>(m) + 1;
@@ -267,7 +264,7 @@ __device__ void lds_build(uint32_t* bufA, uint32_t* bufB, unsigned long long* sr
           const Point<T, IS> p = pts[perm(static_cast<uint32_t>(mn >> 32))];
           T med = p.dim1_val;
           if (L >= 1) med = pts[perm(bufA[st + (q < L ? L : L - 1)])].dim1_val;
-          write_node(kv, dm, (root_slot + 1) * nsub - 1 + k, p, med);
+          write_node(nodes, (root_slot + 1) * nsub - 1 + k, p, med);
         }
         const uint32_t dst0 = static_cast<uint32_t>(sz > 1 ? level_start(Mp, u + 1, 2 * k) : 0);
         for (int64_t x = g; x < sz; x += G) {
@@ -291,7 +288,7 @@ __device__ void lds_build(uint32_t* bufA, uint32_t* bufB, unsigned long long* sr
         const Point<T, IS> p = pts[perm(static_cast<uint32_t>(best >> 32))];
         T med = p.dim1_val;
         if (L >= 1) med = pts[perm(bufA[st + (q < L ? L : L - 1)])].dim1_val;
-        write_node(kv, dm, (root_slot + 1) * nsub - 1 + k, p, med);
+        write_node(nodes, (root_slot + 1) * nsub - 1 + k, p, med);
         if (sz > 1) {
           const uint32_t dst0 = static_cast<uint32_t>(level_start(Mp, u + 1, 2 * k));
           for (int64_t x = 0; x < sz; x++) {
@@ -316,8 +313,7 @@ struct GlobalPerm {
 template <typename T, typename IS>
 __global__ __launch_bounds__(kFinishBlock) void k_finish(
     const uint32_t* __restrict__ A, uint32_t t0, uint64_t M, const uint32_t* __restrict__ starts,
-    const uint32_t* __restrict__ perm2, const Point<T, IS>* __restrict__ pts, KV<T, IS>* __restrict__ kv,
-    DM<T>* __restrict__ dm) {
+    const uint32_t* __restrict__ perm2, const Point<T, IS>* __restrict__ pts, Node<T, IS>* __restrict__ nodes) {
   __shared__ uint32_t bufA[kFinishTile];
   __shared__ uint32_t bufB[kFinishTile];
   __shared__ unsigned long long sres[kFinishTile];
@@ -329,24 +325,12 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish(
   for (uint32_t x = threadIdx.x; x < m; x += blockDim.x) bufA[x] = A[st + x];
   __syncthreads();
   lds_build<T, IS>(bufA, bufB, sres, wred, static_cast<uint32_t>(m), (1ull << t0) - 1 + j, pts,
-                   GlobalPerm{perm2}, kv, dm);
-}
-
-// Copies each child's dim2 into its parent's record (0 where a child is
-// absent; walks test existence from the shape before using the key).
-template <typename T, typename IS>
-__global__ void k_child_keys(KV<T, IS>* __restrict__ kv, uint64_t S, uint64_t M) {
-  const uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (s >= S) return;
-  const int64_t m = subtree_size(M, s);
-  if (m < 1) return;
-  kv[s].cdim2[0] = m >= 2 ? kv[2 * s + 1].dim2 : T(0);
-  kv[s].cdim2[1] = m >= 3 ? kv[2 * s + 2].dim2 : T(0);
+                   GlobalPerm{perm2}, nodes);
 }
 
 // Reference-layout export: dim1[S] dim2[S] med[S] ids[S] bitcodes[S].
 template <typename T, typename IS>
-__global__ void k_export(const KV<T, IS>* __restrict__ kv, const DM<T>* __restrict__ dm, uint64_t S,
+__global__ void k_export(const Node<T, IS>* __restrict__ nodes, uint64_t S,
                          uint64_t M, unsigned char* __restrict__ blob, size_t o1, size_t o2, size_t om,
                          size_t oi, size_t ob) {
   const uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -355,9 +339,9 @@ __global__ void k_export(const KV<T, IS>* __restrict__ kv, const DM<T>* __restri
   T d1 = T(0), d2 = T(0), md = T(0);
   unsigned char bc = 0;
   if (m >= 1) {
-    d1 = dm[s].dim1;
-    md = dm[s].med;
-    d2 = kv[s].dim2;
+    d1 = nodes[s].dim1;
+    md = nodes[s].med;
+    d2 = nodes[s].dim2;
     bc = static_cast<unsigned char>((m >= 2 ? 2 : 0) | (m >= 3 ? 1 : 0));
   }
   reinterpret_cast<T*>(blob + o1)[s] = d1;
@@ -365,7 +349,7 @@ __global__ void k_export(const KV<T, IS>* __restrict__ kv, const DM<T>* __restri
   reinterpret_cast<T*>(blob + om)[s] = md;
   if constexpr (has_id_v<IS>) {
     IS id = 0;
-    if (m >= 1) id = kv[s].id;
+    if (m >= 1) id = nodes[s].id;
     reinterpret_cast<IS*>(blob + oi)[s] = id;
   }
   blob[ob + s] = bc;
@@ -389,7 +373,7 @@ struct DevBuf {
 inline unsigned grid_for(size_t n, unsigned block) { return static_cast<unsigned>((n + block - 1) / block); }
 
 template <typename T, typename I>
-Status build_typed(const void* pts_v, size_t n64, void* kv_v, void* dm_v, hipStream_t s, BuildTimes* times) {
+Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, BuildTimes* times) {
   using IS = typename IdStore<I>::type;
   Status st;
   if (n64 == 0) return st;
@@ -402,8 +386,7 @@ Status build_typed(const void* pts_v, size_t n64, void* kv_v, void* dm_v, hipStr
   const uint64_t M = n64 + 1;
   const auto* pts = static_cast<const Point<T, I>*>(pts_v);
   const auto* ptsS = static_cast<const Point<T, IS>*>(pts_v);
-  auto* kv = static_cast<KV<T, IS>*>(kv_v);
-  auto* dm = static_cast<DM<T>*>(dm_v);
+  auto* nodes = static_cast<Node<T, IS>*>(nodes_v);
   constexpr bool kHasId = has_id_v<I>;
   constexpr bool kSingle = !kHasId ? true : (sizeof(T) == 4 && sizeof(I) == 4);
   using K = typename std::conditional<(!kHasId && sizeof(T) == 4), uint32_t, uint64_t>::type;
@@ -521,18 +504,12 @@ Status build_typed(const void* pts_v, size_t n64, void* kv_v, void* dm_v, hipStr
                                                  segmin + (1u << t) - 1);
     k_level_emit<T, IS><<<tiles, kLevelBlock, 0, s>>>(Acur, Anxt, n_level, t, M, starts + (1u << t) - 1,
                                                       segmin + (1u << t) - 1, perm2b.as<uint32_t>(), ptsS,
-                                                      kv, dm);
+                                                      nodes);
     std::swap(Acur, Anxt);
   }
   mark(2);
   k_finish<T, IS><<<1u << t0, kFinishBlock, 0, s>>>(Acur, t0, M, starts + (1u << t0) - 1,
-                                                    perm2b.as<uint32_t>(), ptsS, kv, dm);
-  {
-    uint64_t S = 1;
-    while (S <= n64) S <<= 1;
-    S -= 1;
-    k_child_keys<T, IS><<<grid_for(S, 256), 256, 0, s>>>(kv, S, M);
-  }
+                                                    perm2b.as<uint32_t>(), ptsS, nodes);
   mark(3);
   PST_TRY(hipGetLastError(), "Error in launching construction kernels");
   int hflag = 0;
@@ -554,14 +531,13 @@ Status build_typed(const void* pts_v, size_t n64, void* kv_v, void* dm_v, hipStr
 }
 
 template <typename T, typename IS>
-Status export_typed(size_t n, const void* kv, const void* dm, void* blob, size_t o1, size_t o2, size_t om,
+Status export_typed(size_t n, const void* nodes, void* blob, size_t o1, size_t o2, size_t om,
                     size_t oi, size_t ob, hipStream_t s) {
   if (n == 0) return Status{};
   uint64_t S = 1;
   while (S <= n) S <<= 1;
   S -= 1;
-  k_export<T, IS><<<grid_for(S, 256), 256, 0, s>>>(static_cast<const KV<T, IS>*>(kv),
-                                                    static_cast<const DM<T>*>(dm), S, n + 1,
+  k_export<T, IS><<<grid_for(S, 256), 256, 0, s>>>(static_cast<const Node<T, IS>*>(nodes), S, n + 1,
                                                     static_cast<unsigned char*>(blob), o1, o2, om, oi, ob);
   return hip_status(hipGetLastError(), "Error in launching tree export");
 }
@@ -592,21 +568,21 @@ Status with_id(int idt, F&& f) {
 
 }  // namespace
 
-Status build_tree(int dtype, int id_dtype, const void* pts_d, size_t n, void* kv_d, void* dm_d,
-                  hipStream_t stream, BuildTimes* times) {
+Status build_tree(int dtype, int id_dtype, const void* pts_d, size_t n, void* nodes_d, hipStream_t stream,
+                  BuildTimes* times) {
   return with_t(dtype, [&](auto t) {
     return with_id(id_dtype, [&](auto i) {
-      return build_typed<decltype(t), decltype(i)>(pts_d, n, kv_d, dm_d, stream, times);
+      return build_typed<decltype(t), decltype(i)>(pts_d, n, nodes_d, stream, times);
     });
   });
 }
 
-Status export_tree(int dtype, int id_dtype, size_t n, const void* kv_d, const void* dm_d, void* blob_d,
+Status export_tree(int dtype, int id_dtype, size_t n, const void* nodes_d, void* blob_d,
                    size_t o1, size_t o2, size_t om, size_t oi, size_t ob, hipStream_t stream) {
   return with_t(dtype, [&](auto t) {
     return with_id(id_dtype, [&](auto i) {
       using IS = typename IdStore<decltype(i)>::type;
-      return export_typed<decltype(t), IS>(n, kv_d, dm_d, blob_d, o1, o2, om, oi, ob, stream);
+      return export_typed<decltype(t), IS>(n, nodes_d, blob_d, o1, o2, om, oi, ob, stream);
     });
   });
 }

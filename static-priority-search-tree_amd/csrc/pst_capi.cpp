@@ -17,10 +17,10 @@ struct pst_tree {
   int dtype = 0, id_dtype = -1;
   size_t n = 0, S = 0;
   int dev_ind = 0, num_devs = 1, device = 0;
-  void* kv = nullptr;
-  void* dm = nullptr;
+  void* nodes = nullptr;
   pst::SearchScratch sc;
   int rounds = 4, budget_small = 0, budget_large = 0, waves_per_cu = 0;
+  int group = pst::kGroup;  // queries per walk in batched searches
   hipStream_t stream = nullptr;
   unsigned long long* debug = nullptr;  // per-round search counters (debug builds)
 };
@@ -97,12 +97,12 @@ pst::SearchArgs base_args(pst_tree* t) {
   a.dtype = t->dtype;
   a.id_dtype = t->id_dtype;
   a.n = t->n;
-  a.kv = t->kv;
-  a.dm = t->dm;
+  a.nodes = t->nodes;
   a.rounds = t->rounds;
   a.budget_small = t->budget_small;
   a.budget_large = t->budget_large;
   a.waves_per_cu = t->waves_per_cu;
+  a.group = t->group;
   a.debug = t->debug;
   return a;
 }
@@ -184,15 +184,14 @@ Status hip_status(hipError_t e, const std::string& what) {
 
 size_t dtype_size(int dt) { return (dt == PST_F64 || dt == PST_I64 || dt == PST_U64) ? 8 : 4; }
 
-// sizeof(KV<T, IdStore>): { T dim2; T child_dim2[2]; [IDType id]; }
-size_t kv_elem_size(int dt, int idt) {
+// sizeof(Node<T, IdStore>): { T dim1; T dim2; T med; [IDType id]; }
+size_t node_size(int dt, int idt) {
   const size_t a = dtype_size(dt), b = id_size(idt);
   if (b == 0) return 3 * a;
   const size_t al = a > b ? a : b;
   const size_t off = ((3 * a + b - 1) / b) * b;
   return ((off + b + al - 1) / al) * al;
 }
-size_t dm_elem_size(int dt) { return 2 * dtype_size(dt); }
 
 size_t point_size(int dt, int idt) {
   const size_t a = dtype_size(dt), b = id_size(idt);
@@ -244,15 +243,14 @@ int pst_gpu_create(const void* pt_arr_d, size_t n, int dtype, int id_dtype, int 
     *out = t;
     return PST_OK;
   }
-  hipError_t e = hipMalloc(&t->kv, t->S * pst::kv_elem_size(dtype, id_dtype));
-  if (e == hipSuccess) e = hipMalloc(&t->dm, t->S * pst::dm_elem_size(dtype));
+  hipError_t e = hipMalloc(&t->nodes, t->S * pst::node_size(dtype, id_dtype));
   if (e != hipSuccess) {
     std::string m = "Error in allocating priority search tree storage array" + dev_suffix(dev_ind, num_devs) +
                     hipGetErrorString(e);
     pst_gpu_destroy(t);
     return fail(PST_ERR_HIP, m);
   }
-  Status st = pst::build_tree(dtype, id_dtype, pt_arr_d, n, t->kv, t->dm, t->stream, nullptr);
+  Status st = pst::build_tree(dtype, id_dtype, pt_arr_d, n, t->nodes, t->stream, nullptr);
   if (!st.ok()) {
     pst_gpu_destroy(t);
     return fail(st);
@@ -264,11 +262,7 @@ int pst_gpu_create(const void* pt_arr_d, size_t n, int dtype, int id_dtype, int 
 int pst_gpu_destroy(pst_tree* t) {
   if (!t) return PST_OK;
   hipError_t e = hipSuccess;
-  if (t->kv) e = hipFree(t->kv);
-  if (t->dm) {
-    hipError_t e2 = hipFree(t->dm);
-    if (e == hipSuccess) e = e2;
-  }
+  if (t->nodes) e = hipFree(t->nodes);
   pst::scratch_release(t->sc);
   const int di = t->dev_ind, nd = t->num_devs;
   delete t;
@@ -311,12 +305,13 @@ int pst_gpu_search_into(pst_tree* t, int kind, int ret, const void* min1, const 
 size_t pst_gpu_calc_global_mem_needed(size_t n, int dtype, int id_dtype) {
   if (!valid_t(dtype) || !valid_id(id_dtype)) return 0;
   const size_t S = slots_for(n);
-  const size_t tree = S * (pst::kv_elem_size(dtype, id_dtype) + pst::dm_elem_size(dtype));
+  const size_t tree = S * pst::node_size(dtype, id_dtype);
   // construction: two key arrays, two 32-bit value arrays, order and rank
   // arrays, and radix-sort storage (about one more key array)
   const size_t key = (pst::dtype_size(dtype) == 4 && id_size(id_dtype) == 0) ? 4 : 8;
   const size_t construct = n * (3 * key + 4 * 4);
-  const size_t search = n * pst::point_size(dtype, id_dtype) + (2ull << 22) * 8;
+  // search: a result array of n points plus two 4M-entry work lists
+  const size_t search = n * pst::point_size(dtype, id_dtype) + (2ull << 22) * 16;
   return tree + (construct > search ? construct : search);
 }
 
@@ -346,7 +341,7 @@ int pst_gpu_export_tree(pst_tree* t, void* host_blob, size_t blob_bytes, size_t*
   void* d = nullptr;
   hipError_t e = hipMalloc(&d, need);
   if (e != hipSuccess) return fail(pst::hip_status(e, "Error in allocating tree export buffer"));
-  Status st = pst::export_tree(t->dtype, t->id_dtype, t->n, t->kv, t->dm, d, off6[0], off6[1], off6[2], off6[3],
+  Status st = pst::export_tree(t->dtype, t->id_dtype, t->n, t->nodes, d, off6[0], off6[1], off6[2], off6[3],
                                off6[4], t->stream);
   if (st.ok()) st = pst::hip_status(hipMemcpyAsync(host_blob, d, need, hipMemcpyDeviceToHost, t->stream),
                                     "Error in copying tree export");
@@ -426,6 +421,7 @@ int pst_gpu_batch_search(pst_tree* t, const void* queries_d, size_t nq, int ret,
   a.cap_all = ~0ull;
   a.out = out_d;
   a.out_total = out_capacity;
+  a.groups_ready = 1;  // the count pass above grouped this batch
   st = pst::run_search(a, t->sc, s);
   return st.ok() ? PST_OK : fail(st);
 }
@@ -436,6 +432,14 @@ int pst_gpu_set_search_tuning(pst_tree* t, int rounds, int budget_small, int bud
   if (budget_small > 0) t->budget_small = budget_small;
   if (budget_large > 0) t->budget_large = budget_large;
   if (waves_per_cu >= 0) t->waves_per_cu = waves_per_cu;
+  return PST_OK;
+}
+
+int pst_gpu_set_query_grouping(pst_tree* t, int queries_per_walk) {
+  if (!t) return fail(PST_ERR_INVALID, "Error: null tree");
+  if (queries_per_walk != 1 && queries_per_walk != pst::kGroup)
+    return fail(PST_ERR_INVALID, "Error: queries per walk must be 1 or " + std::to_string(pst::kGroup));
+  t->group = queries_per_walk;
   return PST_OK;
 }
 

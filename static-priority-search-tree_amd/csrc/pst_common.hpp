@@ -1,12 +1,10 @@
 // Shared definitions for the MI355X (gfx950) static priority search tree.
 //
 // Device-side tree layout (one tree, S = 2^ceil(log2(n+1)) - 1 slots):
-//   kv[S] : { T dim2; T child_dim2[2]; IDType id; }
-//           read by every visit: the node's prune key, its children's prune
-//           keys (so a walk pushes only children that survive the dim2
-//           bound and never loads a pruned node) and the payload
-//   dm[S] : { T dim1; T med; }        read only on the O(log n) search paths
-//                                     and when whole points are returned
+//   nodes[S] : { T dim1; T dim2; T med; IDType id; }
+//           one record per slot, read with one vector load per visit: a
+//           walk needs dim2 (prune key), dim1 and med (bounds and split) and
+//           the ID (payload) of every node it reports.
 // Children of slot s sit at 2s+1 and 2s+2 (pst/gpu-tree-node.h:11-13 of the
 // reference). No bitcode array is stored: the tree shape depends on n only
 // (a node holding m points gives floor(m/2) to its left child and
@@ -27,27 +25,22 @@ namespace pst {
 // dtype codes of the C-ABI (include/pst/pst_capi.h)
 enum DType : int { F32 = 0, F64 = 1, I32 = 2, I64 = 3, U32 = 4, U64 = 5, NO_ID = -1 };
 
-// Search codes (the reference's SearchCodes, pst/static-pst-gpu.h:408-414).
-enum Code : uint32_t { REPORT_ABOVE = 0, LEFT_SEARCH = 1, RIGHT_SEARCH = 2, THREE_SEARCH = 3 };
 // Query kinds of the C-ABI.
 enum Kind : uint32_t { KIND_THREE = 0, KIND_LEFT = 1, KIND_RIGHT = 2 };
 
 struct NoId {};
 
 template <typename T, typename I>
-struct KV {
+struct Node {
+  T dim1;
   T dim2;
-  T cdim2[2];  // dim2 of the left and right child (unused when absent)
+  T med;
   I id;
 };
 template <typename T>
-struct KV<T, NoId> {
-  T dim2;
-  T cdim2[2];
-};
-template <typename T>
-struct DM {
+struct Node<T, NoId> {
   T dim1;
+  T dim2;
   T med;
 };
 
@@ -171,6 +164,86 @@ __host__ __device__ inline int64_t level_start(uint64_t M, uint32_t t, uint32_t 
     }
   }
   return static_cast<int64_t>(j) * base + extra;
+}
+
+// ---------------------------------------------------------------------------
+// Search state of a query at a node, as bits j of three masks for the G
+// queries of a group: `a` the query is walking this subtree, `l` its lower
+// dim1 bound still cuts the subtree, `h` its upper bound still does. These
+// are the reference's search codes (pst/static-pst-gpu.h:408-414):
+// REPORT_ABOVE = a only, RIGHT_SEARCH = a+l, LEFT_SEARCH = a+h,
+// THREE_SEARCH = a+l+h. A closed bound holds for every point of the subtree,
+// because a left subtree's dim1 values are <= the parent's med and a right
+// subtree's are >= it.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Bounds {
+  T lo, hi, min2;
+};
+
+struct StepOut {
+  uint32_t rep;         // queries that report the node
+  uint32_t al, ll, hl;  // state of the left child (al == 0: not visited)
+  uint32_t ar, lr, hr;  // state of the right child
+};
+
+// One node visit for G queries. d1, d2, md: the node's dim1, dim2 and med;
+// m: its point count (left child exists iff m >= 2, right iff m >= 3).
+// Per query this is the reference's delegation, with its comparisons
+// (pst/static-pst-cpu-iter.tpp:600-735): a node with dim2 < min_dim2 ends
+// the query's walk below it; otherwise it is reported if its dim1 meets the
+// open bounds, and its children inherit
+//   REPORT_ABOVE -> both children REPORT_ABOVE;
+//   LEFT_SEARCH  -> med <= hi ? (left REPORT_ABOVE, right LEFT) : left LEFT;
+//   RIGHT_SEARCH -> med >= lo ? (left RIGHT, right REPORT_ABOVE) : right RIGHT;
+//   THREE_SEARCH -> lo <= med <= hi ? (left RIGHT, right LEFT)
+//                   : hi < med ? left THREE : med < lo ? right THREE : none.
+template <typename T, int G>
+__host__ __device__ inline StepOut node_step(T d1, T d2, T md, int64_t m, const Bounds<T>* qb, uint32_t a,
+                                             uint32_t l, uint32_t h) {
+  uint32_t ge2 = 0, d1lo = 0, d1hi = 0, mdhi = 0, mdlo = 0, himd = 0, mdltlo = 0;
+#pragma unroll
+  for (int j = 0; j < G; j++) {
+    const uint32_t bit = 1u << j;
+    ge2 |= (qb[j].min2 <= d2) ? bit : 0u;
+    d1lo |= (qb[j].lo <= d1) ? bit : 0u;
+    d1hi |= (d1 <= qb[j].hi) ? bit : 0u;
+    mdhi |= (md <= qb[j].hi) ? bit : 0u;
+    mdlo |= (md >= qb[j].lo) ? bit : 0u;
+    himd |= (qb[j].hi < md) ? bit : 0u;
+    mdltlo |= (md < qb[j].lo) ? bit : 0u;
+  }
+  const uint32_t act = a & ge2;
+  const uint32_t ra = act & ~l & ~h, lf = act & h & ~l, rt = act & l & ~h, th = act & l & h;
+  const uint32_t split = th & mdlo & mdhi;
+  const uint32_t tl = th & ~split & himd;
+  const uint32_t tr = th & ~split & ~himd & mdltlo;
+  StepOut o;
+  o.rep = act & (~l | d1lo) & (~h | d1hi);
+  o.al = ra | lf | (rt & mdlo) | split | tl;
+  o.ll = (rt & mdlo) | split | tl;
+  o.hl = (lf & ~mdhi) | tl;
+  o.ar = ra | (lf & mdhi) | rt | split | tr;
+  o.lr = (rt & ~mdlo) | tr;
+  o.hr = (lf & mdhi) | split | tr;
+  if (m < 2) o.al = o.ll = o.hl = 0;
+  if (m < 3) o.ar = o.lr = o.hr = 0;
+  return o;
+}
+
+// node_step for items with no open bound (every query REPORT_ABOVE): only
+// the dim2 prune applies, and both children inherit the surviving queries.
+template <typename T, int G>
+__host__ __device__ inline StepOut node_step_report_above(T d2, int64_t m, const Bounds<T>* qb, uint32_t a) {
+  uint32_t ge2 = 0;
+#pragma unroll
+  for (int j = 0; j < G; j++) ge2 |= (qb[j].min2 <= d2) ? (1u << j) : 0u;
+  StepOut o;
+  o.rep = a & ge2;
+  o.al = m >= 2 ? o.rep : 0u;
+  o.ar = m >= 3 ? o.rep : 0u;
+  o.ll = o.hl = o.lr = o.hr = 0;
+  return o;
 }
 
 // ---------------------------------------------------------------------------

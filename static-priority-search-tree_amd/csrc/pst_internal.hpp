@@ -26,8 +26,7 @@ struct Status {
 };
 
 // Sizes of one tree's device arrays.
-size_t kv_elem_size(int dtype, int id_dtype);
-size_t dm_elem_size(int dtype);
+size_t node_size(int dtype, int id_dtype);
 size_t point_size(int dtype, int id_dtype);
 size_t dtype_size(int dtype);
 
@@ -35,38 +34,32 @@ size_t dtype_size(int dtype);
 struct BuildTimes {
   float keys_ms = 0, sort_ms = 0, levels_ms = 0, finish_ms = 0;
 };
-// Builds the tree for n AoS points (device pointer) into kv/dm, which hold
-// S = 2^ceil(log2(n+1)) - 1 slots each. Returns PST_ERR_NAN if a NaN dim
+// Builds the tree for n AoS points (device pointer) into `nodes`, which
+// holds S = 2^ceil(log2(n+1)) - 1 records. Returns PST_ERR_NAN if a NaN dim
 // (or floating ID) is found.
-Status build_tree(int dtype, int id_dtype, const void* pts_d, size_t n, void* kv_d, void* dm_d,
-                  hipStream_t stream, BuildTimes* times);
+Status build_tree(int dtype, int id_dtype, const void* pts_d, size_t n, void* nodes_d, hipStream_t stream,
+                  BuildTimes* times);
 
 // Reference-layout blob (dim1, dim2, med, ids, bitcodes) written to a
 // device buffer, for export / parity checks.
-Status export_tree(int dtype, int id_dtype, size_t n, const void* kv_d, const void* dm_d,
-                   void* blob_d, size_t off_dim1, size_t off_dim2, size_t off_med, size_t off_id,
-                   size_t off_bits, hipStream_t stream);
-
-// ---- forest of shallow trees (pst_forest.hip) ----
-// Tree c covers input points [c*full, min(n, (c+1)*full)); full =
-// 2^ceil(log2(tpb+1)) - 1. Tree c's kv/dm arrays start at slot c*full.
-Status build_forest(int dtype, int id_dtype, const void* pts_d, size_t n, size_t full_slots,
-                    void* kv_d, void* dm_d, hipStream_t stream);
-Status forest_search(int dtype, int id_dtype, size_t n, size_t full_slots, const void* kv_d,
-                     const void* dm_d, int kind, int ret_point, const void* min1,
-                     const void* max1, const void* min2, void* out_d, size_t cap,
-                     unsigned long long* counter_d, hipStream_t stream);
+Status export_tree(int dtype, int id_dtype, size_t n, const void* nodes_d, void* blob_d, size_t off_dim1,
+                   size_t off_dim2, size_t off_med, size_t off_id, size_t off_bits, hipStream_t stream);
 
 // ---- searches on one deep tree (pst_search.hip) ----
+constexpr int kGroup = 8;  // queries per group walk in batched searches
 struct SearchScratch {
-  void* list[2] = {nullptr, nullptr};  // work lists (u64 items)
-  size_t list_cap = 0;                 // items per list
+  void* list[2] = {nullptr, nullptr};  // work lists (16-byte entries)
+  size_t list_cap = 0;                 // entries per list
   unsigned long long* ctr = nullptr;   // round counters
-  void* qbuf = nullptr;                // query records for single searches
-  size_t qbuf_bytes = 0;
   unsigned long long* cursor = nullptr;  // per-query output cursors
   size_t cursor_cap = 0;
-  int num_waves = 0;
+  uint32_t* groups = nullptr;  // group table: kGroup query indices per group
+  void* gkeys = nullptr;       // grouping sort keys and values
+  void* gtmp = nullptr;        // grouping sort temporary storage
+  size_t gtmp_bytes = 0;
+  size_t gcap = 0;  // queries the grouping storage holds (a multiple of kGroup)
+  void* gstack = nullptr;  // per-wave overflow stacks of the walks
+  size_t gstack_bytes = 0;
 };
 Status scratch_reserve(SearchScratch& sc, size_t nq, int dtype, hipStream_t stream);
 void scratch_release(SearchScratch& sc);
@@ -77,8 +70,7 @@ void scratch_release(SearchScratch& sc);
 struct SearchArgs {
   int dtype, id_dtype;
   size_t n;
-  const void* kv;
-  const void* dm;
+  const void* nodes;
   const void* queries;  // device Query<T>[nq]
   size_t nq;
   int mode;             // 0 count, 1 write
@@ -91,6 +83,8 @@ struct SearchArgs {
   int rounds;           // kernel launches enqueued (>= 1)
   int budget_small, budget_large;
   int waves_per_cu;
+  int group;            // 1: one query per walk; otherwise kGroup
+  int groups_ready;     // the group table in the scratch already holds this batch
   const void* single_query;  // host Query<T> for one query (queries ignored)
   unsigned long long* debug;  // optional per-round counters (8 per round)
   unsigned long long out_total;  // records in `out` (0: unlimited)

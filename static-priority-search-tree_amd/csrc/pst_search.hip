@@ -7,27 +7,26 @@
 // (pst/static-pst-gpu-search-functions.tu:12-722, pst/static-pst-gpu.tu:
 // 812-1148). Here every wavefront is an independent worker:
 //
-//   * work items are (query, slot, search code); a wave owns one query at a
-//     time and walks it depth first with a stack of (slot, code) words in
-//     LDS, popping up to 64 items per step (one per lane);
-//   * the delegation rules per code are the reference's
-//     (pst/static-pst-cpu-iter.tpp:600-735): REPORT_ABOVE reports every node
-//     with dim2 >= min_dim2 below it, LEFT/RIGHT split into REPORT_ABOVE
-//     plus one continuing child at the median, THREE splits into RIGHT +
-//     LEFT when the median falls in [min_dim1, max_dim1];
-//   * results are staged per wave in LDS and flushed with one atomic per
+//   * a batch's queries are sorted by their leading dim1 bound and cut into
+//     groups of G (8) neighbours; one walk serves a whole group. A work
+//     item is (group, slot, per-query state masks, node_step in
+//     pst_common.hpp): a node is loaded once for all queries of the group
+//     that still walk its subtree, and each query's report set is exactly
+//     the one its own walk would give. Neighbouring queries share most of
+//     their paths and reported subtrees, so one node load feeds several
+//     queries' outputs. Single searches use G = 1 and 32-bit items;
+//   * a wave owns one group at a time and walks it depth first with a stack
+//     of items in LDS, popping up to 64 items per step (one per lane);
+//   * results are staged per query in LDS and flushed with one atomic per
 //     flush on the query's output cursor (the reference's warp scan +
 //     atomicAdd, helpers/calc-alloc-report-ind-offset.h:17-44, becomes
-//     ballot/mbcnt within the wave and one atomic per ~200 results);
+//     ballot/mbcnt within the wave and one atomic per 64..256 results);
 //   * load balance: the launch is repeated for a few rounds; in all but the
 //     last, a wave that spent its step budget on one batch pushes its
 //     remaining stack to a shared list that the next round spreads over all
 //     waves. The last round runs every stack to empty.
 // Whether a child exists follows from n alone (pst_common.hpp), so no
-// bitcode is read, and each node record carries its children's dim2, so
-// pruned children are never pushed or loaded: every visit is a live node. A
-// REPORT_ABOVE node needs no dim1/med load because every node below a
-// REPORT_ABOVE split already satisfies the dim1 bounds.
+// bitcode is read.
 #include <algorithm>
 
 #include "pst_internal.hpp"
@@ -36,18 +35,61 @@ namespace pst {
 
 namespace {
 
-constexpr int kStageIds = 256;  // staged results per wave before a flush
-constexpr int kFlushAt = kStageIds - 2 * kWave;
 constexpr int kMaxRounds = 8;
 constexpr uint64_t kEmpty = ~0ull;
+// Per-wave LDS stack; deeper stacks move their oldest kMoveBlock items at a
+// time to a per-wave overflow stack in device memory (the walk pops the
+// newest 64 items per step, so the stack can grow by 64 per tree level).
+// One-query walks (32-bit items) get room for 30 levels in LDS: they are
+// latency bound and a refill from device memory stalls the only wave on a
+// subtree. Group walks (64-bit items) keep a small stack for occupancy.
+template <int G>
+constexpr int lds_stack() {
+  return G == 1 ? 2048 : 512;
+}
+constexpr int kMoveBlock = 256;
+
+// Spilled work item: the group it belongs to (kEmpty: no item) and the item.
+struct Entry {
+  uint64_t item;
+  uint64_t group;
+};
+
+// Work items. G == 1: slot << 2 | l | h << 1 (a query is active wherever an
+// item exists). G == 8: slot | a << 32 | l << 40 | h << 48.
+template <int G>
+struct Items;
+template <>
+struct Items<1> {
+  using type = uint32_t;
+  __device__ static type make(uint64_t slot, uint32_t, uint32_t l, uint32_t h) {
+    return static_cast<uint32_t>(slot << 2) | l | (h << 1);
+  }
+  __device__ static uint64_t slot(type it) { return it >> 2; }
+  __device__ static uint32_t a(type) { return 1u; }
+  __device__ static uint32_t l(type it) { return it & 1u; }
+  __device__ static uint32_t h(type it) { return (it >> 1) & 1u; }
+};
+template <>
+struct Items<8> {
+  using type = uint64_t;
+  __device__ static type make(uint64_t slot, uint32_t a, uint32_t l, uint32_t h) {
+    return slot | (static_cast<uint64_t>(a) << 32) | (static_cast<uint64_t>(l) << 40) |
+           (static_cast<uint64_t>(h) << 48);
+  }
+  __device__ static uint64_t slot(type it) { return it & 0xffffffffull; }
+  __device__ static uint32_t a(type it) { return static_cast<uint32_t>(it >> 32) & 0xffu; }
+  __device__ static uint32_t l(type it) { return static_cast<uint32_t>(it >> 40) & 0xffu; }
+  __device__ static uint32_t h(type it) { return static_cast<uint32_t>(it >> 48) & 0xffu; }
+};
 
 template <typename T, typename IS>
 struct Params {
-  const KV<T, IS>* kv;
-  const DM<T>* dm;
+  const Node<T, IS>* nodes;
   uint64_t M;  // n + 1
   const Query<T>* queries;
-  uint64_t* list[2];
+  const uint32_t* gq;  // G query indices per group (~0u: none); null when G == 1
+  Entry* list[2];
   unsigned long long list_cap;
   unsigned long long* ctr;  // [2r] items in list r, [2r+1] grab counter of round r
   unsigned long long* counts;  // count mode: totals; write mode: cursors
@@ -60,30 +102,63 @@ struct Params {
   unsigned long long* dbg;  // optional per-round counters (nullable)
   Query<T> single;          // the query when `queries` is null (one-query searches)
   uint64_t S;               // slot count: items outside [0, S) are rejected
-  uint32_t nq;              // query count: items of other queries are rejected
-  uint32_t stack_cap;
+  uint32_t nq;              // query count
+  uint32_t ngroups;         // groups: items of other groups are rejected
+  void* gstack;              // per-wave overflow stacks, gcap items each
+  uint32_t gcap;
   uint32_t total_waves;
   uint32_t budget_small, budget_large;
 };
 
 __device__ inline uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
-__device__ inline uint32_t start_code(uint32_t kind) {
-  return kind == KIND_THREE ? THREE_SEARCH : (kind == KIND_LEFT ? LEFT_SEARCH : RIGHT_SEARCH);
+template <typename V>
+__device__ inline V read_lane(V v, int j) {
+  if constexpr (sizeof(V) == 4) {
+    return __builtin_bit_cast(V, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), j));
+  } else {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(u), j);
+    const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(u >> 32), j);
+    return __builtin_bit_cast(V, (static_cast<uint64_t>(hi) << 32) | lo);
+  }
 }
 
-template <typename T>
-__global__ void k_search_init(const Query<T>* __restrict__ queries, Query<T> single, uint32_t nq,
-                              uint64_t* __restrict__ roots, unsigned long long* __restrict__ ctr,
-                              uint32_t nctr, unsigned long long* __restrict__ counts,
-                              unsigned long long* err) {
+// Initial state bits of a query of the given kind (codes other than 0..2
+// walk as right searches).
+__host__ __device__ inline uint32_t lo_open(uint32_t kind) { return kind != KIND_LEFT ? 1u : 0u; }
+__host__ __device__ inline uint32_t hi_open(uint32_t kind) {
+  return (kind == KIND_THREE || kind == KIND_LEFT) ? 1u : 0u;
+}
+
+template <typename T, int G>
+__global__ void k_search_init(const Query<T>* __restrict__ queries, Query<T> single, const uint32_t* gq,
+                              uint32_t nq, uint32_t ngroups, Entry* __restrict__ roots,
+                              unsigned long long* __restrict__ ctr, uint32_t nctr,
+                              unsigned long long* __restrict__ counts, unsigned long long* err) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nq) {
-    const uint32_t kind = queries ? queries[i].kind : single.kind;
-    roots[i] = (static_cast<uint64_t>(i) << 40) | start_code(kind);
-    counts[i] = 0;
+  if (i < ngroups) {
+    uint64_t item;
+    if constexpr (G == 1) {
+      const uint32_t kind = queries ? queries[i].kind : single.kind;
+      item = lo_open(kind) | (hi_open(kind) << 1);
+    } else {
+      uint32_t a = 0, l = 0, h = 0;
+      for (int j = 0; j < G; j++) {
+        const uint32_t q = gq[static_cast<uint64_t>(i) * G + j];
+        if (q < nq) {
+          const uint32_t kind = queries[q].kind;
+          a |= 1u << j;
+          l |= lo_open(kind) << j;
+          h |= hi_open(kind) << j;
+        }
+      }
+      item = (static_cast<uint64_t>(a) << 32) | (static_cast<uint64_t>(l) << 40) | (static_cast<uint64_t>(h) << 48);
+    }
+    roots[i] = Entry{item, i};
   }
-  if (i < nctr) ctr[i] = (i == 0) ? nq : 0;
+  if (i < nq) counts[i] = 0;
+  if (i < nctr) ctr[i] = (i == 0) ? ngroups : 0;
   if (i == 0) *err = 0;
 }
 
@@ -92,12 +167,23 @@ struct OutRec {
   using type = typename std::conditional<RET_POINT, Point<T, IS>, IS>::type;
 };
 
-template <typename T, typename IS, bool RET_POINT, bool WRITE>
+// Staged results per query before a flush (a step adds at most 64).
+template <int G, size_t REC>
+constexpr int stage_per_query() {
+  return G == 1 ? 256 : (REC <= 8 ? 128 : 64);
+}
+
+template <typename T, typename IS, bool RET_POINT, bool WRITE, int G>
 __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_t round, uint32_t last) {
   using Rec = typename OutRec<T, IS, RET_POINT>::type;
-  extern __shared__ uint32_t stack[];
-  __shared__ uint64_t pend[kWave];
-  __shared__ Rec stage[WRITE ? kStageIds : 1];
+  using It = Items<G>;
+  using Item = typename It::type;
+  constexpr int kStage = stage_per_query<G, sizeof(Rec)>();
+  constexpr int kLdsStack = lds_stack<G>();
+  __shared__ Item stack[kLdsStack];
+  __shared__ Entry pend[kWave];
+  Item* gstack = static_cast<Item*>(P.gstack) + static_cast<uint64_t>(blockIdx.x) * P.gcap;
+  __shared__ Rec stage[WRITE ? G * kStage : 1];
 
   const uint32_t lane = threadIdx.x;
   // Counters and lists written by the previous launch are read with
@@ -106,207 +192,319 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
       min(__hip_atomic_load(&P.ctr[2 * round], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), P.list_cap);
   if (P.dbg && lane == 0) {
     atomicMax(&P.dbg[round * 8 + 0], count_in);
-    atomicMin(&P.dbg[round * 8 + 1], count_in);
   }
   unsigned long long* grab = &P.ctr[2 * round + 1];
   unsigned long long* count_out = &P.ctr[2 * round + 2];
-  const uint64_t* list_in = P.list[round & 1];
-  uint64_t* list_out = P.list[(round + 1) & 1];
+  const Entry* list_in = P.list[round & 1];
+  Entry* list_out = P.list[(round + 1) & 1];
 
   unsigned long long chunk = count_in / P.total_waves;
   chunk = chunk < 1 ? 1 : (chunk > kWave ? kWave : chunk);
   const uint32_t budget =
       last ? 0xffffffffu : (count_in < P.total_waves ? P.budget_small : P.budget_large);
 
-  uint32_t npend = 0;  // uniform: valid pending entries are pend[0..npend) != kEmpty
+  // The first chunk of each wave is fixed by its index; later chunks are
+  // grabbed from a shared counter, which is read first so that waves with
+  // nothing left to take leave without an atomic update.
+  const unsigned long long dyn0 = static_cast<unsigned long long>(P.total_waves) * chunk;
+  bool fixed_done = false;
+  uint32_t npend = 0;  // uniform: valid pending entries are pend[0..npend) with group != kEmpty
   while (true) {
-    // ---------- pick the next batch: the pending items of one query ----------
-    const uint64_t mine = lane < npend ? pend[lane] : kEmpty;
-    const uint64_t have = __ballot(mine != kEmpty);
+    // ---------- pick the next batch: the pending items of one group ----------
+    const uint64_t mine_g = lane < npend ? pend[lane].group : kEmpty;
+    const uint64_t have = __ballot(mine_g != kEmpty);
     if (have == 0) {
       unsigned long long base = 0;
-      if (lane == 0) base = atomicAdd(grab, chunk);
-      base = __shfl(base, 0, kWave);
-      if (base >= count_in) break;
+      if (!fixed_done) {
+        fixed_done = true;
+        base = static_cast<unsigned long long>(blockIdx.x) * chunk;
+        if (base >= count_in) continue;
+      } else {
+        if (dyn0 >= count_in ||
+            dyn0 + __hip_atomic_load(grab, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= count_in)
+          break;
+        if (lane == 0) base = dyn0 + atomicAdd(grab, chunk);
+        base = __shfl(base, 0, kWave);
+        if (base >= count_in) break;
+      }
       const unsigned long long cnt = min(chunk, count_in - base);
-      uint64_t it = kEmpty;
-      if (lane < cnt)
-        it = __hip_atomic_load(&list_in[base + lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      Entry e{0, kEmpty};
+      if (lane < cnt) {
+        e.group = __hip_atomic_load(&list_in[base + lane].group, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        e.item = __hip_atomic_load(&list_in[base + lane].item, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
       // reject items that cannot come from this search
-      if (it != kEmpty && (static_cast<uint32_t>(it >> 40) >= P.nq || ((it & 0xffffffffull) >> 2) >= P.S)) {
+      if (e.group != kEmpty && (e.group >= P.ngroups || It::slot(static_cast<Item>(e.item)) >= P.S ||
+                                It::a(static_cast<Item>(e.item)) == 0)) {
         atomicOr(P.err, 2ull);
         if (P.dbg) atomicAdd(&P.dbg[round * 8 + 3], 1ull);
-        it = kEmpty;
+        e.group = kEmpty;
       }
-      pend[lane] = it;
+      pend[lane] = e;
       npend = static_cast<uint32_t>(cnt);
       if (P.dbg && lane == 0) atomicAdd(&P.dbg[round * 8 + 2], cnt);
       __builtin_amdgcn_wave_barrier();
       continue;
     }
     const int first = __builtin_ctzll(have);
-    const uint64_t head = __shfl(mine, first, kWave);
-    const uint32_t q = uniform(static_cast<uint32_t>(head >> 40));
-    const bool take = mine != kEmpty && static_cast<uint32_t>(mine >> 40) == q;
+    const uint32_t g = uniform(static_cast<uint32_t>(__shfl(mine_g, first, kWave)));
+    const bool take = mine_g == g;
     const uint64_t tmask = __ballot(take);
     if (take) {
-      stack[lanes_below(tmask)] = static_cast<uint32_t>(mine);
-      pend[lane] = kEmpty;
+      stack[lanes_below(tmask)] = static_cast<Item>(pend[lane].item);
+      pend[lane].group = kEmpty;
     }
     uint32_t sp = __popcll(tmask);
     __builtin_amdgcn_wave_barrier();
 
-    const Query<T> Q = P.queries ? P.queries[q] : P.single;
-    const uint32_t kind = uniform(Q.kind);
-    unsigned long long found = 0;  // results of this batch (count mode)
-    uint32_t nstage = 0;           // staged results (write mode)
-    unsigned long long off_q = 0, cap_q = 0;
-    if constexpr (WRITE) {
-      off_q = P.offsets ? P.offsets[q] : 0;
-      cap_q = P.caps ? P.caps[q] : P.cap_all;
+    // ---------- the group's queries: lane j holds query j ----------
+    uint32_t qid = ~0u;
+    Query<T> qv{};
+    if (lane < G) {
+      qid = (G == 1) ? g : P.gq[static_cast<uint64_t>(g) * G + lane];
+      if (qid < P.nq) qv = P.queries ? P.queries[qid] : P.single;
     }
-    auto flush = [&]() {
+    Bounds<T> qb[G];
+    uint32_t qi[G];
+#pragma unroll
+    for (int j = 0; j < G; j++) {
+      qb[j].lo = read_lane(qv.min_dim1, j);
+      qb[j].hi = read_lane(qv.max_dim1, j);
+      qb[j].min2 = read_lane(qv.min_dim2, j);
+      qi[j] = read_lane(qid, j);
+    }
+    unsigned long long found[G];  // results of this batch (count mode)
+    uint32_t nst[G];              // staged results (write mode)
+    unsigned long long off[G], cap[G];
+#pragma unroll
+    for (int j = 0; j < G; j++) {
+      found[j] = 0;
+      nst[j] = 0;
+      off[j] = 0;
+      cap[j] = 0;
+    }
+    if constexpr (WRITE) {
+      unsigned long long o = 0, c = 0;
+      if (lane < G && qid < P.nq) {
+        o = P.offsets ? P.offsets[qid] : 0;
+        c = P.caps ? P.caps[qid] : P.cap_all;
+      }
+#pragma unroll
+      for (int j = 0; j < G; j++) {
+        off[j] = read_lane(o, j);
+        cap[j] = read_lane(c, j);
+      }
+    }
+    // Flushes every query whose stage holds at least `thr` results: lanes
+    // 0..G-1 reserve their query's output range together (one atomic
+    // round trip for the group), then the wave copies the stages out.
+    auto flush = [&](uint32_t thr) {
       if constexpr (WRITE) {
-        if (nstage == 0) return;
+        uint32_t fl = 0, mine = 0;
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          if (nst[j] >= thr && nst[j] > 0) fl |= 1u << j;
+          if (lane == static_cast<uint32_t>(j)) mine = nst[j];
+        }
+        if (fl == 0) return;
         unsigned long long b = 0;
-        if (lane == 0) b = atomicAdd(&P.counts[q], static_cast<unsigned long long>(nstage));
-        b = __shfl(b, 0, kWave);
-        Rec* out = static_cast<Rec*>(P.out) + off_q;
-        for (uint32_t x = lane; x < nstage; x += kWave)
-          if (b + x < cap_q && off_q + b + x < P.out_total) out[b + x] = stage[x];
-        nstage = 0;
+        if (lane < G && ((fl >> lane) & 1u)) b = atomicAdd(&P.counts[qid], static_cast<unsigned long long>(mine));
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          if (!((fl >> j) & 1u)) continue;
+          const unsigned long long bj = read_lane(b, j);
+          Rec* out = static_cast<Rec*>(P.out) + off[j];
+          for (uint32_t x = lane; x < nst[j]; x += kWave)
+            if (bj + x < cap[j] && off[j] + bj + x < P.out_total) out[bj + x] = stage[j * kStage + x];
+          nst[j] = 0;
+        }
         __builtin_amdgcn_wave_barrier();
       }
     };
+    // a step adds at most 64 results per query; a full stage triggers a
+    // flush of every stage at least this full
+    constexpr uint32_t kFlushAll = kStage > kWave ? (kStage - kWave) / 2 + 1 : 1;
 
     uint32_t steps = 0;
-    while (sp > 0) {
-      // children (<= 2 per popped item) must fit: sp - take + 2 take <= cap
-      const uint32_t room = P.stack_cap - sp;
-      uint32_t takecnt = sp < kWave ? sp : kWave;
-      if (takecnt > room) takecnt = room;
-      if (takecnt == 0) {
-        if (lane == 0) atomicOr(P.err, 1ull);
-        sp = 0;
-        break;
-      }
-      sp -= takecnt;
-      const bool act = lane < takecnt;
-      const uint32_t item = act ? stack[sp + lane] : 0;
-      __builtin_amdgcn_wave_barrier();
-      const uint64_t slot = item >> 2;
-      const uint32_t code = item & 3u;
-      bool rep = false, cl = false, cr = false;
-      uint32_t codel = REPORT_ABOVE, coder = REPORT_ABOVE;
-      KV<T, IS> a{};
-      DM<T> b{};
-      bool ok = act;
-      if (act && (slot >= P.S || subtree_size(P.M, slot) < 1)) {
-        atomicOr(P.err, 2ull);
-        ok = false;
-      }
-      if (ok) {
-        a = P.kv[slot];
-        if (RET_POINT || code != REPORT_ABOVE) b = P.dm[slot];
-        if (Q.min_dim2 <= a.dim2) {
-          // a child is pushed only if it exists and survives the dim2 bound,
-          // which its parent's record tells without loading the child
-          const int64_t m = subtree_size(P.M, slot);
-          const bool hl = m >= 2 && Q.min_dim2 <= a.cdim2[0];
-          const bool hr = m >= 3 && Q.min_dim2 <= a.cdim2[1];
-          if (code == REPORT_ABOVE) {
-            rep = true;
-            cl = hl;
-            cr = hr;
-          } else {
-            const T d1 = b.dim1, md = b.med;
-            if (kind == KIND_THREE) rep = Q.min_dim1 <= d1 && d1 <= Q.max_dim1;
-            else if (kind == KIND_LEFT) rep = d1 <= Q.max_dim1;
-            else rep = d1 >= Q.min_dim1;
-            if (code == LEFT_SEARCH) {
-              if (md <= Q.max_dim1) {
-                cl = hl; codel = REPORT_ABOVE;
-                cr = hr; coder = LEFT_SEARCH;
-              } else {
-                cl = hl; codel = LEFT_SEARCH;
-              }
-            } else if (code == RIGHT_SEARCH) {
-              if (md >= Q.min_dim1) {
-                cl = hl; codel = RIGHT_SEARCH;
-                cr = hr; coder = REPORT_ABOVE;
-              } else {
-                cr = hr; coder = RIGHT_SEARCH;
-              }
-            } else {  // THREE_SEARCH
-              if (Q.min_dim1 <= md && md <= Q.max_dim1) {
-                cl = hl; codel = RIGHT_SEARCH;
-                cr = hr; coder = LEFT_SEARCH;
-              } else if (Q.max_dim1 < md) {
-                cl = hl; codel = THREE_SEARCH;
-              } else if (md < Q.min_dim1) {
-                cr = hr; coder = THREE_SEARCH;
-              }
-            }
-          }
-        }
-      }
-      // ---- results
-      const uint64_t rmask = __ballot(rep);
-      if constexpr (WRITE) {
-        if (rep) {
-          Rec r;
-          if constexpr (RET_POINT) {
-            r.dim1_val = b.dim1;
-            r.dim2_val = a.dim2;
-            if constexpr (has_id_v<IS>) r.id = a.id;
-          } else {
-            r = a.id;
-          }
-          stage[nstage + lanes_below(rmask)] = r;
-        }
-        nstage += __popcll(rmask);
+    unsigned long long visits = 0;  // items popped (debug counters only)
+    uint32_t gsp = 0;  // items in the wave's global overflow stack
+    while (sp > 0 || gsp > 0) {
+      if (sp == 0) {
+        // refill from the overflow stack (blocks of kMoveBlock, each lane
+        // reads back the entries it wrote)
+        gsp -= kMoveBlock;
+#pragma unroll
+        for (int k = 0; k < kMoveBlock / kWave; k++) stack[lane + k * kWave] = gstack[gsp + lane + k * kWave];
+        sp = kMoveBlock;
         __builtin_amdgcn_wave_barrier();
-        if (nstage >= kFlushAt) flush();
-      } else {
-        found += __popcll(rmask);
       }
-      // ---- children
+      if (sp > kLdsStack - kWave) {
+        // a step grows the stack by at most 64: move the oldest kMoveBlock
+        // items to the overflow stack and shift the rest down
+        if (gsp + kMoveBlock > P.gcap) {
+          if (lane == 0) atomicOr(P.err, 1ull);
+          sp = 0;
+          gsp = 0;
+          break;
+        }
+#pragma unroll
+        for (int k = 0; k < kMoveBlock / kWave; k++) gstack[gsp + lane + k * kWave] = stack[lane + k * kWave];
+        gsp += kMoveBlock;
+        // shift in passes of kMoveBlock: each pass reads a block above the
+        // moved one before writing it kMoveBlock lower
+        for (uint32_t b0 = kMoveBlock; b0 < sp; b0 += kMoveBlock) {
+          Item keep[kMoveBlock / kWave];
+#pragma unroll
+          for (int k = 0; k < kMoveBlock / kWave; k++) {
+            const uint32_t x = b0 + lane + k * kWave;
+            keep[k] = x < sp ? stack[x] : Item(0);
+          }
+          __builtin_amdgcn_wave_barrier();
+#pragma unroll
+          for (int k = 0; k < kMoveBlock / kWave; k++) {
+            const uint32_t x = b0 + lane + k * kWave;
+            if (x < sp) stack[x - kMoveBlock] = keep[k];
+          }
+          __builtin_amdgcn_wave_barrier();
+        }
+        sp -= kMoveBlock;
+        __builtin_amdgcn_wave_barrier();
+      }
+      const uint32_t takecnt = sp < kWave ? sp : kWave;
+      sp -= takecnt;
+      visits += takecnt;
+      const bool act = lane < takecnt;
+      const Item item = act ? stack[sp + lane] : Item(0);
+      __builtin_amdgcn_wave_barrier();
+      const uint64_t slot = It::slot(item);
+      int64_t m = 0;
+      bool ok = act;
+      if (act) {
+        m = subtree_size(P.M, slot);
+        if (slot >= P.S || m < 1) {
+          atomicOr(P.err, 2ull);
+          ok = false;
+        }
+      }
+      Node<T, IS> nd{};
+      StepOut so{};
+      if (ok) nd = P.nodes[slot];
+      // deep in reported subtrees every item is REPORT_ABOVE for all its
+      // queries: then only the dim2 prune is evaluated
+      const bool open = ok && (It::l(item) | It::h(item)) != 0;
+      if (__ballot(open) == 0) {
+        if (ok) so = node_step_report_above<T, G>(nd.dim2, m, qb, It::a(item));
+      } else if (ok) {
+        so = node_step<T, G>(nd.dim1, nd.dim2, nd.med, m, qb, It::a(item), It::l(item), It::h(item));
+      }
+      // ---- results, per query of the group
+#pragma unroll
+      for (int j = 0; j < G; j++) {
+        const bool r = (so.rep >> j) & 1u;
+        const uint64_t rmask = __ballot(r);
+        if (rmask == 0) continue;
+        if constexpr (WRITE) {
+          if (r) {
+            Rec rec;
+            if constexpr (RET_POINT) {
+              rec.dim1_val = nd.dim1;
+              rec.dim2_val = nd.dim2;
+              if constexpr (has_id_v<IS>) rec.id = nd.id;
+            } else {
+              rec = nd.id;
+            }
+            stage[j * kStage + nst[j] + lanes_below(rmask)] = rec;
+          }
+          nst[j] += __popcll(rmask);
+        } else {
+          found[j] += __popcll(rmask);
+        }
+      }
+      if constexpr (WRITE) {
+        __builtin_amdgcn_wave_barrier();
+        bool full = false;
+#pragma unroll
+        for (int j = 0; j < G; j++) full |= nst[j] > static_cast<uint32_t>(kStage - kWave);
+        if (full) flush(kFlushAll);
+      }
+      // ---- children, pushed as (left, right) pairs in lane order: the next
+      // pop then takes whole sibling pairs of neighbouring parents, which sit
+      // side by side in the level below (a few cache lines per pop)
+      const bool cl = so.al != 0, cr = so.ar != 0;
       const uint64_t lm = __ballot(cl), rm = __ballot(cr);
-      const uint32_t nl = __popcll(lm);
-      if (cl) stack[sp + lanes_below(lm)] = static_cast<uint32_t>(((2 * slot + 1) << 2) | codel);
-      if (cr) stack[sp + nl + lanes_below(rm)] = static_cast<uint32_t>(((2 * slot + 2) << 2) | coder);
-      sp += nl + __popcll(rm);
+      const uint32_t pos = sp + lanes_below(lm) + lanes_below(rm);
+      if (cl) stack[pos] = It::make(2 * slot + 1, so.al, so.ll, so.hl);
+      if (cr) stack[pos + (cl ? 1u : 0u)] = It::make(2 * slot + 2, so.ar, so.lr, so.hr);
+      sp += __popcll(lm) + __popcll(rm);
       __builtin_amdgcn_wave_barrier();
       // ---- hand remaining work to the next round when over budget
-      if (++steps >= budget && sp > 0) {
+      if (++steps >= budget && sp + gsp > 0) {
         // Reservations only grow the counter; one that ends past the list
         // capacity fails, marks its in-capacity part empty (the next round
         // skips empty entries and reads at most list_cap of them) and the
         // wave keeps walking its batch instead.
         unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(count_out, static_cast<unsigned long long>(sp));
+        const uint32_t tot = sp + gsp;
+        if (lane == 0) base = atomicAdd(count_out, static_cast<unsigned long long>(tot));
         base = __shfl(base, 0, kWave);
-        if (base + sp <= P.list_cap) {
-          for (uint32_t x = lane; x < sp; x += kWave)
-            list_out[base + x] = (static_cast<uint64_t>(q) << 40) | stack[x];
+        if (base + tot <= P.list_cap) {
+          for (uint32_t x = lane; x < gsp; x += kWave) list_out[base + x] = Entry{gstack[x], g};
+          for (uint32_t x = lane; x < sp; x += kWave) list_out[base + gsp + x] = Entry{stack[x], g};
           if (P.dbg && lane == 0) {
             atomicAdd(&P.dbg[round * 8 + 4], 1ull);
-            atomicAdd(&P.dbg[round * 8 + 5], static_cast<unsigned long long>(sp));
+            atomicAdd(&P.dbg[round * 8 + 5], static_cast<unsigned long long>(tot));
           }
           sp = 0;
+          gsp = 0;
         } else {
-          for (unsigned long long x = base + lane; x < P.list_cap && x < base + sp; x += kWave)
-            list_out[x] = kEmpty;
+          for (unsigned long long x = base + lane; x < P.list_cap && x < base + tot; x += kWave)
+            list_out[x].group = kEmpty;
           steps = 0;
         }
       }
     }
+    if (P.dbg && lane == 0) {
+      atomicAdd(&P.dbg[round * 8 + 6], static_cast<unsigned long long>(steps));
+      atomicMax(&P.dbg[round * 8 + 7], static_cast<unsigned long long>(steps));
+      atomicAdd(&P.dbg[round * 8 + 1], visits);
+    }
     if constexpr (WRITE) {
-      flush();
+      flush(1);
     } else {
-      if (lane == 0 && found) atomicAdd(&P.counts[q], found);
+#pragma unroll
+      for (int j = 0; j < G; j++)
+        if (lane == 0 && found[j]) atomicAdd(&P.counts[qi[j]], found[j]);
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Grouping: queries are ordered by kind, then by their leading dim1 bound
+// (max_dim1 for left searches, min_dim1 otherwise), with two stable radix
+// sorts; group k holds sorted positions [kG, kG + G).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_group_keys(const Query<T>* __restrict__ q, uint32_t nq, uint64_t* __restrict__ key,
+                             uint32_t* __restrict__ idx) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const Query<T> r = q[i];
+  key[i] = static_cast<uint64_t>(ord_key(r.kind == KIND_LEFT ? r.max_dim1 : r.min_dim1));
+  idx[i] = i;
+}
+
+template <typename T>
+__global__ void k_kind_keys(const Query<T>* __restrict__ q, const uint32_t* __restrict__ perm, uint32_t nq,
+                            uint32_t* __restrict__ kk) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const uint32_t k = q[perm[i]].kind;
+  kk[i] = k > KIND_RIGHT ? KIND_RIGHT : k;
+}
+
+__global__ void k_group_pad(uint32_t* __restrict__ gq, uint32_t nq, uint32_t total) {
+  const uint32_t i = nq + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < total) gq[i] = ~0u;
 }
 
 // Exclusive prefix sum of per-query counts: offsets[q] = sum of counts
@@ -355,16 +553,20 @@ __global__ __launch_bounds__(kScanBlock) void k_exclusive_scan(const unsigned lo
   if (tid == 0) out[n] = carry;
 }
 
-template <typename T, typename IS, bool RP, bool W>
+inline uint32_t group_size(const SearchArgs& a) {
+  return (a.single_query || a.nq == 1 || a.group == 1) ? 1u : static_cast<uint32_t>(kGroup);
+}
+
+template <typename T, typename IS, bool RP, bool W, int G>
 Status launch_rounds(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
   Params<T, IS> P{};
-  P.kv = static_cast<const KV<T, IS>*>(a.kv);
-  P.dm = static_cast<const DM<T>*>(a.dm);
+  P.nodes = static_cast<const Node<T, IS>*>(a.nodes);
   P.M = a.n + 1;
   P.queries = static_cast<const Query<T>*>(a.queries);
   if (a.single_query) P.single = *static_cast<const Query<T>*>(a.single_query);
-  P.list[0] = static_cast<uint64_t*>(sc.list[0]);
-  P.list[1] = static_cast<uint64_t*>(sc.list[1]);
+  P.gq = G == 1 ? nullptr : sc.groups;
+  P.list[0] = static_cast<Entry*>(sc.list[0]);
+  P.list[1] = static_cast<Entry*>(sc.list[1]);
   P.list_cap = sc.list_cap;
   P.ctr = sc.ctr;
   P.counts = a.counts;
@@ -377,26 +579,41 @@ Status launch_rounds(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
   P.dbg = a.debug;
   P.S = (a.n == 0) ? 0 : ((1ull << (64 - __builtin_clzll(a.n))) - 1);
   P.nq = static_cast<uint32_t>(a.nq);
+  P.ngroups = static_cast<uint32_t>((a.nq + G - 1) / G);
   uint32_t H = 0;
   while ((1ull << H) <= a.n) H++;  // levels
-  P.stack_cap = static_cast<uint32_t>(kWave * (H + 2) + 2 * kWave);
+  // deepest stack: 64 items per level below the first popped batch, plus a
+  // step's growth, in whole move blocks
+  const uint32_t bound = kWave * (H + 2) + 2 * kWave;
+  P.gcap = ((bound + kMoveBlock - 1) / kMoveBlock) * kMoveBlock;
   P.budget_small = a.budget_small > 0 ? a.budget_small : 6;
   P.budget_large = a.budget_large > 0 ? a.budget_large : 384;
-  const size_t lds = P.stack_cap * sizeof(uint32_t);
-  auto kern = k_search_round<T, IS, RP, W>;
+  auto kern = k_search_round<T, IS, RP, W, G>;
   int dev = 0;
   hipGetDevice(&dev);
   int cus = 0;
   hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  int occ = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kWave, lds);
-  if (e != hipSuccess || occ <= 0) occ = 8;
-  if (a.waves_per_cu > 0 && a.waves_per_cu < occ) occ = a.waves_per_cu;
+  // Waves per CU (one wave per workgroup). Measured on the 2^26-point
+  // batch: group walks stop gaining past about 16 per CU, one-query walks
+  // (more latency bound) gain up to 28.
+  int occ = G == 1 ? 28 : 16;
+  if (a.waves_per_cu > 0) occ = std::min(a.waves_per_cu, 64);
+  hipError_t e = hipSuccess;
   const uint32_t grid = static_cast<uint32_t>(cus * occ);
   P.total_waves = grid;
+  const size_t gbytes = static_cast<size_t>(grid) * P.gcap * sizeof(typename Items<G>::type);
+  if (sc.gstack_bytes < gbytes) {
+    if (sc.gstack) hipFree(sc.gstack);
+    sc.gstack = nullptr;
+    sc.gstack_bytes = 0;
+    e = hipMalloc(&sc.gstack, gbytes);
+    if (e != hipSuccess) return hip_status(e, "Error in allocating search overflow stacks");
+    sc.gstack_bytes = gbytes;
+  }
+  P.gstack = sc.gstack;
   const int R = std::max(1, std::min(a.rounds, kMaxRounds));
   for (int r = 0; r < R; r++) {
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kWave), lds, s, P, static_cast<uint32_t>(r),
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kWave), 0, s, P, static_cast<uint32_t>(r),
                        static_cast<uint32_t>(r == R - 1));
   }
   return hip_status(hipGetLastError(), "Error in launching search kernel");
@@ -422,9 +639,31 @@ Status with_is(int idt, F&& f) {
   return Status{1, "Error: unsupported ID type code " + std::to_string(idt)};
 }
 
+template <typename T>
+Status make_groups(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
+  const uint32_t nq = static_cast<uint32_t>(a.nq);
+  const uint32_t total = ((nq + kGroup - 1) / kGroup) * kGroup;
+  const unsigned blocks = (nq + 255) / 256;
+  auto* keys = static_cast<uint64_t*>(sc.gkeys);
+  auto* idx = reinterpret_cast<uint32_t*>(keys + 2 * sc.gcap);
+  uint32_t* perm1 = idx + sc.gcap;
+  uint32_t* kk = perm1 + sc.gcap;
+  k_group_keys<T><<<blocks, 256, 0, s>>>(static_cast<const Query<T>*>(a.queries), nq, keys, idx);
+  size_t tb = sc.gtmp_bytes;
+  hipError_t e = radix_sort_u64(sc.gtmp, tb, keys, keys + sc.gcap, idx, perm1, nq, s);
+  if (e != hipSuccess) return hip_status(e, "Error in sorting a query batch");
+  k_kind_keys<T><<<blocks, 256, 0, s>>>(static_cast<const Query<T>*>(a.queries), perm1, nq, kk);
+  tb = sc.gtmp_bytes;
+  e = radix_sort_u32(sc.gtmp, tb, kk, kk + sc.gcap, perm1, sc.groups, nq, s);
+  if (e != hipSuccess) return hip_status(e, "Error in sorting a query batch");
+  if (total > nq) k_group_pad<<<1, 256, 0, s>>>(sc.groups, nq, total);
+  return hip_status(hipGetLastError(), "Error in launching query grouping");
+}
+
 }  // namespace
 
 Status scratch_reserve(SearchScratch& sc, size_t nq, int dtype, hipStream_t) {
+  (void)dtype;
   const size_t spill_cap = 1ull << 22;  // 4M items per round list
   const size_t need = std::max(nq, spill_cap);
   if (sc.list_cap < need) {
@@ -432,8 +671,8 @@ Status scratch_reserve(SearchScratch& sc, size_t nq, int dtype, hipStream_t) {
     if (sc.list[1]) hipFree(sc.list[1]);
     sc.list[0] = sc.list[1] = nullptr;
     sc.list_cap = 0;
-    hipError_t e = hipMalloc(&sc.list[0], need * 8);
-    if (e == hipSuccess) e = hipMalloc(&sc.list[1], need * 8);
+    hipError_t e = hipMalloc(&sc.list[0], need * sizeof(Entry));
+    if (e == hipSuccess) e = hipMalloc(&sc.list[1], need * sizeof(Entry));
     if (e != hipSuccess) return hip_status(e, "Error in allocating search work lists");
     sc.list_cap = need;
   }
@@ -441,20 +680,31 @@ Status scratch_reserve(SearchScratch& sc, size_t nq, int dtype, hipStream_t) {
     hipError_t e = hipMalloc(&sc.ctr, (2 * kMaxRounds + 4) * sizeof(unsigned long long));
     if (e != hipSuccess) return hip_status(e, "Error in allocating search counters");
   }
-  const size_t qb = 32;
-  if (sc.qbuf_bytes < qb) {
-    if (sc.qbuf) hipFree(sc.qbuf);
-    hipError_t e = hipMalloc(&sc.qbuf, qb);
-    if (e != hipSuccess) return hip_status(e, "Error in allocating query record");
-    sc.qbuf_bytes = qb;
-  }
   if (sc.cursor_cap < std::max<size_t>(nq, 1)) {
     if (sc.cursor) hipFree(sc.cursor);
     hipError_t e = hipMalloc(&sc.cursor, std::max<size_t>(nq, 1) * 8);
     if (e != hipSuccess) return hip_status(e, "Error in allocating result cursors");
     sc.cursor_cap = std::max<size_t>(nq, 1);
   }
-  (void)dtype;
+  if (nq > 1 && sc.gcap < nq) {
+    // group table (padded to whole groups) and the sort's keys and values
+    const size_t cap = ((nq + kGroup - 1) / kGroup) * kGroup;
+    if (sc.groups) hipFree(sc.groups);
+    if (sc.gkeys) hipFree(sc.gkeys);
+    if (sc.gtmp) hipFree(sc.gtmp);
+    sc.groups = nullptr;
+    sc.gkeys = sc.gtmp = nullptr;
+    sc.gcap = 0;
+    size_t t64 = 0, t32 = 0;
+    hipError_t e = radix_sort_u64(nullptr, t64, nullptr, nullptr, nullptr, nullptr, cap, nullptr);
+    if (e == hipSuccess) e = radix_sort_u32(nullptr, t32, nullptr, nullptr, nullptr, nullptr, cap, nullptr);
+    if (e == hipSuccess) e = hipMalloc(&sc.groups, cap * 4);
+    if (e == hipSuccess) e = hipMalloc(&sc.gkeys, cap * (2 * 8 + 4 * 4));
+    sc.gtmp_bytes = std::max<size_t>(std::max(t64, t32), 16);
+    if (e == hipSuccess) e = hipMalloc(&sc.gtmp, sc.gtmp_bytes);
+    if (e != hipSuccess) return hip_status(e, "Error in allocating query grouping storage");
+    sc.gcap = cap;
+  }
   return Status{};
 }
 
@@ -462,32 +712,38 @@ void scratch_release(SearchScratch& sc) {
   if (sc.list[0]) hipFree(sc.list[0]);
   if (sc.list[1]) hipFree(sc.list[1]);
   if (sc.ctr) hipFree(sc.ctr);
-  if (sc.qbuf) hipFree(sc.qbuf);
   if (sc.cursor) hipFree(sc.cursor);
+  if (sc.groups) hipFree(sc.groups);
+  if (sc.gkeys) hipFree(sc.gkeys);
+  if (sc.gtmp) hipFree(sc.gtmp);
+  if (sc.gstack) hipFree(sc.gstack);
   sc = SearchScratch{};
 }
 
-template <typename T>
+template <typename T, int G>
 Status init_typed(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
   Query<T> one{};
   if (a.single_query) one = *static_cast<const Query<T>*>(a.single_query);
   const uint32_t nctr = 2 * kMaxRounds + 2;
+  const uint32_t ngroups = static_cast<uint32_t>((a.nq + G - 1) / G);
   const uint32_t work = static_cast<uint32_t>(std::max<size_t>(a.nq, nctr));
-  k_search_init<T><<<(work + 255) / 256, 256, 0, s>>>(
-      a.single_query ? nullptr : static_cast<const Query<T>*>(a.queries), one, static_cast<uint32_t>(a.nq),
-      static_cast<uint64_t*>(sc.list[0]), sc.ctr, nctr, a.counts, sc.ctr + 2 * kMaxRounds + 2);
+  k_search_init<T, G><<<(work + 255) / 256, 256, 0, s>>>(
+      a.single_query ? nullptr : static_cast<const Query<T>*>(a.queries), one, G == 1 ? nullptr : sc.groups,
+      static_cast<uint32_t>(a.nq), ngroups, static_cast<Entry*>(sc.list[0]), sc.ctr, nctr, a.counts,
+      sc.ctr + 2 * kMaxRounds + 2);
   return hip_status(hipGetLastError(), "Error in launching search initialisation");
 }
 
-// Enqueues the initialisation and the search rounds on `s` (asynchronous).
-// A single query is passed by value in a.single_query (host Query<T>) and
-// copied to the device by the initialisation kernel.
+// Enqueues the grouping, initialisation and search rounds on `s`
+// (asynchronous). A single query is passed by value in a.single_query (host
+// Query<T>) and reaches the device as a kernel argument.
 Status run_search(const SearchArgs& a_in, SearchScratch& sc, hipStream_t s) {
   SearchArgs a = a_in;
   if (a.nq == 0) return Status{};
   Status st = scratch_reserve(sc, a.nq, a.dtype, s);
   if (!st.ok()) return st;
   if (a.single_query) a.queries = nullptr;
+  const uint32_t G = group_size(a);
   return with_t(a.dtype, [&](auto t) -> Status {
     using T = decltype(t);
     return with_is(a.id_dtype, [&](auto i) -> Status {
@@ -495,12 +751,23 @@ Status run_search(const SearchArgs& a_in, SearchScratch& sc, hipStream_t s) {
       if (!a.ret_point && a.mode == 1) {
         if constexpr (!has_id_v<IS>) return Status{1, "Error: IDs requested from a tree built without IDs"};
       }
-      Status s0 = init_typed<T>(a, sc, s);
-      if (!s0.ok()) return s0;
-      if (a.mode == 0) return launch_rounds<T, IS, false, false>(a, sc, s);
-      if (a.ret_point) return launch_rounds<T, IS, true, true>(a, sc, s);
-      if constexpr (has_id_v<IS>) return launch_rounds<T, IS, false, true>(a, sc, s);
-      return Status{1, "Error: IDs requested from a tree built without IDs"};
+      auto go = [&](auto gtag) -> Status {
+        constexpr int GG = decltype(gtag)::value;
+        if constexpr (GG > 1) {
+          if (!a.groups_ready) {
+            Status sg = make_groups<T>(a, sc, s);
+            if (!sg.ok()) return sg;
+          }
+        }
+        Status s0 = init_typed<T, GG>(a, sc, s);
+        if (!s0.ok()) return s0;
+        if (a.mode == 0) return launch_rounds<T, IS, false, false, GG>(a, sc, s);
+        if (a.ret_point) return launch_rounds<T, IS, true, true, GG>(a, sc, s);
+        if constexpr (has_id_v<IS>) return launch_rounds<T, IS, false, true, GG>(a, sc, s);
+        return Status{1, "Error: IDs requested from a tree built without IDs"};
+      };
+      if (G == 1) return go(std::integral_constant<int, 1>{});
+      return go(std::integral_constant<int, kGroup>{});
     });
   });
 }

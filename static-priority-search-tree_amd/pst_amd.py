@@ -33,7 +33,8 @@ EXPORTED = [
     "pst_gpu_num_devs", "pst_gpu_three_sided_search", "pst_gpu_two_sided_left_search",
     "pst_gpu_two_sided_right_search", "pst_gpu_search_into", "pst_gpu_calc_global_mem_needed",
     "pst_gpu_export_tree", "pst_gpu_export_bytes", "pst_gpu_batch_count", "pst_gpu_batch_write",
-    "pst_gpu_batch_search", "pst_gpu_set_search_tuning", "pst_interval_parallel_search",
+    "pst_gpu_batch_search", "pst_gpu_set_search_tuning", "pst_gpu_set_query_grouping",
+    "pst_interval_parallel_search",
     "pst_interval_parallel_search_into",
 ]
 
@@ -98,6 +99,7 @@ def lib():
         L.pst_gpu_batch_write.argtypes = [vp, vp, sz, i32, vp, vp, vp, vp]
         L.pst_gpu_batch_search.argtypes = [vp, vp, sz, i32, vp, vp, vp, sz, vp]
         L.pst_gpu_set_search_tuning.argtypes = [vp, i32, i32, i32, i32]
+        L.pst_gpu_set_query_grouping.argtypes = [vp, i32]
         L.pst_interval_parallel_search.argtypes = [vp, sz, i32, i32, i32, c.POINTER(vp),
                                                    c.POINTER(sz), vp, i32, i32, i32,
                                                    c.c_uint, c.c_uint]
@@ -253,6 +255,10 @@ class StaticPSTGPU:
 
     def set_tuning(self, rounds=0, budget_small=0, budget_large=0, waves_per_cu=-1):
         _check(lib().pst_gpu_set_search_tuning(self.h, rounds, budget_small, budget_large, waves_per_cu))
+
+    def set_grouping(self, queries_per_walk: int):
+        """Queries per tree walk in batched searches: 8 (default) or 1."""
+        _check(lib().pst_gpu_set_query_grouping(self.h, queries_per_walk))
 
     # ---- layout ----
     def export_tree(self):

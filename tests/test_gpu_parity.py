@@ -160,14 +160,18 @@ def test_nan_input_rejected(torch, P):
     assert e.value.code == 3
 
 
-@pytest.mark.parametrize("t,i", [("f32", "u32"), ("i32", "u32"), ("f64", None)])
-def test_batched_count_write_equal_single(torch, P, t, i):
+@pytest.mark.parametrize("group", [8, 1])
+@pytest.mark.parametrize("t,i", [("f32", "u32"), ("i32", "u32"), ("f64", None), ("f32", "u64"),
+                                 ("i64", "u32")])
+def test_batched_count_write_equal_single(torch, P, t, i, group):
     """Count pass + exclusive scan + write pass over a mixed batch equals
-    one-at-a-time searches and brute force."""
+    brute force, with grouped (8 queries per walk) and ungrouped walks."""
     rng = np.random.default_rng(9)
     n = 30000
     p = _rand_points(rng, t, i, n, dup=False)
     tr, keep = _tree(P, p, t, i)
+    tr.set_grouping(group)
+    print(f"batched {t}/{i} group={group}", flush=True)
     nq = 300
     qd = P.query_dtype(t)
     qs = np.zeros(nq, qd)
@@ -217,3 +221,44 @@ def test_interval_parallel_search_equals_brute_force(torch, P, t, i):
             want = G.canon(exp, True, i is not None) if rp else G.canon(exp["id"], False, True)
             got = G.canon(res, True, i is not None) if rp else G.canon(res, False, True)
             assert got == want
+
+
+@pytest.mark.parametrize("group", [8, 1])
+def test_batch_search_interval_workload(torch, P, group):
+    """One-call count + scan + write (pst_gpu_batch_search) on a 2^20-point
+    interval tree with a batch of 515 queries: stabbing queries (both
+    sides), three-sided queries of log-uniform width, a duplicate, two
+    queries that report every point, and a batch size that leaves the last
+    group partial. Every query's results equal the oracle's."""
+    import pst_data as D
+    n = 1 << 20
+    pts = D.interval_points(n, seed=4)
+    tr, keep = _tree(P, pts, "f32", "u32")
+    tr.set_grouping(group)
+    qs = D.float_query_mix(128, 384, seed=5)
+    # a duplicate query, and two that report every point (deep walk stacks
+    # that move to the overflow stack): 515 in all
+    full = np.zeros(2, qs.dtype)
+    full[0] = (0.0, 1.0, 0.0, 0)
+    full[1] = (0.0, 0.0, 0.0, 2)
+    qs = np.concatenate([qs, qs[:1], full])
+    nq = len(qs)
+    print(f"batch_search n=2^20 nq={nq} group={group}", flush=True)
+    qd = P.to_device(qs)
+    counts = torch.zeros(nq, dtype=torch.int64, device="cuda")
+    offs = torch.zeros(nq + 1, dtype=torch.int64, device="cuda")
+    ora = O.build(pts, "f32", "u32")
+    exp = [O.search(ora, int(q["kind"]), q["min_dim1"], q["max_dim1"], q["min_dim2"]) for q in qs]
+    total = sum(len(e) for e in exp)
+    out = torch.zeros(total + 16, dtype=torch.int32, device="cuda")
+    tr.batch_search(qd, nq, counts, offs, out, total + 16)
+    torch.cuda.synchronize()
+    cnt = counts.cpu().numpy()
+    off = offs.cpu().numpy()
+    res = out.cpu().numpy().view(np.uint32)
+    assert off[nq] == total
+    for k in range(nq):
+        assert cnt[k] == len(exp[k]), k
+        got = np.sort(res[off[k]:off[k] + cnt[k]])
+        assert np.array_equal(got, np.sort(exp[k])), k
+    tr.close()
