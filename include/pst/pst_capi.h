@@ -80,8 +80,9 @@ typedef struct { double min_dim1, max_dim1, min_dim2; uint32_t kind; } pst_query
 typedef struct { int32_t min_dim1, max_dim1, min_dim2; uint32_t kind; } pst_query_i32;
 typedef struct { int64_t min_dim1, max_dim1, min_dim2; uint32_t kind; } pst_query_i64;
 
-typedef struct pst_tree pst_tree;     /* StaticPSTGPU */
-typedef struct pst_forest pst_forest; /* StaticPSTGPUArr */
+/* One tree (StaticPSTGPU) or a forest of shallow trees (StaticPSTGPUArr);
+ * both are searched, exported and destroyed through the same entry points. */
+typedef struct pst_tree pst_tree;
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* pst_last_error(void);
@@ -147,6 +148,33 @@ size_t pst_gpu_calc_global_mem_needed(size_t num_elems, int dtype, int id_dtype)
  * five arrays and S. Used by StaticPSTGPU::print and parity tests. */
 int pst_gpu_export_tree(pst_tree* tree, void* host_blob, size_t blob_bytes, size_t* offsets6);
 size_t pst_gpu_export_bytes(const pst_tree* tree);
+
+/* ------------------------------------------------------------------ */
+/* StaticPSTGPUArr (pst/static-pst-gpu-arr.h:45-366)                   */
+/* ------------------------------------------------------------------ */
+
+/* Replaces StaticPSTGPUArr::StaticPSTGPUArr(pt_arr_d, num_elems,
+ * threads_per_block, dev_ind, num_devs, dev_props)
+ * (pst/static-pst-gpu-arr.h:51-54, pst/static-pst-gpu-arr.tu:17-487): the
+ * input is cut into consecutive chunks of St = (smallest power of two above
+ * threads_per_block) - 1 points (pst/static-pst-gpu-arr.h:256-261) and each
+ * chunk gets its own tree, the one StaticPSTGPU would build for that chunk
+ * alone. threads_per_block in [1, 1024]. The returned handle is used with
+ * the searches, export, batch entry points and pst_gpu_destroy above and
+ * below; searches report from every tree. Right and three-sided searches,
+ * empty stubs in the reference (pst/static-pst-gpu-arr.tu:564-578,
+ * 645-658), are defined like the other searches. Blocking. */
+int pst_gpu_arr_create(const void* pt_arr_d, size_t num_elems, int dtype, int id_dtype,
+                       unsigned threads_per_block, int dev_ind, int num_devs, pst_stream_t stream,
+                       pst_tree** out);
+/* Replaces StaticPSTGPUArr::calcGlobalMemNeeded(num_elems, threads_per_block)
+ * (pst/static-pst-gpu-arr.h:103, pst/static-pst-gpu-arr.tu:662-686). */
+size_t pst_gpu_arr_calc_global_mem_needed(size_t num_elems, unsigned threads_per_block, int dtype,
+                                          int id_dtype);
+/* St of a forest (0 for a StaticPSTGPU handle). The forest export holds
+ * each tree in the reference's per-tree layout, trees concatenated: tree c
+ * at position c*St of every array, the last tree with its own S. */
+size_t pst_gpu_slots_per_tree(const pst_tree* tree);
 
 /* ---- Batched searches (no counterpart in the reference) ----
  * queries_d: device array of pst_query_<dtype>. Count pass then write pass:

@@ -328,6 +328,141 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish(
                    GlobalPerm{perm2}, nodes);
 }
 
+// ---------------------------------------------------------------------------
+// Forest of shallow trees (StaticPSTGPUArr). Tree c holds the input points
+// [c*St, min(n, (c+1)*St)), the reference's consecutive chunks
+// (pst/static-pst-gpu-arr-populate-trees.tu:25-40, pst/static-pst-gpu-arr.tu:
+// 1037-1060), and is the tree the deep construction would build for that
+// chunk alone. One workgroup per tree: the chunk's key1 and key2 orders come
+// from bitonic sorts in LDS, then lds_build fills the tree level by level.
+// Tree c's records start at c << L, with St = 2^L - 1.
+// ---------------------------------------------------------------------------
+template <typename Before>
+__device__ void bitonic_sort(uint16_t* idx, uint32_t N, Before before) {
+  for (uint32_t k = 2; k <= N; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t t = threadIdx.x; t < N / 2; t += blockDim.x) {
+        const uint32_t i = (t / j) * 2 * j + (t % j);
+        const uint32_t l = i + j;
+        const uint16_t a = idx[i], b = idx[l];
+        const bool up = (i & k) == 0;
+        if (up ? before(b, a) : before(a, b)) {
+          idx[i] = b;
+          idx[l] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+struct LocalPerm {
+  const uint16_t* ord2;
+  uint64_t first;
+  __device__ uint32_t operator()(uint32_t r) const { return static_cast<uint32_t>(first + ord2[r]); }
+};
+
+// Dynamic LDS for a tree of up to N (a power of two) points.
+__host__ __device__ constexpr size_t forest_lds_bytes(uint32_t N) {
+  return static_cast<size_t>(N) * (3 * 8 + 2 * 2 + 2 * 4) + kFinishBlock * 8 + (kFinishBlock / kWave) * 8;
+}
+
+template <typename T, typename I>
+__global__ __launch_bounds__(kFinishBlock) void k_forest_build(const Point<T, I>* __restrict__ pts, uint64_t n,
+                                                               uint32_t St, uint32_t L, uint32_t N,
+                                                               Node<T, typename IdStore<I>::type>* __restrict__ nodes,
+                                                               int* __restrict__ nan_flag) {
+  using IS = typename IdStore<I>::type;
+  extern __shared__ __align__(16) unsigned char lds[];
+  uint64_t* k1 = reinterpret_cast<uint64_t*>(lds);
+  uint64_t* k2 = k1 + N;
+  uint64_t* kid = k2 + N;
+  unsigned long long* sres = reinterpret_cast<unsigned long long*>(kid + N);
+  unsigned long long* wred = sres + kFinishBlock;
+  uint32_t* bufA = reinterpret_cast<uint32_t*>(wred + kFinishBlock / kWave);
+  uint32_t* bufB = bufA + N;
+  uint16_t* o1 = reinterpret_cast<uint16_t*>(bufB + N);
+  uint16_t* o2 = o1 + N;
+
+  const uint64_t c = blockIdx.x;
+  const uint64_t first = c * St;
+  const uint32_t m = static_cast<uint32_t>(n - first < St ? n - first : St);
+  for (uint32_t x = threadIdx.x; x < N; x += blockDim.x) {
+    o1[x] = static_cast<uint16_t>(x);
+    o2[x] = static_cast<uint16_t>(x);
+    if (x < m) {
+      const Point<T, I> p = pts[first + x];
+      bool bad = is_nan(p.dim1_val) || is_nan(p.dim2_val);
+      k1[x] = static_cast<uint64_t>(ord_key(p.dim1_val));
+      k2[x] = static_cast<uint64_t>(ord_key(p.dim2_val));
+      if constexpr (has_id_v<I>) {
+        bad |= is_nan(p.id);
+        kid[x] = static_cast<uint64_t>(ord_key(p.id));
+      } else {
+        kid[x] = 0;
+      }
+      if (bad) *nan_flag = 1;
+    }
+  }
+  __syncthreads();
+  // key1 = (dim1, id, index) ascending; key2 = (dim2, id, index) descending
+  // (rank 0 is the node point), the reference's compareDim1/compareDim2 with
+  // comparisonTiebreaker (helpers/point-struct.h:54-92, 146-188)
+  bitonic_sort(o1, N, [&](uint16_t a, uint16_t b) {
+    if (a >= m) return false;
+    if (b >= m) return true;
+    if (k1[a] != k1[b]) return k1[a] < k1[b];
+    if (kid[a] != kid[b]) return kid[a] < kid[b];
+    return a < b;
+  });
+  bitonic_sort(o2, N, [&](uint16_t a, uint16_t b) {
+    if (a >= m) return false;
+    if (b >= m) return true;
+    if (k2[a] != k2[b]) return k2[a] > k2[b];
+    if (kid[a] != kid[b]) return kid[a] > kid[b];
+    return a > b;
+  });
+  for (uint32_t r = threadIdx.x; r < m; r += blockDim.x) bufB[o2[r]] = r;
+  __syncthreads();
+  for (uint32_t x = threadIdx.x; x < m; x += blockDim.x) bufA[x] = bufB[o1[x]];
+  __syncthreads();
+  lds_build<T, IS>(bufA, bufB, sres, wred, m, 0, reinterpret_cast<const Point<T, IS>*>(pts), LocalPerm{o2, first},
+                   nodes + (c << L));
+}
+
+// Forest export: per tree the reference's layout for its own slot count,
+// trees concatenated (tree c at position c*St of every array).
+template <typename T, typename IS>
+__global__ void k_forest_export(const Node<T, IS>* __restrict__ nodes, uint64_t n, uint64_t St, uint32_t L,
+                                uint64_t ntrees, uint64_t total, unsigned char* __restrict__ blob, size_t o1,
+                                size_t o2, size_t om, size_t oi, size_t ob) {
+  const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= total) return;
+  uint64_t c = p / St;
+  if (c >= ntrees) c = ntrees - 1;
+  const uint64_t s = p - c * St;
+  const uint64_t mc = (c + 1 < ntrees) ? St : n - c * St;
+  const int64_t m = subtree_size(mc + 1, s);
+  T d1 = T(0), d2 = T(0), md = T(0);
+  unsigned char bc = 0;
+  const Node<T, IS>* nd = nodes + (c << L) + s;
+  if (m >= 1) {
+    d1 = nd->dim1;
+    md = nd->med;
+    d2 = nd->dim2;
+    bc = static_cast<unsigned char>((m >= 2 ? 2 : 0) | (m >= 3 ? 1 : 0));
+  }
+  reinterpret_cast<T*>(blob + o1)[p] = d1;
+  reinterpret_cast<T*>(blob + o2)[p] = d2;
+  reinterpret_cast<T*>(blob + om)[p] = md;
+  if constexpr (has_id_v<IS>) {
+    IS id = 0;
+    if (m >= 1) id = nd->id;
+    reinterpret_cast<IS*>(blob + oi)[p] = id;
+  }
+  blob[ob + p] = bc;
+}
+
 // Reference-layout export: dim1[S] dim2[S] med[S] ids[S] bitcodes[S].
 template <typename T, typename IS>
 __global__ void k_export(const Node<T, IS>* __restrict__ nodes, uint64_t S,
@@ -542,6 +677,51 @@ Status export_typed(size_t n, const void* nodes, void* blob, size_t o1, size_t o
   return hip_status(hipGetLastError(), "Error in launching tree export");
 }
 
+template <typename T, typename I>
+Status forest_build_typed(const void* pts_v, size_t n, size_t St, void* nodes_v, hipStream_t s) {
+  using IS = typename IdStore<I>::type;
+  if (n == 0) return Status{};
+  uint32_t L = 0;
+  while ((1ull << L) < St + 1) L++;
+  if ((1ull << L) != St + 1 || St > kFinishTile - 1)
+    return Status{1, "Error: forest trees hold 2^k - 1 <= 2047 points"};
+  const uint64_t ntrees = (n + St - 1) / St;
+  const uint32_t N = static_cast<uint32_t>(St + 1);
+  const size_t lds = forest_lds_bytes(N);
+  auto kern = k_forest_build<T, I>;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+  if (e != hipSuccess) return hip_status(e, "Error in sizing forest construction LDS");
+  DevBuf flag;
+  e = flag.alloc(sizeof(int));
+  if (e == hipSuccess) e = hipMemsetAsync(flag.p, 0, sizeof(int), s);
+  if (e != hipSuccess) return hip_status(e, "Error in allocating construction flag");
+  hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(ntrees)), dim3(kFinishBlock), lds, s,
+                     static_cast<const Point<T, I>*>(pts_v), static_cast<uint64_t>(n), static_cast<uint32_t>(St), L,
+                     N, static_cast<Node<T, IS>*>(nodes_v), flag.as<int>());
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_status(e, "Error in launching forest construction");
+  int hflag = 0;
+  e = hipMemcpyAsync(&hflag, flag.p, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_status(e, "Error in synchronising after forest construction");
+  if (hflag) return Status{3, "Error: input contains NaN values, which have no order"};
+  return Status{};
+}
+
+template <typename T, typename IS>
+Status forest_export_typed(size_t n, size_t St, const void* nodes, void* blob, size_t total, size_t o1, size_t o2,
+                           size_t om, size_t oi, size_t ob, hipStream_t s) {
+  if (n == 0) return Status{};
+  uint32_t L = 0;
+  while ((1ull << L) < St + 1) L++;
+  const uint64_t ntrees = (n + St - 1) / St;
+  k_forest_export<T, IS><<<grid_for(total, 256), 256, 0, s>>>(static_cast<const Node<T, IS>*>(nodes), n, St, L,
+                                                              ntrees, total, static_cast<unsigned char*>(blob), o1,
+                                                              o2, om, oi, ob);
+  return hip_status(hipGetLastError(), "Error in launching forest export");
+}
+
 template <typename F>
 Status with_t(int dt, F&& f) {
   switch (dt) {
@@ -583,6 +763,31 @@ Status export_tree(int dtype, int id_dtype, size_t n, const void* nodes_d, void*
     return with_id(id_dtype, [&](auto i) {
       using IS = typename IdStore<decltype(i)>::type;
       return export_typed<decltype(t), IS>(n, nodes_d, blob_d, o1, o2, om, oi, ob, stream);
+    });
+  });
+}
+
+}  // namespace pst
+
+namespace pst {
+
+Status build_forest(int dtype, int id_dtype, const void* pts_d, size_t n, size_t slots_per_tree, void* nodes_d,
+                    hipStream_t stream) {
+  return with_t(dtype, [&](auto t) {
+    return with_id(id_dtype, [&](auto i) {
+      return forest_build_typed<decltype(t), decltype(i)>(pts_d, n, slots_per_tree, nodes_d, stream);
+    });
+  });
+}
+
+Status export_forest(int dtype, int id_dtype, size_t n, size_t slots_per_tree, const void* nodes_d, void* blob_d,
+                     size_t total_slots, size_t o1, size_t o2, size_t om, size_t oi, size_t ob,
+                     hipStream_t stream) {
+  return with_t(dtype, [&](auto t) {
+    return with_id(id_dtype, [&](auto i) {
+      using IS = typename IdStore<decltype(i)>::type;
+      return forest_export_typed<decltype(t), IS>(n, slots_per_tree, nodes_d, blob_d, total_slots, o1, o2, om, oi,
+                                                  ob, stream);
     });
   });
 }

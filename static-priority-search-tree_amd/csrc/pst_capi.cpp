@@ -18,6 +18,7 @@ struct pst_tree {
   size_t n = 0, S = 0;
   int dev_ind = 0, num_devs = 1, device = 0;
   void* nodes = nullptr;
+  size_t slots_per_tree = 0;  // forest (StaticPSTGPUArr): slots of a full tree; 0 for one deep tree
   pst::SearchScratch sc;
   int rounds = 4, budget_small = 0, budget_large = 0, waves_per_cu = 0;
   int group = pst::kGroup;  // queries per walk in batched searches
@@ -60,6 +61,15 @@ size_t slots_for(size_t n) {
   return p - 1;
 }
 
+// Slots of a full forest tree: the smallest power of two above
+// threads_per_block, less one (the reference's calcNumElemSlotsPerTree with
+// minPowerOf2GreaterThan, pst/static-pst-gpu-arr.h:256-261).
+size_t forest_slots_per_tree(unsigned tpb) {
+  size_t p = 1;
+  while (p <= tpb) p <<= 1;
+  return p - 1;
+}
+
 // Per-device counter used by the blocking interval search.
 std::mutex g_ctr_mu;
 std::map<int, unsigned long long*> g_ctr;
@@ -98,6 +108,7 @@ pst::SearchArgs base_args(pst_tree* t) {
   a.id_dtype = t->id_dtype;
   a.n = t->n;
   a.nodes = t->nodes;
+  a.slots_per_tree = t->slots_per_tree;
   a.rounds = t->rounds;
   a.budget_small = t->budget_small;
   a.budget_large = t->budget_large;
@@ -117,7 +128,7 @@ int single_search(pst_tree* t, int kind, int ret, const void* a, const void* b, 
   args.nq = 1;
   args.mode = 1;
   args.ret_point = ret;
-  Status st = pst::scratch_reserve(t->sc, 1, t->dtype, s);
+  Status st = pst::scratch_reserve(t->sc, 1, 0, s);
   if (!st.ok()) return fail(st);
   args.counts = t->sc.cursor;
   args.cap_all = cap;
@@ -271,6 +282,50 @@ int pst_gpu_destroy(pst_tree* t) {
   return PST_OK;
 }
 
+int pst_gpu_arr_create(const void* pt_arr_d, size_t n, int dtype, int id_dtype, unsigned threads_per_block,
+                       int dev_ind, int num_devs, pst_stream_t stream, pst_tree** out) {
+  if (!out) return fail(PST_ERR_INVALID, "Error: null output handle");
+  *out = nullptr;
+  if (!valid_t(dtype) || !valid_id(id_dtype))
+    return fail(PST_ERR_INVALID, "Error: unsupported data/ID type codes " + std::to_string(dtype) + "/" +
+                                     std::to_string(id_dtype));
+  if (threads_per_block == 0 || threads_per_block > 1024)
+    return fail(PST_ERR_INVALID, "Error: threads_per_block must be in [1, 1024]");
+  if (n > 0 && !pt_arr_d) return fail(PST_ERR_INVALID, "Error: null point array");
+  if (n >= (1ull << 29)) return fail(PST_ERR_INVALID, "Error: StaticPSTGPUArr holds fewer than 2^29 points");
+  pst_tree* t = new pst_tree();
+  t->dtype = dtype;
+  t->id_dtype = id_dtype;
+  t->n = n;
+  t->slots_per_tree = forest_slots_per_tree(threads_per_block);
+  const size_t ntrees = n ? (n + t->slots_per_tree - 1) / t->slots_per_tree : 0;
+  t->S = ntrees * (t->slots_per_tree + 1);  // records: trees at a stride of slots_per_tree + 1
+  t->dev_ind = dev_ind;
+  t->num_devs = num_devs;
+  t->stream = reinterpret_cast<hipStream_t>(stream);
+  hipGetDevice(&t->device);
+  if (n == 0) {
+    *out = t;
+    return PST_OK;
+  }
+  hipError_t e = hipMalloc(&t->nodes, t->S * pst::node_size(dtype, id_dtype));
+  if (e != hipSuccess) {
+    std::string m = "Error in allocating array of PSTs on device " + std::to_string(dev_ind + 1) +
+                    " (1-indexed) of " + std::to_string(num_devs) + ": " + hipGetErrorString(e);
+    pst_gpu_destroy(t);
+    return fail(PST_ERR_HIP, m);
+  }
+  Status st = pst::build_forest(dtype, id_dtype, pt_arr_d, n, t->slots_per_tree, t->nodes, t->stream);
+  if (!st.ok()) {
+    pst_gpu_destroy(t);
+    return fail(st);
+  }
+  *out = t;
+  return PST_OK;
+}
+
+size_t pst_gpu_slots_per_tree(const pst_tree* t) { return t ? t->slots_per_tree : 0; }
+
 size_t pst_gpu_num_elems(const pst_tree* t) { return t ? t->n : 0; }
 int pst_gpu_dev_ind(const pst_tree* t) { return t ? t->dev_ind : 0; }
 int pst_gpu_num_devs(const pst_tree* t) { return t ? t->num_devs : 0; }
@@ -315,19 +370,40 @@ size_t pst_gpu_calc_global_mem_needed(size_t n, int dtype, int id_dtype) {
   return tree + (construct > search ? construct : search);
 }
 
+size_t pst_gpu_arr_calc_global_mem_needed(size_t n, unsigned threads_per_block, int dtype, int id_dtype) {
+  if (!valid_t(dtype) || !valid_id(id_dtype) || threads_per_block == 0 || threads_per_block > 1024) return 0;
+  const size_t St = forest_slots_per_tree(threads_per_block);
+  const size_t ntrees = n ? (n + St - 1) / St : 0;
+  const size_t tree = ntrees * (St + 1) * pst::node_size(dtype, id_dtype);
+  // construction works in LDS; a search needs a result array of n points
+  // and two 4M-entry work lists
+  const size_t search = n * pst::point_size(dtype, id_dtype) + (2ull << 22) * 16;
+  return tree + search;
+}
+
+// Reference-layout slot count of the export: one tree's S, or for a forest
+// the full trees' slots plus the last tree's own S.
+static size_t export_slots(const pst_tree* t) {
+  if (t->slots_per_tree == 0 || t->n == 0) return t->S;
+  const size_t St = t->slots_per_tree;
+  const size_t ntrees = (t->n + St - 1) / St;
+  return (ntrees - 1) * St + slots_for(t->n - (ntrees - 1) * St);
+}
+
 size_t pst_gpu_export_bytes(const pst_tree* t) {
   if (!t) return 0;
   const size_t sT = pst::dtype_size(t->dtype), sI = id_size(t->id_dtype);
-  size_t after = 3 * t->S * sT;
+  const size_t S = export_slots(t);
+  size_t after = 3 * S * sT;
   size_t off_id = sI ? ((after + sI - 1) / sI) * sI : after;
-  size_t off_bits = off_id + t->S * sI;
-  return ((off_bits + t->S + 7) / 8) * 8;
+  size_t off_bits = off_id + S * sI;
+  return ((off_bits + S + 7) / 8) * 8;
 }
 
 int pst_gpu_export_tree(pst_tree* t, void* host_blob, size_t blob_bytes, size_t* off6) {
   if (!t || !off6) return fail(PST_ERR_INVALID, "Error: null argument");
   const size_t sT = pst::dtype_size(t->dtype), sI = id_size(t->id_dtype);
-  const size_t S = t->S;
+  const size_t S = export_slots(t);
   off6[0] = 0;
   off6[1] = S * sT;
   off6[2] = 2 * S * sT;
@@ -341,8 +417,11 @@ int pst_gpu_export_tree(pst_tree* t, void* host_blob, size_t blob_bytes, size_t*
   void* d = nullptr;
   hipError_t e = hipMalloc(&d, need);
   if (e != hipSuccess) return fail(pst::hip_status(e, "Error in allocating tree export buffer"));
-  Status st = pst::export_tree(t->dtype, t->id_dtype, t->n, t->nodes, d, off6[0], off6[1], off6[2], off6[3],
-                               off6[4], t->stream);
+  Status st = t->slots_per_tree
+                  ? pst::export_forest(t->dtype, t->id_dtype, t->n, t->slots_per_tree, t->nodes, d, S, off6[0],
+                                       off6[1], off6[2], off6[3], off6[4], t->stream)
+                  : pst::export_tree(t->dtype, t->id_dtype, t->n, t->nodes, d, off6[0], off6[1], off6[2], off6[3],
+                                     off6[4], t->stream);
   if (st.ok()) st = pst::hip_status(hipMemcpyAsync(host_blob, d, need, hipMemcpyDeviceToHost, t->stream),
                                     "Error in copying tree export");
   if (st.ok()) st = pst::hip_status(hipStreamSynchronize(t->stream), "Error in synchronising tree export");
@@ -379,7 +458,7 @@ int pst_gpu_batch_write(pst_tree* t, const void* queries_d, size_t nq, int ret, 
   if (ret != PST_RET_POINT && t->id_dtype == PST_NO_ID)
     return fail(PST_ERR_INVALID, "Error: IDs requested from a tree built without IDs");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  Status st = pst::scratch_reserve(t->sc, nq, t->dtype, s);
+  Status st = pst::scratch_reserve(t->sc, nq, 0, s);
   if (!st.ok()) return fail(st);
   pst::SearchArgs a = base_args(t);
   a.queries = queries_d;
@@ -408,7 +487,7 @@ int pst_gpu_batch_search(pst_tree* t, const void* queries_d, size_t nq, int ret,
   Status st = pst::exclusive_scan(counts_d, offsets_d, nq, s);
   if (!st.ok()) return fail(st);
   if (t->n == 0) return PST_OK;
-  st = pst::scratch_reserve(t->sc, nq, t->dtype, s);
+  st = pst::scratch_reserve(t->sc, nq, 0, s);
   if (!st.ok()) return fail(st);
   pst::SearchArgs a = base_args(t);
   a.queries = queries_d;

@@ -142,6 +142,19 @@ __host__ __device__ inline int64_t subtree_size(uint64_t M, uint64_t slot) {
   return level_size(M, t, j);
 }
 
+// subtree_size for slot < 2^31 and M < 2^31, in 32-bit arithmetic (the
+// walks' per-visit form).
+__host__ __device__ inline int32_t subtree_size32(uint32_t M, uint32_t slot) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint32_t t = 31u - static_cast<uint32_t>(__clz(static_cast<int>(slot + 1)));
+#else
+  const uint32_t t = 31u - static_cast<uint32_t>(__builtin_clz(slot + 1));
+#endif
+  const uint32_t j = slot + 1 - (1u << t);
+  const uint32_t r = M & ((1u << t) - 1u);
+  return static_cast<int32_t>(M >> t) - 1 + (brev_bits(j, t) < r ? 1 : 0);
+}
+
 // Number of points held by level-t positions 0..j-1 (their segments are
 // consecutive in key1 order), in O(t): position j' < j is grouped by the
 // highest bit where it differs from j; each group is a set of positions
@@ -199,7 +212,7 @@ struct StepOut {
 //   THREE_SEARCH -> lo <= med <= hi ? (left RIGHT, right LEFT)
 //                   : hi < med ? left THREE : med < lo ? right THREE : none.
 template <typename T, int G>
-__host__ __device__ inline StepOut node_step(T d1, T d2, T md, int64_t m, const Bounds<T>* qb, uint32_t a,
+__host__ __device__ inline StepOut node_step(T d1, T d2, T md, int32_t m, const Bounds<T>* qb, uint32_t a,
                                              uint32_t l, uint32_t h) {
   uint32_t ge2 = 0, d1lo = 0, d1hi = 0, mdhi = 0, mdlo = 0, himd = 0, mdltlo = 0;
 #pragma unroll
@@ -234,7 +247,7 @@ __host__ __device__ inline StepOut node_step(T d1, T d2, T md, int64_t m, const 
 // node_step for items with no open bound (every query REPORT_ABOVE): only
 // the dim2 prune applies, and both children inherit the surviving queries.
 template <typename T, int G>
-__host__ __device__ inline StepOut node_step_report_above(T d2, int64_t m, const Bounds<T>* qb, uint32_t a) {
+__host__ __device__ inline StepOut node_step_report_above(T d2, int32_t m, const Bounds<T>* qb, uint32_t a) {
   uint32_t ge2 = 0;
 #pragma unroll
   for (int j = 0; j < G; j++) ge2 |= (qb[j].min2 <= d2) ? (1u << j) : 0u;

@@ -45,6 +45,17 @@ Status build_tree(int dtype, int id_dtype, const void* pts_d, size_t n, void* no
 Status export_tree(int dtype, int id_dtype, size_t n, const void* nodes_d, void* blob_d, size_t off_dim1,
                    size_t off_dim2, size_t off_med, size_t off_id, size_t off_bits, hipStream_t stream);
 
+// ---- forest of shallow trees (pst_build.hip) ----
+// Tree c holds points [c*St, min(n, (c+1)*St)), St = 2^L - 1 <= 2047; its
+// records start at slot c << L of `nodes` (ceil(n/St) << L records).
+Status build_forest(int dtype, int id_dtype, const void* pts_d, size_t n, size_t slots_per_tree, void* nodes_d,
+                    hipStream_t stream);
+// Per-tree reference layouts, trees concatenated: tree c at position c*St
+// of each array; total_slots = (trees - 1)*St + S(last tree's points).
+Status export_forest(int dtype, int id_dtype, size_t n, size_t slots_per_tree, const void* nodes_d, void* blob_d,
+                     size_t total_slots, size_t off_dim1, size_t off_dim2, size_t off_med, size_t off_id,
+                     size_t off_bits, hipStream_t stream);
+
 // ---- searches on one deep tree (pst_search.hip) ----
 constexpr int kGroup = 8;  // queries per group walk in batched searches
 struct SearchScratch {
@@ -61,7 +72,8 @@ struct SearchScratch {
   void* gstack = nullptr;  // per-wave overflow stacks of the walks
   size_t gstack_bytes = 0;
 };
-Status scratch_reserve(SearchScratch& sc, size_t nq, int dtype, hipStream_t stream);
+// Work lists for nq queries whose walks start from `roots` root items.
+Status scratch_reserve(SearchScratch& sc, size_t nq, size_t roots, hipStream_t stream);
 void scratch_release(SearchScratch& sc);
 
 // mode 0: count into counts_d[q]; mode 1: write results for query q at
@@ -71,6 +83,7 @@ struct SearchArgs {
   int dtype, id_dtype;
   size_t n;
   const void* nodes;
+  size_t slots_per_tree;  // 0: one deep tree; else a forest of trees of this many slots
   const void* queries;  // device Query<T>[nq]
   size_t nq;
   int mode;             // 0 count, 1 write

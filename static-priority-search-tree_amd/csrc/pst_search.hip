@@ -62,10 +62,8 @@ struct Items;
 template <>
 struct Items<1> {
   using type = uint32_t;
-  __device__ static type make(uint64_t slot, uint32_t, uint32_t l, uint32_t h) {
-    return static_cast<uint32_t>(slot << 2) | l | (h << 1);
-  }
-  __device__ static uint64_t slot(type it) { return it >> 2; }
+  __device__ static type make(uint32_t slot, uint32_t, uint32_t l, uint32_t h) { return (slot << 2) | l | (h << 1); }
+  __device__ static uint32_t slot(type it) { return it >> 2; }
   __device__ static uint32_t a(type) { return 1u; }
   __device__ static uint32_t l(type it) { return it & 1u; }
   __device__ static uint32_t h(type it) { return (it >> 1) & 1u; }
@@ -73,11 +71,10 @@ struct Items<1> {
 template <>
 struct Items<8> {
   using type = uint64_t;
-  __device__ static type make(uint64_t slot, uint32_t a, uint32_t l, uint32_t h) {
-    return slot | (static_cast<uint64_t>(a) << 32) | (static_cast<uint64_t>(l) << 40) |
-           (static_cast<uint64_t>(h) << 48);
+  __device__ static type make(uint32_t slot, uint32_t a, uint32_t l, uint32_t h) {
+    return static_cast<uint64_t>(slot) | (static_cast<uint64_t>(a | (l << 8) | (h << 16)) << 32);
   }
-  __device__ static uint64_t slot(type it) { return it & 0xffffffffull; }
+  __device__ static uint32_t slot(type it) { return static_cast<uint32_t>(it); }
   __device__ static uint32_t a(type it) { return static_cast<uint32_t>(it >> 32) & 0xffu; }
   __device__ static uint32_t l(type it) { return static_cast<uint32_t>(it >> 40) & 0xffu; }
   __device__ static uint32_t h(type it) { return static_cast<uint32_t>(it >> 48) & 0xffu; }
@@ -86,7 +83,6 @@ struct Items<8> {
 template <typename T, typename IS>
 struct Params {
   const Node<T, IS>* nodes;
-  uint64_t M;  // n + 1
   const Query<T>* queries;
   const uint32_t* gq;  // G query indices per group (~0u: none); null when G == 1
   Entry* list[2];
@@ -101,7 +97,12 @@ struct Params {
   unsigned long long* err;  // bit 0: stack would overflow; bit 1: invalid work item
   unsigned long long* dbg;  // optional per-round counters (nullable)
   Query<T> single;          // the query when `queries` is null (one-query searches)
-  uint64_t S;               // slot count: items outside [0, S) are rejected
+  uint32_t S;               // slot count (< 2^30): items outside [0, S) are rejected
+  // Forest of trees: slot = tree << tree_shift | slot within the tree; tree
+  // c holds full_M - 1 points, the last one last_M - 1. One deep tree:
+  // tree_shift 31, ntrees 1, last_M = n + 1.
+  uint32_t tree_shift;
+  uint32_t ntrees, full_M, last_M;
   uint32_t nq;              // query count
   uint32_t ngroups;         // groups: items of other groups are rejected
   void* gstack;              // per-wave overflow stacks, gcap items each
@@ -133,11 +134,14 @@ __host__ __device__ inline uint32_t hi_open(uint32_t kind) {
 
 template <typename T, int G>
 __global__ void k_search_init(const Query<T>* __restrict__ queries, Query<T> single, const uint32_t* gq,
-                              uint32_t nq, uint32_t ngroups, Entry* __restrict__ roots,
-                              unsigned long long* __restrict__ ctr, uint32_t nctr,
+                              uint32_t nq, uint32_t ngroups, uint64_t ntrees, uint32_t tree_shift,
+                              Entry* __restrict__ roots, unsigned long long* __restrict__ ctr, uint32_t nctr,
                               unsigned long long* __restrict__ counts, unsigned long long* err) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < ngroups) {
+  const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  // root entry k: group k / ntrees at the root of tree k % ntrees
+  if (k < ngroups * ntrees) {
+    const uint32_t i = static_cast<uint32_t>(k / ntrees);
+    const uint64_t tree = k - static_cast<uint64_t>(i) * ntrees;
     uint64_t item;
     if constexpr (G == 1) {
       const uint32_t kind = queries ? queries[i].kind : single.kind;
@@ -155,11 +159,13 @@ __global__ void k_search_init(const Query<T>* __restrict__ queries, Query<T> sin
       }
       item = (static_cast<uint64_t>(a) << 32) | (static_cast<uint64_t>(l) << 40) | (static_cast<uint64_t>(h) << 48);
     }
-    roots[i] = Entry{item, i};
+    const uint64_t root = tree << tree_shift;
+    item |= (G == 1) ? (root << 2) : root;
+    roots[k] = Entry{item, i};
   }
-  if (i < nq) counts[i] = 0;
-  if (i < nctr) ctr[i] = (i == 0) ? ngroups : 0;
-  if (i == 0) *err = 0;
+  if (k < nq) counts[k] = 0;
+  if (k < nctr) ctr[k] = (k == 0) ? ngroups * ntrees : 0;
+  if (k == 0) *err = 0;
 }
 
 template <typename T, typename IS, bool RET_POINT>
@@ -265,34 +271,25 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
       if (qid < P.nq) qv = P.queries ? P.queries[qid] : P.single;
     }
     Bounds<T> qb[G];
-    uint32_t qi[G];
 #pragma unroll
     for (int j = 0; j < G; j++) {
       qb[j].lo = read_lane(qv.min_dim1, j);
       qb[j].hi = read_lane(qv.max_dim1, j);
       qb[j].min2 = read_lane(qv.min_dim2, j);
-      qi[j] = read_lane(qid, j);
     }
-    unsigned long long found[G];  // results of this batch (count mode)
-    uint32_t nst[G];              // staged results (write mode)
-    unsigned long long off[G], cap[G];
+    uint32_t found[G];  // results of this batch (count mode; a batch reports < 2^32 per query)
+    uint32_t nst[G];    // staged results (write mode)
 #pragma unroll
     for (int j = 0; j < G; j++) {
       found[j] = 0;
       nst[j] = 0;
-      off[j] = 0;
-      cap[j] = 0;
     }
+    // lane j keeps query j's output offset and capacity (write mode)
+    unsigned long long qoff = 0, qcap = 0;
     if constexpr (WRITE) {
-      unsigned long long o = 0, c = 0;
       if (lane < G && qid < P.nq) {
-        o = P.offsets ? P.offsets[qid] : 0;
-        c = P.caps ? P.caps[qid] : P.cap_all;
-      }
-#pragma unroll
-      for (int j = 0; j < G; j++) {
-        off[j] = read_lane(o, j);
-        cap[j] = read_lane(c, j);
+        qoff = P.offsets ? P.offsets[qid] : 0;
+        qcap = P.caps ? P.caps[qid] : P.cap_all;
       }
     }
     // Flushes every query whose stage holds at least `thr` results: lanes
@@ -307,15 +304,25 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
           if (lane == static_cast<uint32_t>(j)) mine = nst[j];
         }
         if (fl == 0) return;
+        // lane j: first free index of query j's range after the reservation,
+        // and how many of its staged records still fit
         unsigned long long b = 0;
-        if (lane < G && ((fl >> lane) & 1u)) b = atomicAdd(&P.counts[qid], static_cast<unsigned long long>(mine));
+        uint32_t fit = 0;
+        if (lane < G && ((fl >> lane) & 1u)) {
+          b = atomicAdd(&P.counts[qid], static_cast<unsigned long long>(mine));
+          const unsigned long long room_q = b < qcap ? qcap - b : 0;
+          const unsigned long long at = qoff + b;
+          const unsigned long long room_o = at < P.out_total ? P.out_total - at : 0;
+          const unsigned long long r = room_q < room_o ? room_q : room_o;
+          fit = static_cast<uint32_t>(r < mine ? r : mine);
+          b = at;
+        }
 #pragma unroll
         for (int j = 0; j < G; j++) {
           if (!((fl >> j) & 1u)) continue;
-          const unsigned long long bj = read_lane(b, j);
-          Rec* out = static_cast<Rec*>(P.out) + off[j];
-          for (uint32_t x = lane; x < nst[j]; x += kWave)
-            if (bj + x < cap[j] && off[j] + bj + x < P.out_total) out[bj + x] = stage[j * kStage + x];
+          Rec* out = static_cast<Rec*>(P.out) + read_lane(b, j);
+          const uint32_t nfit = read_lane(fit, j);
+          for (uint32_t x = lane; x < nfit; x += kWave) out[x] = stage[j * kStage + x];
           nst[j] = 0;
         }
         __builtin_amdgcn_wave_barrier();
@@ -376,11 +383,14 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
       const bool act = lane < takecnt;
       const Item item = act ? stack[sp + lane] : Item(0);
       __builtin_amdgcn_wave_barrier();
-      const uint64_t slot = It::slot(item);
-      int64_t m = 0;
+      const uint32_t slot = It::slot(item);
+      int32_t m = 0;
       bool ok = act;
+      uint32_t tbase = 0;  // first slot of the item's tree
       if (act) {
-        m = subtree_size(P.M, slot);
+        const uint32_t tree = slot >> P.tree_shift;
+        tbase = tree << P.tree_shift;
+        m = subtree_size32(tree + 1 < P.ntrees ? P.full_M : P.last_M, slot - tbase);
         if (slot >= P.S || m < 1) {
           atomicOr(P.err, 2ull);
           ok = false;
@@ -433,8 +443,9 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
       const bool cl = so.al != 0, cr = so.ar != 0;
       const uint64_t lm = __ballot(cl), rm = __ballot(cr);
       const uint32_t pos = sp + lanes_below(lm) + lanes_below(rm);
-      if (cl) stack[pos] = It::make(2 * slot + 1, so.al, so.ll, so.hl);
-      if (cr) stack[pos + (cl ? 1u : 0u)] = It::make(2 * slot + 2, so.ar, so.lr, so.hr);
+      const uint32_t lchild = tbase + 2 * (slot - tbase) + 1;
+      if (cl) stack[pos] = It::make(lchild, so.al, so.ll, so.hl);
+      if (cr) stack[pos + (cl ? 1u : 0u)] = It::make(lchild + 1, so.ar, so.lr, so.hr);
       sp += __popcll(lm) + __popcll(rm);
       __builtin_amdgcn_wave_barrier();
       // ---- hand remaining work to the next round when over budget
@@ -471,9 +482,11 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
     if constexpr (WRITE) {
       flush(1);
     } else {
+      uint32_t mine = 0;
 #pragma unroll
       for (int j = 0; j < G; j++)
-        if (lane == 0 && found[j]) atomicAdd(&P.counts[qi[j]], found[j]);
+        if (lane == static_cast<uint32_t>(j)) mine = found[j];
+      if (lane < G && mine) atomicAdd(&P.counts[qid], static_cast<unsigned long long>(mine));
     }
   }
 }
@@ -557,11 +570,36 @@ inline uint32_t group_size(const SearchArgs& a) {
   return (a.single_query || a.nq == 1 || a.group == 1) ? 1u : static_cast<uint32_t>(kGroup);
 }
 
+struct TreeShape {
+  uint64_t S, ntrees, full_M, last_M;
+  uint32_t shift, levels;
+};
+TreeShape tree_shape(const SearchArgs& a) {
+  TreeShape t{};
+  if (a.slots_per_tree == 0) {
+    t.S = (a.n == 0) ? 0 : ((1ull << (64 - __builtin_clzll(a.n))) - 1);
+    t.ntrees = 1;
+    t.full_M = t.last_M = a.n + 1;
+    t.shift = 31;
+  } else {
+    const uint64_t St = a.slots_per_tree;
+    t.shift = 0;
+    while ((1ull << t.shift) < St + 1) t.shift++;
+    t.ntrees = (a.n + St - 1) / St;
+    t.full_M = St + 1;
+    t.last_M = a.n - (t.ntrees - 1) * St + 1;
+    t.S = t.ntrees << t.shift;
+  }
+  t.levels = 0;
+  const uint64_t biggest = t.ntrees > 1 ? t.full_M - 1 : t.last_M - 1;
+  while ((1ull << t.levels) <= biggest) t.levels++;
+  return t;
+}
+
 template <typename T, typename IS, bool RP, bool W, int G>
 Status launch_rounds(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
   Params<T, IS> P{};
   P.nodes = static_cast<const Node<T, IS>*>(a.nodes);
-  P.M = a.n + 1;
   P.queries = static_cast<const Query<T>*>(a.queries);
   if (a.single_query) P.single = *static_cast<const Query<T>*>(a.single_query);
   P.gq = G == 1 ? nullptr : sc.groups;
@@ -577,11 +615,15 @@ Status launch_rounds(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
   P.out = a.out;
   P.err = sc.ctr + 2 * kMaxRounds + 2;
   P.dbg = a.debug;
-  P.S = (a.n == 0) ? 0 : ((1ull << (64 - __builtin_clzll(a.n))) - 1);
+  const TreeShape sh = tree_shape(a);
+  P.S = static_cast<uint32_t>(sh.S);
+  P.tree_shift = sh.shift;
+  P.ntrees = static_cast<uint32_t>(sh.ntrees);
+  P.full_M = static_cast<uint32_t>(sh.full_M);
+  P.last_M = static_cast<uint32_t>(sh.last_M);
   P.nq = static_cast<uint32_t>(a.nq);
   P.ngroups = static_cast<uint32_t>((a.nq + G - 1) / G);
-  uint32_t H = 0;
-  while ((1ull << H) <= a.n) H++;  // levels
+  const uint32_t H = sh.levels;
   // deepest stack: 64 items per level below the first popped batch, plus a
   // step's growth, in whole move blocks
   const uint32_t bound = kWave * (H + 2) + 2 * kWave;
@@ -662,10 +704,9 @@ Status make_groups(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
 
 }  // namespace
 
-Status scratch_reserve(SearchScratch& sc, size_t nq, int dtype, hipStream_t) {
-  (void)dtype;
+Status scratch_reserve(SearchScratch& sc, size_t nq, size_t roots, hipStream_t) {
   const size_t spill_cap = 1ull << 22;  // 4M items per round list
-  const size_t need = std::max(nq, spill_cap);
+  const size_t need = std::max(std::max(nq, roots), spill_cap);
   if (sc.list_cap < need) {
     if (sc.list[0]) hipFree(sc.list[0]);
     if (sc.list[1]) hipFree(sc.list[1]);
@@ -726,11 +767,12 @@ Status init_typed(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
   if (a.single_query) one = *static_cast<const Query<T>*>(a.single_query);
   const uint32_t nctr = 2 * kMaxRounds + 2;
   const uint32_t ngroups = static_cast<uint32_t>((a.nq + G - 1) / G);
-  const uint32_t work = static_cast<uint32_t>(std::max<size_t>(a.nq, nctr));
-  k_search_init<T, G><<<(work + 255) / 256, 256, 0, s>>>(
+  const TreeShape sh = tree_shape(a);
+  const uint64_t work = std::max<uint64_t>(std::max<uint64_t>(a.nq, nctr), ngroups * sh.ntrees);
+  k_search_init<T, G><<<static_cast<unsigned>((work + 255) / 256), 256, 0, s>>>(
       a.single_query ? nullptr : static_cast<const Query<T>*>(a.queries), one, G == 1 ? nullptr : sc.groups,
-      static_cast<uint32_t>(a.nq), ngroups, static_cast<Entry*>(sc.list[0]), sc.ctr, nctr, a.counts,
-      sc.ctr + 2 * kMaxRounds + 2);
+      static_cast<uint32_t>(a.nq), ngroups, sh.ntrees, sh.shift, static_cast<Entry*>(sc.list[0]), sc.ctr, nctr,
+      a.counts, sc.ctr + 2 * kMaxRounds + 2);
   return hip_status(hipGetLastError(), "Error in launching search initialisation");
 }
 
@@ -740,10 +782,11 @@ Status init_typed(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
 Status run_search(const SearchArgs& a_in, SearchScratch& sc, hipStream_t s) {
   SearchArgs a = a_in;
   if (a.nq == 0) return Status{};
-  Status st = scratch_reserve(sc, a.nq, a.dtype, s);
-  if (!st.ok()) return st;
   if (a.single_query) a.queries = nullptr;
   const uint32_t G = group_size(a);
+  const size_t roots = ((a.nq + G - 1) / G) * tree_shape(a).ntrees;
+  Status st = scratch_reserve(sc, a.nq, roots, s);
+  if (!st.ok()) return st;
   return with_t(a.dtype, [&](auto t) -> Status {
     using T = decltype(t);
     return with_is(a.id_dtype, [&](auto i) -> Status {

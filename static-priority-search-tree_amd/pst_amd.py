@@ -34,6 +34,7 @@ EXPORTED = [
     "pst_gpu_two_sided_right_search", "pst_gpu_search_into", "pst_gpu_calc_global_mem_needed",
     "pst_gpu_export_tree", "pst_gpu_export_bytes", "pst_gpu_batch_count", "pst_gpu_batch_write",
     "pst_gpu_batch_search", "pst_gpu_set_search_tuning", "pst_gpu_set_query_grouping",
+    "pst_gpu_arr_create", "pst_gpu_arr_calc_global_mem_needed", "pst_gpu_slots_per_tree",
     "pst_interval_parallel_search",
     "pst_interval_parallel_search_into",
 ]
@@ -84,6 +85,11 @@ def lib():
         L.pst_free.argtypes = [vp]
         L.pst_gpu_create.argtypes = [vp, sz, i32, i32, i32, i32, i32, vp, c.POINTER(vp)]
         L.pst_gpu_destroy.argtypes = [vp]
+        L.pst_gpu_arr_create.argtypes = [vp, sz, i32, i32, c.c_uint, i32, i32, vp, c.POINTER(vp)]
+        L.pst_gpu_arr_calc_global_mem_needed.restype = sz
+        L.pst_gpu_arr_calc_global_mem_needed.argtypes = [sz, c.c_uint, i32, i32]
+        L.pst_gpu_slots_per_tree.restype = sz
+        L.pst_gpu_slots_per_tree.argtypes = [vp]
         L.pst_gpu_num_elems.restype = sz
         L.pst_gpu_num_elems.argtypes = [vp]
         L.pst_gpu_three_sided_search.argtypes = [vp, i32, c.POINTER(sz), c.POINTER(vp), vp, vp, vp]
@@ -281,6 +287,36 @@ class StaticPSTGPU:
     @staticmethod
     def calcGlobalMemNeeded(num_elems, dtype="f32", id_dtype="u32"):
         return lib().pst_gpu_calc_global_mem_needed(num_elems, DT[dtype], DT[id_dtype])
+
+
+class StaticPSTGPUArr(StaticPSTGPU):
+    """Drop-in for the reference's StaticPSTGPUArr (pst/static-pst-gpu-arr.h:45):
+    a forest of shallow trees over consecutive chunks of the input. Searches,
+    batches and export are those of StaticPSTGPU; the export concatenates the
+    trees' reference layouts (see forest_sizes)."""
+
+    def __init__(self, pt_arr_d, num_elems: int, threads_per_block: int, dtype: str = "f32",
+                 id_dtype: Optional[str] = "u32", dev_ind: int = 0, num_devs: int = 1, stream=None):
+        self.t, self.i, self.n = dtype, id_dtype, int(num_elems)
+        h = ctypes.c_void_p()
+        ptr = pt_arr_d.data_ptr() if hasattr(pt_arr_d, "data_ptr") else pt_arr_d
+        _check(lib().pst_gpu_arr_create(ptr, self.n, DT[dtype], DT[id_dtype], threads_per_block,
+                                        dev_ind, num_devs, _stream_ptr(stream), ctypes.byref(h)))
+        self.h = h
+        self.slots_per_tree = lib().pst_gpu_slots_per_tree(self.h)
+
+    def forest_sizes(self):
+        """Slot count of each tree in the export, in order."""
+        if self.n == 0:
+            return []
+        st = self.slots_per_tree
+        nt = (self.n + st - 1) // st
+        last = self.n - (nt - 1) * st
+        return [st] * (nt - 1) + [(1 << int(last).bit_length()) - 1]
+
+    @staticmethod
+    def calcGlobalMemNeeded(num_elems, threads_per_block, dtype="f32", id_dtype="u32"):
+        return lib().pst_gpu_arr_calc_global_mem_needed(num_elems, threads_per_block, DT[dtype], DT[id_dtype])
 
 
 def intervalParallelSearch(pt_arr_d, num_elems, search_val, dtype="f32", id_dtype="u32",

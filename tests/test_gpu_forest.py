@@ -1,0 +1,161 @@
+"""GPU parity of the forest of shallow trees (StaticPSTGPUArr, through the C
+ABI): each tree equals the oracle's tree of its input chunk (and the
+reference's own forest fixture), and searches report exactly the points
+that satisfy the predicate, including the right and three-sided searches
+that the reference leaves as stubs."""
+import numpy as np
+import pytest
+
+import golden_util as G
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    assert t.cuda.is_available()
+    return t
+
+
+@pytest.fixture(scope="module")
+def P():
+    import pst_amd
+    pst_amd.lib()
+    return pst_amd
+
+
+def _forest(P, pts, t, i, tpb):
+    import torch
+    d = P.to_device(pts) if len(pts) else None
+    f = P.StaticPSTGPUArr(d.data_ptr() if d is not None else 0, len(pts), tpb, t, i)
+    torch.cuda.synchronize()
+    return f, d
+
+
+def _check_trees(P, f, pts, t, i):
+    """Export of every tree equals the oracle's tree of its chunk."""
+    if len(pts) == 0:
+        return
+    ex = f.export_tree()
+    st = f.slots_per_tree
+    off = 0
+    for c, c0 in enumerate(range(0, len(pts), st)):
+        tr = O.build(pts[c0:c0 + st], t, i)
+        for k, v in tr.arrays().items():
+            got = ex[k][off:off + tr.S]
+            assert np.array_equal(G.bits_arr(got), G.bits_arr(v)), (c, k)
+        off += tr.S
+    assert off == len(ex["dim1"])
+
+
+def test_forest_fixture(torch, P):
+    fx = G.Fixture("forest_f32_u32_tpb64")
+    chunk = int(fx.forest["chunk"])
+    f, keep = _forest(P, fx.points, fx.t, fx.i, 64)
+    assert f.slots_per_tree == chunk
+    ex = f.export_tree()
+    assert f.forest_sizes() == [int(x) for x in fx.forest["sizes"]]
+    for k in ("dim1", "dim2", "med", "bits", "ids"):
+        assert np.array_equal(G.bits_arr(ex[k]), G.bits_arr(fx.forest[k])), k
+    for q, kind, rp, a, b, c in fx.queries():
+        res = f.search(kind, a, b, c, ret=P.RET_POINT if rp else P.RET_ID)
+        got = G.canon(res, True, True) if rp else G.canon(res, False, True)
+        assert got == fx.expected_sorted(q), (q, kind)
+    f.close()
+
+
+def _rand(rng, t, i, n):
+    dt = P_point_dtype(t, i)
+    p = np.zeros(n, dt)
+    if t in ("f32", "f64"):
+        a = rng.integers(0, 40, n).astype(np.float64) / 8
+        w = rng.integers(0, 16, n).astype(np.float64) / 8
+    else:
+        a = rng.integers(-50, 50, n)
+        w = rng.integers(0, 20, n)
+    p["dim1"] = a
+    p["dim2"] = a + w
+    if i is not None:
+        p["id"] = rng.integers(0, max(2, n // 3), n)
+    return p
+
+
+def P_point_dtype(t, i):
+    import pst_amd
+    return pst_amd.point_dtype(t, i)
+
+
+FOREST_CASES = [("f32", "u32", 64, 5000), ("f32", "u32", 1, 300), ("f32", "u32", 31, 3001),
+                ("i32", "u32", 100, 20000), ("f64", None, 1024, 9000), ("i64", "i64", 128, 4099),
+                ("f32", "u64", 255, 7777), ("f64", "f32", 64, 2000), ("f32", "u32", 64, 127),
+                ("f32", "u32", 64, 128), ("f32", "u32", 64, 1), ("f32", "u32", 64, 0)]
+
+
+@pytest.mark.parametrize("t,i,tpb,n", FOREST_CASES)
+def test_forest_random(torch, P, t, i, tpb, n):
+    rng = np.random.default_rng(n + tpb)
+    print(f"forest {t}/{i} tpb={tpb} n={n}", flush=True)
+    p = _rand(rng, t, i, n)
+    f, keep = _forest(P, p, t, i, tpb)
+    _check_trees(P, f, p, t, i)
+    lo, hi = (-1, 8) if t in ("f32", "f64") else (-60, 80)
+    for it in range(6):
+        a, b = np.sort(rng.uniform(lo, hi, 2))
+        c = rng.uniform(lo, hi)
+        if t in ("i32", "i64"):
+            a, b, c = int(a), int(b), int(c)
+        for kind in (0, 1, 2):
+            exp = p[O.brute_force(p, kind, a, b, c)]
+            rp = i is None or (it % 2 == 1)
+            if n == 0:
+                continue
+            res = f.search(kind, a, b, c, ret=P.RET_POINT if rp else P.RET_ID)
+            if rp:
+                assert G.canon(res, True, i is not None) == G.canon(exp, True, i is not None), (it, kind)
+            else:
+                assert G.canon(res, False, True) == G.canon(exp["id"], False, True), (it, kind)
+    f.close()
+
+
+@pytest.mark.parametrize("group", [8, 1])
+def test_forest_batch_isovalues(torch, P, group):
+    """Batched stabbing (isovalue) and mixed queries on a forest of 2^18
+    metacell-like intervals, count + scan + write in one call."""
+    import pst_data as D
+    n = (1 << 18) + 77
+    pts = D.interval_points(n, seed=8)
+    f, keep = _forest(P, pts, "f32", "u32", 64)
+    f.set_grouping(group)
+    qs = D.float_query_mix(40, 160, seed=9)
+    iso = np.linspace(0.0, 1.0, 41).astype(np.float32)
+    st = np.zeros(len(iso), qs.dtype)
+    st["min_dim1"] = st["max_dim1"] = st["min_dim2"] = iso
+    st["kind"] = 1
+    qs = np.concatenate([qs, st])
+    nq = len(qs)
+    print(f"forest batch nq={nq} group={group}", flush=True)
+    qd = P.to_device(qs)
+    exp = [pts["id"][O.brute_force(pts, int(q["kind"]), q["min_dim1"], q["max_dim1"], q["min_dim2"])]
+           for q in qs]
+    total = sum(len(e) for e in exp)
+    counts = torch.zeros(nq, dtype=torch.int64, device="cuda")
+    offs = torch.zeros(nq + 1, dtype=torch.int64, device="cuda")
+    out = torch.zeros(total + 8, dtype=torch.int32, device="cuda")
+    f.batch_search(qd, nq, counts, offs, out, total + 8)
+    torch.cuda.synchronize()
+    cnt, off = counts.cpu().numpy(), offs.cpu().numpy()
+    res = out.cpu().numpy().view(np.uint32)
+    assert off[nq] == total
+    for k in range(nq):
+        assert cnt[k] == len(exp[k]), k
+        assert np.array_equal(np.sort(res[off[k]:off[k] + cnt[k]]), np.sort(exp[k])), k
+    f.close()
+
+
+def test_forest_calc_global_mem_needed(P):
+    n, tpb = 1 << 20, 64
+    m = P.StaticPSTGPUArr.calcGlobalMemNeeded(n, tpb, "f32", "u32")
+    trees = (n + 126) // 127
+    assert m >= trees * 128 * 16 + n * 12
