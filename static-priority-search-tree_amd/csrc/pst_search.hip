@@ -635,10 +635,10 @@ Status launch_rounds(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
   hipGetDevice(&dev);
   int cus = 0;
   hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  // Waves per CU (one wave per workgroup). Measured on the 2^26-point
-  // batch: group walks stop gaining past about 16 per CU, one-query walks
-  // (more latency bound) gain up to 28.
-  int occ = G == 1 ? 28 : 16;
+  // Waves per CU (one wave per workgroup), measured on the 2^26-point batch:
+  // both walk kinds gain up to about 28-32 (launching more workgroups than
+  // fit at once also overlaps the round's tail with late starters).
+  int occ = G == 1 ? 28 : 32;
   if (a.waves_per_cu > 0) occ = std::min(a.waves_per_cu, 64);
   hipError_t e = hipSuccess;
   const uint32_t grid = static_cast<uint32_t>(cus * occ);

@@ -12,6 +12,7 @@
 
 #include "interval-parallel-search.h"
 #include "pst-point.h"
+#include "static-pst-gpu-arr.h"
 #include "static-pst-gpu.h"
 
 static int failures = 0;
@@ -119,12 +120,58 @@ void run_no_ids() {
   EXPECT(k == 0 && res == nullptr, "empty tree");
 }
 
+// StaticPSTGPUArr as the reference's dataset tester uses it
+// (testers/pst-tester-functions.tu:172-183): isovalue (q, q) left searches,
+// plus the right and three-sided searches the reference leaves empty.
+void run_forest(size_t n, unsigned tpb, unsigned seed) {
+  using Pt = PointStruct<float, unsigned, 1>;
+  std::mt19937 rng(seed);
+  std::uniform_real_distribution<float> u(0.0f, 1.0f);
+  std::vector<Pt> pts(n);
+  for (size_t i = 0; i < n; i++) {
+    const float a = u(rng);
+    pts[i] = Pt(a, a + 0.05f * u(rng), static_cast<unsigned>(i));
+  }
+  Pt* d = nullptr;
+  (void)hipMalloc(&d, n * sizeof(Pt));
+  (void)hipMemcpy(d, pts.data(), n * sizeof(Pt), hipMemcpyHostToDevice);
+  StaticPSTGPUArr<float, PointStruct, unsigned, 1> forest(d, n, tpb);
+  for (int q = 0; q < 10; q++) {
+    const float a = u(rng), b = a + 0.1f * u(rng), c = u(rng);
+    for (int kind = 0; kind < 3; kind++) {
+      size_t k = 0;
+      unsigned* ids = nullptr;
+      if (kind == 0) forest.template threeSidedSearch<unsigned>(k, ids, a, b, c);
+      if (kind == 1) forest.template twoSidedLeftSearch<unsigned>(k, ids, a, a);
+      if (kind == 2) forest.template twoSidedRightSearch<unsigned>(k, ids, a, c);
+      std::vector<unsigned> got = to_host<float>(ids, k), want;
+      (void)hipFree(ids);
+      for (auto& p : pts) {
+        const bool hit = kind == 0 ? (a <= p.dim1_val && p.dim1_val <= b && c <= p.dim2_val)
+                         : kind == 1 ? (p.dim1_val <= a && a <= p.dim2_val)
+                                     : (a <= p.dim1_val && c <= p.dim2_val);
+        if (hit) want.push_back(p.id);
+      }
+      std::sort(got.begin(), got.end());
+      std::sort(want.begin(), want.end());
+      EXPECT(got == want, "StaticPSTGPUArr kind=%d n=%zu tpb=%u q=%d", kind, n, tpb, q);
+    }
+  }
+  EXPECT(forest.calcGlobalMemNeeded(n, tpb) > 0, "StaticPSTGPUArr::calcGlobalMemNeeded");
+  std::ostringstream os;
+  os << forest;
+  EXPECT(os.str().find("└─(L)─ ") != std::string::npos, "StaticPSTGPUArr print draws trees");
+  (void)hipFree(d);
+}
+
 int main() {
   run_with_ids<float>(1000, 1);
   run_with_ids<float>(5000, 2);
   run_with_ids<int>(3000, 3);
   run_with_ids<double>(2049, 4);
   run_no_ids();
+  run_forest(10000, 64, 5);
+  run_forest(3001, 1000, 6);
   if (failures) {
     std::printf("DROPIN FAILED (%d)\n", failures);
     return 1;
