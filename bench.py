@@ -35,6 +35,104 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def iso_bench(args, P, D, torch, dev, stream):
+    """Isosurface form (StaticPSTGPUArr): a side^3 float32 volume (sum of 64
+    Gaussians + N(0, 0.01) noise, seed 1), 4^3-voxel metacells, one tree per
+    64-thread block, 200 isovalues evenly spaced over [vmin, vmax] (both
+    ends included) as twoSidedLeftSearch(q, q)."""
+    import numpy as np
+    side = args.iso_side
+    dims = (side, side, side)
+    prm = torch.from_numpy(P.gaussian_volume_params(64, 1).ravel()).to(dev)
+    vol = torch.empty(side ** 3, dtype=torch.float32, device=dev)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+    ev[0].record(stream)
+    P.synth_gaussian_volume(vol, dims, prm, 64, 0.01, 1, stream=stream)
+    ev[1].record(stream)
+    nmc, g = P.calcNumMetacells(dims, (4, 4, 4))
+    rec = P.point_dtype("f32", "u32")
+    mc = torch.empty(nmc * rec.itemsize, dtype=torch.uint8, device=dev)
+    P.form_metacells_into(vol, dims, (4, 4, 4), mc, stream=stream)  # warm
+    ev[2].record(stream)
+    P.form_metacells_into(vol, dims, (4, 4, 4), mc, stream=stream)
+    ev[3].record(stream)
+    torch.cuda.synchronize()
+    vmin, vmax = float(vol.min().item()), float(vol.max().item())
+    del vol
+    t0 = time.perf_counter()
+    forest = P.StaticPSTGPUArr(mc.data_ptr(), nmc, 64, "f32", "u32")
+    torch.cuda.synchronize()
+    forest_ms = (time.perf_counter() - t0) * 1e3
+    iso = np.linspace(vmin, vmax, args.iso_queries).astype(np.float32)
+    qs = np.zeros(len(iso), P.query_dtype("f32"))
+    qs["min_dim1"] = qs["max_dim1"] = qs["min_dim2"] = iso
+    qs["kind"] = 1
+    nq = len(qs)
+    d_q = P.to_device(qs, dev)
+    counts = torch.zeros(nq, dtype=torch.int64, device=dev)
+    offsets = torch.zeros(nq + 1, dtype=torch.int64, device=dev)
+    forest.batch_count(d_q, nq, counts, stream=stream)
+    torch.cuda.synchronize()
+    total = int(counts.sum().item())
+    out = torch.empty(total + 1, dtype=torch.int32, device=dev)
+    forest.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
+    torch.cuda.synchronize()
+    reps = 3
+    ev[4].record(stream)
+    for _ in range(reps):
+        forest.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
+    ev[5].record(stream)
+    torch.cuda.synchronize()
+    batch_ms = ev[4].elapsed_time(ev[5]) / reps
+    ok = int(offsets[nq].item()) == total
+    # spot check against the metacells themselves
+    cells = mc.cpu().numpy().view(rec)
+    host_counts = counts.cpu().numpy()
+    for k in (0, nq // 3, nq // 2, nq - 1):
+        ok &= int(np.count_nonzero((cells["dim1"] <= iso[k]) & (iso[k] <= cells["dim2"]))) == int(host_counts[k])
+    # plain path and IPS on a sample of the isovalues
+    sel = np.linspace(0, nq - 1, min(nq, args.iso_plain)).astype(int)
+    cap = int(host_counts.max()) + 1
+    pout = torch.empty(cap, dtype=torch.int32, device=dev)
+    forest.search_into(1, 0, iso[sel[0]], iso[sel[0]], pout, cap)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in sel:
+        ok &= forest.search_into(1, 0, iso[k], iso[k], pout, cap) == int(host_counts[k])
+    plain_qps = len(sel) / (time.perf_counter() - t0)
+    ctr = torch.zeros(1, dtype=torch.int64, device=dev)
+    P.ips_into(mc, nmc, iso[sel[0]], pout, cap, ctr, stream=stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for k in sel:
+        P.ips_into(mc, nmc, iso[k], pout, cap, ctr, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ips_ms = e0.elapsed_time(e1) / len(sel)
+    forest.close()
+    mc_ms = ev[2].elapsed_time(ev[3])
+    mc_bytes = side ** 3 * 4 + nmc * rec.itemsize
+    log(f"[bench] iso: volume {ev[0].elapsed_time(ev[1]):.1f} ms, metacells {mc_ms:.2f} ms "
+        f"({mc_bytes / (mc_ms * 1e-3) / 1e9:.0f} GB/s), forest {forest_ms:.1f} ms, "
+        f"{nq} isovalues batched {batch_ms:.2f} ms ({total} IDs), plain {plain_qps:.0f} q/s, "
+        f"IPS {1e3 / ips_ms:.0f} q/s")
+    return {
+        "workload": f"{side}^3 f32 volume (64 Gaussians + N(0, 0.01) noise, seed 1), 4^3-voxel metacells "
+                    f"({nmc} intervals), StaticPSTGPUArr with 64 threads per block, {nq} isovalues over "
+                    "[vmin, vmax] as twoSidedLeftSearch(q, q)",
+        "volume_synthesis_ms": round(ev[0].elapsed_time(ev[1]), 3),
+        "metacell_ms": round(mc_ms, 3),
+        "metacell_GBps": round(mc_bytes / (mc_ms * 1e-3) / 1e9, 1),
+        "forest_build_ms": round(forest_ms, 2),
+        "batch_ms": round(batch_ms, 3),
+        "batch_qps": round(nq / (batch_ms * 1e-3), 1),
+        "reported_ids": total,
+        "plain_qps": round(plain_qps, 1),
+        "ips_qps": round(1e3 / ips_ms, 1),
+        "results_checked": bool(ok),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -47,6 +145,9 @@ def main():
     ap.add_argument("--ips", type=int, default=64, help="queries timed for interval parallel search")
     ap.add_argument("--cpu-sample", type=int, default=512, help="queries in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--iso-side", type=int, default=1024, help="isosurface volume side (0: skip)")
+    ap.add_argument("--iso-queries", type=int, default=200)
+    ap.add_argument("--iso-plain", type=int, default=50, help="isovalues timed on the plain path and IPS")
     ap.add_argument("--sort-queries", type=int, default=0,
                     help="hand the batch to the library pre-ordered by min_dim1/max_dim1 "
                          "(the library groups queries itself)")
@@ -260,6 +361,9 @@ def main():
         del tr
 
     tree.close()
+    del d_pts, out
+    torch.cuda.empty_cache()
+    iso = iso_bench(args, P, D, torch, dev, stream) if args.iso_side > 0 else None
     res = {
         "metric": "queries/s + reported IDs/s (GB/s vs HBM peak), 2^26-pt PST; p50/p99 latency",
         "value": round(qps, 2),
@@ -296,7 +400,8 @@ def main():
         "ips_GBps": round(float(ips_gbps), 1),
         "build_ms": round(build_ms, 2),
         "build_GBps_io_floor": round(build_bytes / (build_ms * 1e-3) / 1e9, 1),
-        "results_checked": bool(ok),
+        "results_checked": bool(ok and (iso is None or iso["results_checked"])),
+        "isosurface": iso,
     }
     print(json.dumps(res), flush=True)
 

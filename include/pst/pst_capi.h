@@ -176,6 +176,40 @@ size_t pst_gpu_arr_calc_global_mem_needed(size_t num_elems, unsigned threads_per
  * at position c*St of every array, the last tree with its own S. */
 size_t pst_gpu_slots_per_tree(const pst_tree* tree);
 
+/* ------------------------------------------------------------------ */
+/* Metacell formation (helpers/isosurface-data-processing.h:21-137)    */
+/* ------------------------------------------------------------------ */
+
+/* Replaces calcNumMetacells(pt_grid_dims, metacell_dims,
+ * metacell_grid_dims, num_metacells) (isosurface-data-processing.h:21-36):
+ * ceil((pt_grid_dims[d] - 1) / metacell_dims[d]) metacells along axis d;
+ * returns their product (0 on invalid dimensions). */
+size_t pst_calc_num_metacells(const size_t pt_grid_dims[3], const size_t metacell_dims[3],
+                              size_t metacell_grid_dims[3]);
+/* Replaces formMetacellTags<PointStruct, T, GridDimType>(vertex_arr_d,
+ * pt_grid_dims, metacell_dims, metacell_grid_dims, num_metacells, dev_ind,
+ * num_devs, warp_size) (isosurface-data-processing.h:66-137,
+ * isosurface-data-processing.tu:7-127): vertex_arr_d is a T grid, x fastest;
+ * metacell (a, b, c) gets (min, max) over the vertices
+ * [a*mx, min((a+1)*mx, px-1)] x ... and ID a + (b + c*gy)*gx, stored at
+ * index ID of a newly allocated PointStruct array (free with pst_free).
+ * Blocking. For NaN-free volumes. */
+int pst_form_metacell_tags(const void* vertex_arr_d, int dtype, int id_dtype, const size_t pt_grid_dims[3],
+                           const size_t metacell_dims[3], void** metacell_arr_d, size_t* num_metacells,
+                           int dev_ind, int num_devs);
+/* The same into a caller-owned array, asynchronous on `stream`. No
+ * counterpart in the reference. */
+int pst_form_metacells_into(const void* vertex_arr_d, int dtype, int id_dtype, const size_t pt_grid_dims[3],
+                            const size_t metacell_dims[3], void* metacell_arr_d, pst_stream_t stream);
+/* Synthetic f32 volume (no counterpart in the reference; a seeded input of
+ * the isosurface workload): value = sum_g amp_g * exp(-|p - c_g|^2 /
+ * (2 s_g^2)) + noise_std * N(0, 1) at p = (i, j, k) / (dims - 1), x fastest.
+ * gaussians_d: device floats, 8 per Gaussian: cx, cy, cz, -1/(2 s^2), amp,
+ * 0, 0, 0. The normal draw of vertex v comes from SplitMix64 over (seed, v),
+ * so the volume depends only on the inputs. At most 256 Gaussians. */
+int pst_synth_gaussian_volume(float* out_d, const size_t dims[3], const float* gaussians_d, unsigned num_gaussians,
+                              float noise_std, unsigned long long seed, pst_stream_t stream);
+
 /* ---- Batched searches (no counterpart in the reference) ----
  * queries_d: device array of pst_query_<dtype>. Count pass then write pass:
  *   pst_gpu_batch_count   counts_d[q] = number of results of query q

@@ -522,6 +522,79 @@ int pst_gpu_set_query_grouping(pst_tree* t, int queries_per_walk) {
   return PST_OK;
 }
 
+size_t pst_calc_num_metacells(const size_t pt_grid_dims[3], const size_t metacell_dims[3],
+                              size_t metacell_grid_dims[3]) {
+  if (!pt_grid_dims || !metacell_dims) return 0;
+  size_t total = 1;
+  for (int d = 0; d < 3; d++) {
+    if (pt_grid_dims[d] < 1 || metacell_dims[d] == 0) return 0;
+    const size_t g = (pt_grid_dims[d] - 1 + metacell_dims[d] - 1) / metacell_dims[d];
+    if (metacell_grid_dims) metacell_grid_dims[d] = g;
+    total *= g;
+  }
+  return total;
+}
+
+static int metacell_dims_u32(const size_t pt[3], const size_t mc[3], uint32_t p32[3], uint32_t m32[3]) {
+  for (int d = 0; d < 3; d++) {
+    if (pt[d] >= (1ull << 31) || mc[d] >= (1ull << 31))
+      return fail(PST_ERR_INVALID, "Error: grid dimensions must be below 2^31");
+    p32[d] = static_cast<uint32_t>(pt[d]);
+    m32[d] = static_cast<uint32_t>(mc[d]);
+  }
+  return PST_OK;
+}
+
+int pst_form_metacells_into(const void* vertex_arr_d, int dtype, int id_dtype, const size_t pt_grid_dims[3],
+                            const size_t metacell_dims[3], void* metacell_arr_d, pst_stream_t stream) {
+  if (!vertex_arr_d || !metacell_arr_d || !pt_grid_dims || !metacell_dims)
+    return fail(PST_ERR_INVALID, "Error: null argument");
+  if (!valid_t(dtype) || !valid_id(id_dtype)) return fail(PST_ERR_INVALID, "Error: unsupported type codes");
+  uint32_t p32[3], m32[3];
+  int rc = metacell_dims_u32(pt_grid_dims, metacell_dims, p32, m32);
+  if (rc != PST_OK) return rc;
+  Status st = pst::form_metacells(dtype, id_dtype, vertex_arr_d, p32, m32, metacell_arr_d,
+                                  reinterpret_cast<hipStream_t>(stream));
+  return st.ok() ? PST_OK : fail(st);
+}
+
+int pst_form_metacell_tags(const void* vertex_arr_d, int dtype, int id_dtype, const size_t pt_grid_dims[3],
+                           const size_t metacell_dims[3], void** metacell_arr_d, size_t* num_metacells, int dev_ind,
+                           int num_devs) {
+  if (!metacell_arr_d || !num_metacells) return fail(PST_ERR_INVALID, "Error: null argument");
+  *metacell_arr_d = nullptr;
+  const size_t nm = pst_calc_num_metacells(pt_grid_dims, metacell_dims, nullptr);
+  if (nm == 0) return fail(PST_ERR_INVALID, "Error: empty metacell grid");
+  void* out = nullptr;
+  hipError_t e = hipMalloc(&out, nm * pst::point_size(dtype, id_dtype));
+  if (e != hipSuccess)
+    return fail(PST_ERR_HIP, "Error in allocating metacell tag storage array" + dev_suffix(dev_ind, num_devs) +
+                                 hipGetErrorString(e));
+  int rc = pst_form_metacells_into(vertex_arr_d, dtype, id_dtype, pt_grid_dims, metacell_dims, out, nullptr);
+  if (rc == PST_OK && hipStreamSynchronize(nullptr) != hipSuccess)
+    rc = fail(PST_ERR_HIP, "Error in synchronising after metacell formation");
+  if (rc != PST_OK) {
+    hipFree(out);
+    return rc;
+  }
+  *metacell_arr_d = out;
+  *num_metacells = nm;
+  return PST_OK;
+}
+
+int pst_synth_gaussian_volume(float* out_d, const size_t dims[3], const float* gaussians_d, unsigned num_gaussians,
+                              float noise_std, unsigned long long seed, pst_stream_t stream) {
+  if (!out_d || !dims || (num_gaussians && !gaussians_d)) return fail(PST_ERR_INVALID, "Error: null argument");
+  uint32_t d32[3];
+  for (int d = 0; d < 3; d++) {
+    if (dims[d] == 0 || dims[d] >= (1ull << 31)) return fail(PST_ERR_INVALID, "Error: bad volume dimensions");
+    d32[d] = static_cast<uint32_t>(dims[d]);
+  }
+  Status st = pst::gauss_volume(out_d, d32, gaussians_d, num_gaussians, noise_std, seed,
+                                reinterpret_cast<hipStream_t>(stream));
+  return st.ok() ? PST_OK : fail(st);
+}
+
 int pst_interval_parallel_search(const void* pt_arr_d, size_t n, int dtype, int id_dtype, int ret, void** res,
                                  size_t* num, const void* search_val, int dev_ind, int num_devs, int warp_size,
                                  unsigned num_thread_blocks, unsigned threads_per_block) {

@@ -114,8 +114,13 @@ Status ips_search(int dtype, int id_dtype, const void* pts_d, size_t n, int ret_
                   unsigned long long* counter_d, unsigned num_blocks, hipStream_t stream);
 
 // ---- metacells (pst_metacell.hip) ----
+// One PointStruct per metacell, at index = its ID (asynchronous).
 Status form_metacells(int dtype, int id_dtype, const void* vertices_d, const uint32_t pt_dims[3],
                       const uint32_t mc_dims[3], void* out_pts_d, hipStream_t stream);
+// Sum of Gaussians plus normal noise on a dims[0] x dims[1] x dims[2] grid
+// (x fastest); params_d: per Gaussian (cx, cy, cz, -1/(2 s^2), amp, 0, 0, 0).
+Status gauss_volume(float* out_d, const uint32_t dims[3], const float* params_d, uint32_t ngauss, float noise_std,
+                    uint64_t seed, hipStream_t stream);
 
 // helpers
 Status hip_status(hipError_t e, const std::string& what);

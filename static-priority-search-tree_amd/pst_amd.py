@@ -35,6 +35,8 @@ EXPORTED = [
     "pst_gpu_export_tree", "pst_gpu_export_bytes", "pst_gpu_batch_count", "pst_gpu_batch_write",
     "pst_gpu_batch_search", "pst_gpu_set_search_tuning", "pst_gpu_set_query_grouping",
     "pst_gpu_arr_create", "pst_gpu_arr_calc_global_mem_needed", "pst_gpu_slots_per_tree",
+    "pst_calc_num_metacells", "pst_form_metacell_tags", "pst_form_metacells_into",
+    "pst_synth_gaussian_volume",
     "pst_interval_parallel_search",
     "pst_interval_parallel_search_into",
 ]
@@ -90,6 +92,12 @@ def lib():
         L.pst_gpu_arr_calc_global_mem_needed.argtypes = [sz, c.c_uint, i32, i32]
         L.pst_gpu_slots_per_tree.restype = sz
         L.pst_gpu_slots_per_tree.argtypes = [vp]
+        dims3 = c.c_size_t * 3
+        L.pst_calc_num_metacells.restype = sz
+        L.pst_calc_num_metacells.argtypes = [dims3, dims3, dims3]
+        L.pst_form_metacell_tags.argtypes = [vp, i32, i32, dims3, dims3, c.POINTER(vp), c.POINTER(sz), i32, i32]
+        L.pst_form_metacells_into.argtypes = [vp, i32, i32, dims3, dims3, vp, vp]
+        L.pst_synth_gaussian_volume.argtypes = [vp, dims3, vp, c.c_uint, c.c_float, c.c_ulonglong, vp]
         L.pst_gpu_num_elems.restype = sz
         L.pst_gpu_num_elems.argtypes = [vp]
         L.pst_gpu_three_sided_search.argtypes = [vp, i32, c.POINTER(sz), c.POINTER(vp), vp, vp, vp]
@@ -346,3 +354,40 @@ def ips_into(pt_arr_d, num_elems, search_val, out_d, capacity, counter_d, dtype=
                                                    DT[id_dtype], ret, v.ctypes.data,
                                                    out_d.data_ptr(), capacity,
                                                    counter_d.data_ptr(), _stream_ptr(stream)))
+
+
+# ---- metacells (helpers/isosurface-data-processing.h) ----
+def calcNumMetacells(pt_grid_dims, metacell_dims):
+    """(number of metacells, metacell grid dims), as calcNumMetacells."""
+    d3 = ctypes.c_size_t * 3
+    g = d3()
+    n = lib().pst_calc_num_metacells(d3(*pt_grid_dims), d3(*metacell_dims), g)
+    return n, tuple(g)
+
+
+def form_metacells_into(vertices_d, pt_grid_dims, metacell_dims, out_d, dtype="f32", id_dtype="u32", stream=None):
+    """Metacell intervals (min, max, id) into out_d (one PointStruct per metacell)."""
+    d3 = ctypes.c_size_t * 3
+    _check(lib().pst_form_metacells_into(vertices_d.data_ptr(), DT[dtype], DT[id_dtype], d3(*pt_grid_dims),
+                                         d3(*metacell_dims), out_d.data_ptr(), _stream_ptr(stream)))
+
+
+def gaussian_volume_params(n_gauss, seed):
+    """Host parameters of the synthetic isosurface field: centres U[0,1]^3,
+    sigma ~ U[0.02, 0.15], amplitude ~ U[-1, 1] (SplitMix64 streams 10-14
+    of pst_data), packed 8 floats per Gaussian as the volume kernel reads them."""
+    import pst_data as D
+    c = np.stack([D.u01_f64(seed, 10 + a, 0, n_gauss) for a in range(3)], axis=1)
+    s = 0.02 + 0.13 * D.u01_f64(seed, 13, 0, n_gauss)
+    amp = -1.0 + 2.0 * D.u01_f64(seed, 14, 0, n_gauss)
+    p = np.zeros((n_gauss, 8), np.float32)
+    p[:, 0:3] = c
+    p[:, 3] = -1.0 / (2.0 * s * s)
+    p[:, 4] = amp
+    return p
+
+
+def synth_gaussian_volume(out_d, dims, params_d, n_gauss, noise_std=0.01, seed=1, stream=None):
+    d3 = ctypes.c_size_t * 3
+    _check(lib().pst_synth_gaussian_volume(out_d.data_ptr(), d3(*dims), params_d.data_ptr(), n_gauss,
+                                           noise_std, seed, _stream_ptr(stream)))

@@ -63,3 +63,16 @@ def test_invalid_type_codes_rejected_without_device(L):
     rc = L.pst_gpu_create(None, 0, 9, 0, 1, 0, 1, None, ctypes.byref(h))
     assert rc == 1
     assert b"unsupported" in L.pst_last_error()
+
+
+def test_calc_num_metacells(L):
+    """calcNumMetacells: ceil((p - 1) / m) metacells per axis
+    (helpers/isosurface-data-processing.h:21-36)."""
+    import pst_amd as P
+    n, g = P.calcNumMetacells((1024, 1024, 1024), (4, 4, 4))
+    assert g == (256, 256, 256) and n == 256 ** 3
+    n, g = P.calcNumMetacells((10, 7, 2), (3, 3, 1))
+    assert g == (3, 2, 1) and n == 6
+    n, g = P.calcNumMetacells((5, 5, 5), (4, 4, 4))
+    assert g == (1, 1, 1) and n == 1
+    assert P.calcNumMetacells((5, 5, 5), (0, 4, 4))[0] == 0
