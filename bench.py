@@ -133,6 +133,93 @@ def iso_bench(args, P, D, torch, dev, stream):
     }
 
 
+def serve_bench(args, P, D, torch, dev):
+    """Serving form: 8 resident trees of 2^20 ... 2^27 points, alternating
+    float32 intervals and int32 geometric-width points; queries arrive as a
+    seeded Poisson stream at args.serve_qps, pick a tree with probability
+    proportional to its size, and draw a query from that tree's mix (float
+    trees: stabbing (q, q) and three-sided as in the main workload; int32
+    trees: the two-/three-sided mix of the int32 batch workload). The
+    library's server batches them on the fly and admits them by a 32 GB
+    output budget. Latency = arrival to results ready in device memory."""
+    import numpy as np
+    trees, keep, mixes, sizes = [], [], [], []
+    for k, lg in enumerate(range(20, 28)):
+        n = 1 << lg
+        if k % 2 == 0:
+            pts = D.interval_points(n, seed=100 + lg)
+            t = "f32"
+            qs = D.float_query_mix(2048, 2048, seed=200 + lg)
+            qs["kind"][:2048] = 1
+        else:
+            pts = D.int_points(n, seed=100 + lg)
+            t = "i32"
+            qs = D.int_query_mix(4096, seed=200 + lg)
+        d = P.to_device(pts, dev)
+        trees.append(P.StaticPSTGPU(d, n, t, "u32"))
+        torch.cuda.synchronize()
+        del d, pts
+        mixes.append(qs)
+        sizes.append(n)
+    torch.cuda.empty_cache()
+    srv = P.Server(trees, args.serve_budget_gb << 30, max_batch=args.serve_max_batch)
+    rng = np.random.default_rng(args.serve_seed)
+    nq = int(args.serve_qps * args.serve_seconds)
+    gaps = rng.exponential(1.0 / args.serve_qps, nq)
+    arrive = np.cumsum(gaps)
+    prob = np.array(sizes, np.float64) / sum(sizes)
+    tree_of = rng.choice(len(trees), nq, p=prob)
+    pick = rng.integers(0, 4096, nq)
+    arr_at, lat, ids = {}, [], 0
+    i = 0
+    t0 = time.perf_counter()
+    while len(lat) < nq:
+        now = time.perf_counter() - t0
+        while i < nq and arrive[i] <= now:
+            tk = srv.submit(int(tree_of[i]), mixes[tree_of[i]][pick[i]])
+            arr_at[tk] = arrive[i]
+            i += 1
+        srv.step()
+        for tk in srv.poll():
+            tnow = time.perf_counter() - t0
+            _, cnt = srv.result(tk)
+            ids += cnt
+            lat.append(tnow - arr_at.pop(tk))
+            srv.release(tk)
+        if time.perf_counter() - t0 > args.serve_seconds * 4 + 20:
+            break
+    elapsed = time.perf_counter() - t0
+    srv.close()
+    lat_ms = np.array(lat) * 1e3
+    # plain path on a sample of the same mix
+    samp = np.arange(0, nq, max(1, nq // 200))[:200]
+    cap = 1 << 27
+    pout = torch.empty(cap, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    for j in samp:
+        q = mixes[tree_of[j]][pick[j]]
+        trees[tree_of[j]].search_into(int(q["kind"]), q["min_dim1"], q["max_dim1"], q["min_dim2"], pout, cap)
+    plain_qps = len(samp) / (time.perf_counter() - tp)
+    for tr in trees:
+        tr.close()
+    done = len(lat)
+    log(f"[bench] serve: {done}/{nq} queries at {args.serve_qps} q/s offered, "
+        f"{done / elapsed:.0f} q/s served, p50 {np.percentile(lat_ms, 50):.2f} ms, "
+        f"p99 {np.percentile(lat_ms, 99):.2f} ms, plain {plain_qps:.0f} q/s")
+    return {
+        "workload": "8 resident trees 2^20..2^27 (alternating f32 intervals / int32 geometric widths), "
+                    f"Poisson arrivals at {args.serve_qps} q/s for {args.serve_seconds} s, tree chosen with "
+                    "probability proportional to size, 32 GB output budget, IDs returned",
+        "offered_qps": args.serve_qps, "completed": done, "submitted": nq,
+        "served_qps": round(done / elapsed, 1),
+        "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+        "reported_ids": int(ids),
+        "plain_qps_same_mix": round(plain_qps, 1),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +235,11 @@ def main():
     ap.add_argument("--iso-side", type=int, default=1024, help="isosurface volume side (0: skip)")
     ap.add_argument("--iso-queries", type=int, default=200)
     ap.add_argument("--iso-plain", type=int, default=50, help="isovalues timed on the plain path and IPS")
+    ap.add_argument("--serve-qps", type=float, default=20000.0, help="offered load of the serving run (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-budget-gb", type=int, default=32)
+    ap.add_argument("--serve-max-batch", type=int, default=1024)
+    ap.add_argument("--serve-seed", type=int, default=4)
     ap.add_argument("--sort-queries", type=int, default=0,
                     help="hand the batch to the library pre-ordered by min_dim1/max_dim1 "
                          "(the library groups queries itself)")
@@ -364,6 +456,8 @@ def main():
     del d_pts, out
     torch.cuda.empty_cache()
     iso = iso_bench(args, P, D, torch, dev, stream) if args.iso_side > 0 else None
+    torch.cuda.empty_cache()
+    serve = serve_bench(args, P, D, torch, dev) if args.serve_qps > 0 else None
     res = {
         "metric": "queries/s + reported IDs/s (GB/s vs HBM peak), 2^26-pt PST; p50/p99 latency",
         "value": round(qps, 2),
@@ -402,6 +496,7 @@ def main():
         "build_GBps_io_floor": round(build_bytes / (build_ms * 1e-3) / 1e9, 1),
         "results_checked": bool(ok and (iso is None or iso["results_checked"])),
         "isosurface": iso,
+        "serving": serve,
     }
     print(json.dumps(res), flush=True)
 

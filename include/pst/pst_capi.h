@@ -105,8 +105,11 @@ int pst_gpu_create(const void* pt_arr_d, size_t num_elems, int dtype, int id_dty
                    pst_tree** out);
 /* Replaces ~StaticPSTGPU (pst/static-pst-gpu.h:80-88). */
 int pst_gpu_destroy(pst_tree* tree);
-/* Getters (pst/static-pst-gpu.h:96-98) and sizes. */
+/* Getters (pst/static-pst-gpu.h:96-98) and sizes; the dtype getters have
+ * no counterpart (the reference's types are template parameters). */
 size_t pst_gpu_num_elems(const pst_tree* tree);
+int pst_gpu_dtype(const pst_tree* tree);
+int pst_gpu_id_dtype(const pst_tree* tree);
 int pst_gpu_dev_ind(const pst_tree* tree);
 int pst_gpu_num_devs(const pst_tree* tree);
 
@@ -240,6 +243,34 @@ int pst_gpu_set_search_tuning(pst_tree* tree, int rounds, int budget_small, int 
  * each query's results exactly those of its own walk) or 1 (one walk per
  * query). No counterpart in the reference. */
 int pst_gpu_set_query_grouping(pst_tree* tree, int queries_per_walk);
+
+/* ------------------------------------------------------------------ */
+/* Query server (no counterpart in the reference)                      */
+/* ------------------------------------------------------------------ */
+
+/* Continuous batching over resident trees with admission by output memory.
+ * Queries (host pst_query_<dtype> records) are submitted per tree; each
+ * pst_server_step, without blocking: retires finished write batches,
+ * collects finished count passes, admits counted queries in arrival order
+ * while their exact outputs fit one range of the out_budget_bytes pool,
+ * launches their write pass, and starts a count pass (up to max_batch
+ * queries) for newly submitted ones. Each tree works on its own stream, so
+ * batches of different trees overlap on the device. A completed ticket's
+ * results (pst_server_result: device pointer, count) stay valid until
+ * pst_server_release. Each result equals the same query searched alone.
+ * Not thread-safe: one host thread drives a server. */
+typedef struct pst_server pst_server;
+int pst_server_create(pst_tree* const* trees, int num_trees, size_t out_budget_bytes, int max_batch,
+                      pst_server** out);
+int pst_server_destroy(pst_server* server);
+int pst_server_submit(pst_server* server, int tree, const void* query, int ret, unsigned long long* ticket);
+int pst_server_step(pst_server* server);
+int pst_server_poll(pst_server* server, unsigned long long* tickets, size_t capacity, size_t* num);
+int pst_server_result(pst_server* server, unsigned long long ticket, void** res_d, size_t* num_res_elems);
+int pst_server_release(pst_server* server, unsigned long long ticket);
+/* Tickets submitted and not yet released. */
+size_t pst_server_outstanding(const pst_server* server);
+const char* pst_server_last_error(void);
 
 /* ------------------------------------------------------------------ */
 /* Interval parallel search (interval-parallel-search/                 */

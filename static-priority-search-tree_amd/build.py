@@ -20,8 +20,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpst_hip.so")
 
-SOURCES = ["pst_capi.cpp", "pst_build.hip", "pst_search.hip", "pst_sort.hip", "pst_ips.hip",
-           "pst_forest.hip", "pst_metacell.hip"]
+SOURCES = ["pst_capi.cpp", "pst_server.cpp", "pst_build.hip", "pst_search.hip", "pst_sort.hip", "pst_ips.hip",
+           "pst_metacell.hip"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 ARCH = os.environ.get("PST_OFFLOAD_ARCH", "gfx950")

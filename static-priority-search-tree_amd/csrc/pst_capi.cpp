@@ -325,6 +325,8 @@ int pst_gpu_arr_create(const void* pt_arr_d, size_t n, int dtype, int id_dtype, 
 }
 
 size_t pst_gpu_slots_per_tree(const pst_tree* t) { return t ? t->slots_per_tree : 0; }
+int pst_gpu_dtype(const pst_tree* t) { return t ? t->dtype : -2; }
+int pst_gpu_id_dtype(const pst_tree* t) { return t ? t->id_dtype : -2; }
 
 size_t pst_gpu_num_elems(const pst_tree* t) { return t ? t->n : 0; }
 int pst_gpu_dev_ind(const pst_tree* t) { return t ? t->dev_ind : 0; }

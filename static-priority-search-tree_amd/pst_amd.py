@@ -36,7 +36,9 @@ EXPORTED = [
     "pst_gpu_batch_search", "pst_gpu_set_search_tuning", "pst_gpu_set_query_grouping",
     "pst_gpu_arr_create", "pst_gpu_arr_calc_global_mem_needed", "pst_gpu_slots_per_tree",
     "pst_calc_num_metacells", "pst_form_metacell_tags", "pst_form_metacells_into",
-    "pst_synth_gaussian_volume",
+    "pst_synth_gaussian_volume", "pst_gpu_dtype", "pst_gpu_id_dtype",
+    "pst_server_create", "pst_server_destroy", "pst_server_submit", "pst_server_step", "pst_server_poll",
+    "pst_server_result", "pst_server_release", "pst_server_outstanding", "pst_server_last_error",
     "pst_interval_parallel_search",
     "pst_interval_parallel_search_into",
 ]
@@ -92,6 +94,18 @@ def lib():
         L.pst_gpu_arr_calc_global_mem_needed.argtypes = [sz, c.c_uint, i32, i32]
         L.pst_gpu_slots_per_tree.restype = sz
         L.pst_gpu_slots_per_tree.argtypes = [vp]
+        L.pst_gpu_dtype.argtypes = [vp]
+        L.pst_gpu_id_dtype.argtypes = [vp]
+        L.pst_server_create.argtypes = [c.POINTER(vp), i32, sz, i32, c.POINTER(vp)]
+        L.pst_server_destroy.argtypes = [vp]
+        L.pst_server_submit.argtypes = [vp, i32, vp, i32, c.POINTER(c.c_ulonglong)]
+        L.pst_server_step.argtypes = [vp]
+        L.pst_server_poll.argtypes = [vp, vp, sz, c.POINTER(sz)]
+        L.pst_server_result.argtypes = [vp, c.c_ulonglong, c.POINTER(vp), c.POINTER(sz)]
+        L.pst_server_release.argtypes = [vp, c.c_ulonglong]
+        L.pst_server_outstanding.restype = sz
+        L.pst_server_outstanding.argtypes = [vp]
+        L.pst_server_last_error.restype = c.c_char_p
         dims3 = c.c_size_t * 3
         L.pst_calc_num_metacells.restype = sz
         L.pst_calc_num_metacells.argtypes = [dims3, dims3, dims3]
@@ -391,3 +405,56 @@ def synth_gaussian_volume(out_d, dims, params_d, n_gauss, noise_std=0.01, seed=1
     d3 = ctypes.c_size_t * 3
     _check(lib().pst_synth_gaussian_volume(out_d.data_ptr(), d3(*dims), params_d.data_ptr(), n_gauss,
                                            noise_std, seed, _stream_ptr(stream)))
+
+
+# ---- query server (continuous batching over resident trees) ----
+class Server:
+    """Drives the library's query server: submit(tree index, query record),
+    step(), poll() -> completed tickets, result(ticket) -> device pointer
+    and count, release(ticket)."""
+
+    def __init__(self, trees, out_budget_bytes, max_batch=4096):
+        self.trees = list(trees)
+        hs = [t.h.value if isinstance(t.h, ctypes.c_void_p) else t.h for t in self.trees]
+        arr = (ctypes.c_void_p * len(hs))(*hs)
+        h = ctypes.c_void_p()
+        rc = lib().pst_server_create(arr, len(self.trees), out_budget_bytes, max_batch, ctypes.byref(h))
+        if rc != 0:
+            raise PSTError(rc, lib().pst_server_last_error().decode())
+        self.h = h
+        self._buf = (ctypes.c_ulonglong * 65536)()
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise PSTError(rc, lib().pst_server_last_error().decode())
+
+    def submit(self, tree, query, ret=RET_ID):
+        """query: one record of query_dtype(tree's dtype) (numpy void or 1-element array)."""
+        q = np.ascontiguousarray(np.asarray(query).reshape(1))
+        t = ctypes.c_ulonglong()
+        self._chk(lib().pst_server_submit(self.h, tree, q.ctypes.data, ret, ctypes.byref(t)))
+        return t.value
+
+    def step(self):
+        self._chk(lib().pst_server_step(self.h))
+
+    def poll(self):
+        n = ctypes.c_size_t()
+        self._chk(lib().pst_server_poll(self.h, self._buf, len(self._buf), ctypes.byref(n)))
+        return list(self._buf[:n.value])
+
+    def result(self, ticket):
+        p, n = ctypes.c_void_p(), ctypes.c_size_t()
+        self._chk(lib().pst_server_result(self.h, ticket, ctypes.byref(p), ctypes.byref(n)))
+        return p.value, n.value
+
+    def release(self, ticket):
+        self._chk(lib().pst_server_release(self.h, ticket))
+
+    def outstanding(self):
+        return lib().pst_server_outstanding(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pst_server_destroy(self.h)
+            self.h = None
