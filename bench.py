@@ -220,6 +220,77 @@ def serve_bench(args, P, D, torch, dev):
     }
 
 
+def int_batch_bench(args, P, D, torch, dev):
+    """Batched form on int32: a 2^27-point tree (dim1 ~ U{0..2^30-1}, dim2 =
+    dim1 + Geom(2^-12), uint32 IDs = index), 8192 mixed queries (half
+    two-sided, half three-sided, log-uniform widths), count + scan + write
+    captured as one HIP graph, launch settings from the autotuner."""
+    import numpy as np
+    n = 1 << args.int_log2n
+    pts = D.int_points(n, seed=3)
+    d = P.to_device(pts, dev)
+    t0 = time.perf_counter()
+    tree = P.StaticPSTGPU(d, n, "i32", "u32")
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    qs = D.int_query_mix(args.int_queries, seed=3)
+    nq = len(qs)
+    dq = P.to_device(qs, dev)
+    counts = torch.zeros(nq, dtype=torch.int64, device=dev)
+    offs = torch.zeros(nq + 1, dtype=torch.int64, device=dev)
+    tree.batch_count(dq, nq, counts)
+    torch.cuda.synchronize()
+    total = int(counts.sum().item())
+    out = torch.empty(total + 1, dtype=torch.int32, device=dev)
+    host_counts = counts.cpu().numpy()
+    ok = True
+    for k in (0, 1, nq // 2, nq - 1):
+        a, b, c, kind = qs["min_dim1"][k], qs["max_dim1"][k], qs["min_dim2"][k], int(qs["kind"][k])
+        m = (pts["dim2"] >= c) & ((pts["dim1"] <= b) if kind == 1 else (pts["dim1"] >= a))
+        if kind == 0:
+            m &= pts["dim1"] <= b
+        ok &= int(np.count_nonzero(m)) == int(host_counts[k])
+    del d
+    tuned = tree.autotune(dq, nq)
+    s = torch.cuda.Stream()
+    g = tree.batch_graph(dq, nq, counts, offs, out, total + 1, stream=s)
+    g.launch(s)
+    s.synchronize()
+    reps = args.int_steps
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        g.launch(s)
+    e1.record(s)
+    s.synchronize()
+    graph_ms = e0.elapsed_time(e1) / reps
+    ok &= int(offs[nq].item()) == total
+    e0.record(s)
+    for _ in range(reps):
+        tree.batch_search(dq, nq, counts, offs, out, total + 1, stream=s)
+    e1.record(s)
+    s.synchronize()
+    eager_ms = e0.elapsed_time(e1) / reps
+    g.close()
+    tree.close()
+    log(f"[bench] int32 2^{args.int_log2n}: build {build_ms:.1f} ms, {nq} queries, {total} IDs, graph "
+        f"{graph_ms:.2f} ms ({nq / (graph_ms * 1e-3):.0f} q/s), eager {eager_ms:.2f} ms, tuned {tuned}")
+    return {
+        "workload": f"2^{args.int_log2n}-point int32 StaticPSTGPU (geometric widths, p = 2^-12), {nq} queries "
+                    "(50% two-sided, 50% three-sided, log-uniform widths), count + scan + write as one HIP "
+                    "graph, autotuned launch settings, IDs returned",
+        "build_ms": round(build_ms, 2),
+        "reported_ids_per_batch": total,
+        "graph_ms_per_batch": round(graph_ms, 3),
+        "graph_qps": round(nq / (graph_ms * 1e-3), 1),
+        "eager_ms_per_batch": round(eager_ms, 3),
+        "reported_ids_per_s": round(total / (graph_ms * 1e-3), 1),
+        "alg_GBps": round(total * BYTES_PER_ID / (graph_ms * 1e-3) / 1e9, 1),
+        "autotuned": tuned,
+        "results_checked": bool(ok),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,6 +306,9 @@ def main():
     ap.add_argument("--iso-side", type=int, default=1024, help="isosurface volume side (0: skip)")
     ap.add_argument("--iso-queries", type=int, default=200)
     ap.add_argument("--iso-plain", type=int, default=50, help="isovalues timed on the plain path and IPS")
+    ap.add_argument("--int-log2n", type=int, default=27, help="int32 batch workload size (0: skip)")
+    ap.add_argument("--int-queries", type=int, default=8192)
+    ap.add_argument("--int-steps", type=int, default=5)
     ap.add_argument("--serve-qps", type=float, default=20000.0, help="offered load of the serving run (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-budget-gb", type=int, default=32)
@@ -457,6 +531,8 @@ def main():
     torch.cuda.empty_cache()
     iso = iso_bench(args, P, D, torch, dev, stream) if args.iso_side > 0 else None
     torch.cuda.empty_cache()
+    intb = int_batch_bench(args, P, D, torch, dev) if args.int_log2n > 0 else None
+    torch.cuda.empty_cache()
     serve = serve_bench(args, P, D, torch, dev) if args.serve_qps > 0 else None
     res = {
         "metric": "queries/s + reported IDs/s (GB/s vs HBM peak), 2^26-pt PST; p50/p99 latency",
@@ -494,8 +570,10 @@ def main():
         "ips_GBps": round(float(ips_gbps), 1),
         "build_ms": round(build_ms, 2),
         "build_GBps_io_floor": round(build_bytes / (build_ms * 1e-3) / 1e9, 1),
-        "results_checked": bool(ok and (iso is None or iso["results_checked"])),
+        "results_checked": bool(ok and (iso is None or iso["results_checked"])
+                                and (intb is None or intb["results_checked"])),
         "isosurface": iso,
+        "batched_int32": intb,
         "serving": serve,
     }
     print(json.dumps(res), flush=True)

@@ -238,6 +238,26 @@ int pst_gpu_batch_search(pst_tree* tree, const void* queries_d, size_t num_queri
 /* Tuning knobs of the search rounds (0 keeps the default). */
 int pst_gpu_set_search_tuning(pst_tree* tree, int rounds, int budget_small, int budget_large,
                               int waves_per_cu);
+/* Autotuner (no counterpart in the reference): times pst_gpu_batch_search
+ * on this batch over waves per CU {16, 24, 32, 40} x queries per walk
+ * {8, 1} x spill-round step budget {128, 384, 1024}, keeps the fastest
+ * setting on the tree and reports it in chosen6 (nullable): rounds,
+ * budget_small, budget_large, waves_per_cu, queries per walk, microseconds
+ * per batch. Blocking; allocates its own count/offset/output buffers. */
+int pst_gpu_autotune(pst_tree* tree, const void* queries_d, size_t num_queries, int ret, pst_stream_t stream,
+                     int* chosen6);
+/* A batched search (count, scan, write; pst_gpu_batch_search with these
+ * exact arguments) captured as a HIP graph (no counterpart in the
+ * reference). Runs the batch once eagerly (all allocation happens then),
+ * then captures it on `stream` (not the null stream). At most 8192 queries:
+ * such batches are grouped by one kernel, with no library call. Replays read
+ * the current contents of queries_d and rewrite counts_d, offsets_d, out_d. */
+typedef struct pst_graph pst_graph;
+int pst_gpu_batch_graph_create(pst_tree* tree, const void* queries_d, size_t num_queries, int ret,
+                               unsigned long long* counts_d, unsigned long long* offsets_d, void* out_d,
+                               size_t out_capacity, pst_stream_t stream, pst_graph** out);
+int pst_gpu_batch_graph_launch(pst_graph* graph, pst_stream_t stream);
+int pst_gpu_batch_graph_destroy(pst_graph* graph);
 /* Queries per tree walk in batched searches: 8 (default; a batch is sorted
  * by leading dim1 bound and each walk serves 8 neighbouring queries, with
  * each query's results exactly those of its own walk) or 1 (one walk per

@@ -516,6 +516,145 @@ int pst_gpu_set_search_tuning(pst_tree* t, int rounds, int budget_small, int bud
   return PST_OK;
 }
 
+int pst_gpu_autotune(pst_tree* t, const void* queries_d, size_t nq, int ret, pst_stream_t stream, int* chosen6) {
+  if (!t || !queries_d || nq == 0) return fail(PST_ERR_INVALID, "Error: autotuning needs a tree and queries");
+  if (ret != PST_RET_POINT && t->id_dtype == PST_NO_ID)
+    return fail(PST_ERR_INVALID, "Error: IDs requested from a tree built without IDs");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  unsigned long long *counts = nullptr, *offs = nullptr;
+  void* out = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto cleanup = [&]() {
+    if (counts) hipFree(counts);
+    if (offs) hipFree(offs);
+    if (out) hipFree(out);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+  };
+  hipError_t e = hipMalloc(&counts, nq * 8);
+  if (e == hipSuccess) e = hipMalloc(&offs, (nq + 1) * 8);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(pst::hip_status(e, "Error in allocating autotuning buffers"));
+  }
+  int rc = pst_gpu_batch_count(t, queries_d, nq, counts, stream);
+  if (rc != PST_OK) {
+    cleanup();
+    return rc;
+  }
+  Status st = pst::exclusive_scan(counts, offs, nq, s);
+  unsigned long long total = 0;
+  if (st.ok()) st = pst::hip_status(hipMemcpyAsync(&total, offs + nq, 8, hipMemcpyDeviceToHost, s), "Error in reading the batch total");
+  if (st.ok()) st = pst::hip_status(hipStreamSynchronize(s), "Error in synchronising");
+  const size_t rec = ret == PST_RET_POINT ? pst::point_size(t->dtype, t->id_dtype) : id_size(t->id_dtype);
+  if (st.ok()) st = pst::hip_status(hipMalloc(&out, std::max<size_t>(total * rec, 1)), "Error in allocating autotuning output");
+  if (!st.ok()) {
+    cleanup();
+    return fail(st);
+  }
+  // candidates: waves per CU x queries per walk x step budget of the spill rounds
+  const int wpcs[] = {16, 24, 32, 40};
+  const int groups[] = {pst::kGroup, 1};
+  const int budgets[] = {128, 384, 1024};
+  const int keep[5] = {t->rounds, t->budget_small, t->budget_large, t->waves_per_cu, t->group};
+  float best = 1e30f;
+  int best_cfg[5] = {keep[0], keep[1], keep[2], keep[3], keep[4]};
+  for (int g : groups)
+    for (int w : wpcs)
+      for (int b : budgets) {
+        t->group = g;
+        t->waves_per_cu = w;
+        t->budget_large = b;
+        float ms = 0;
+        for (int rep = 0; rep < 3 && rc == PST_OK; rep++) {
+          if (rep == 1) hipEventRecord(e0, s);
+          rc = pst_gpu_batch_search(t, queries_d, nq, ret, counts, offs, out, total, stream);
+        }
+        if (rc != PST_OK) break;
+        hipEventRecord(e1, s);
+        hipEventSynchronize(e1);
+        hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) {
+          best = ms;
+          best_cfg[0] = t->rounds;
+          best_cfg[1] = t->budget_small;
+          best_cfg[2] = b;
+          best_cfg[3] = w;
+          best_cfg[4] = g;
+        }
+      }
+  cleanup();
+  if (rc != PST_OK) {
+    t->rounds = keep[0], t->budget_small = keep[1], t->budget_large = keep[2], t->waves_per_cu = keep[3], t->group = keep[4];
+    return rc;
+  }
+  t->rounds = best_cfg[0];
+  t->budget_small = best_cfg[1];
+  t->budget_large = best_cfg[2];
+  t->waves_per_cu = best_cfg[3];
+  t->group = best_cfg[4];
+  if (chosen6) {
+    for (int k = 0; k < 5; k++) chosen6[k] = best_cfg[k];
+    chosen6[5] = static_cast<int>(best * 500.0f);  // microseconds per batch (two timed repetitions)
+  }
+  return PST_OK;
+}
+
+struct pst_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int pst_gpu_batch_graph_create(pst_tree* t, const void* queries_d, size_t nq, int ret, unsigned long long* counts_d,
+                               unsigned long long* offsets_d, void* out_d, size_t out_capacity, pst_stream_t stream,
+                               pst_graph** out) {
+  if (!out) return fail(PST_ERR_INVALID, "Error: null output handle");
+  *out = nullptr;
+  if (!stream) return fail(PST_ERR_INVALID, "Error: graph capture needs a stream other than the null stream");
+  if (nq > 8192) return fail(PST_ERR_INVALID, "Error: captured batches hold at most 8192 queries");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  // eager run first: every allocation the batch needs happens here
+  int rc = pst_gpu_batch_search(t, queries_d, nq, ret, counts_d, offsets_d, out_d, out_capacity, stream);
+  if (rc != PST_OK) return rc;
+  hipError_t e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail(pst::hip_status(e, "Error in synchronising before capture"));
+  e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
+  if (e != hipSuccess) return fail(pst::hip_status(e, "Error in beginning graph capture"));
+  rc = pst_gpu_batch_search(t, queries_d, nq, ret, counts_d, offsets_d, out_d, out_capacity, stream);
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(s, &g);
+  if (rc != PST_OK || e != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    return rc != PST_OK ? rc : fail(pst::hip_status(e, "Error in ending graph capture"));
+  }
+  pst_graph* pg = new pst_graph();
+  pg->graph = g;
+  e = hipGraphInstantiate(&pg->exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    hipGraphDestroy(g);
+    delete pg;
+    return fail(pst::hip_status(e, "Error in instantiating the batch graph"));
+  }
+  *out = pg;
+  return PST_OK;
+}
+
+int pst_gpu_batch_graph_launch(pst_graph* g, pst_stream_t stream) {
+  if (!g) return fail(PST_ERR_INVALID, "Error: null graph");
+  hipError_t e = hipGraphLaunch(g->exec, reinterpret_cast<hipStream_t>(stream));
+  return e == hipSuccess ? PST_OK : fail(pst::hip_status(e, "Error in launching the batch graph"));
+}
+
+int pst_gpu_batch_graph_destroy(pst_graph* g) {
+  if (!g) return PST_OK;
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  delete g;
+  return PST_OK;
+}
+
 int pst_gpu_set_query_grouping(pst_tree* t, int queries_per_walk) {
   if (!t) return fail(PST_ERR_INVALID, "Error: null tree");
   if (queries_per_walk != 1 && queries_per_walk != pst::kGroup)

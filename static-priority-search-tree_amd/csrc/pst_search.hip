@@ -515,6 +515,55 @@ __global__ void k_kind_keys(const Query<T>* __restrict__ q, const uint32_t* __re
   kk[i] = k > KIND_RIGHT ? KIND_RIGHT : k;
 }
 
+// Batches of up to kSmallSort queries are ordered by one workgroup with a
+// bitonic sort in LDS, on (kind, leading-bound key, index): one kernel and
+// no library calls, so the whole batched search can be captured in a HIP
+// graph. Larger batches use the two radix sorts below.
+constexpr uint32_t kSmallSort = 8192;
+constexpr int kSortBlock = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(kSortBlock) void k_group_sort_small(const Query<T>* __restrict__ q, uint32_t nq,
+                                                                 uint32_t N, uint32_t total,
+                                                                 uint32_t* __restrict__ gq) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  uint64_t* key = reinterpret_cast<uint64_t*>(lds);
+  uint32_t* kind = reinterpret_cast<uint32_t*>(key + N);
+  uint16_t* perm = reinterpret_cast<uint16_t*>(kind + N);
+  for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) {
+    perm[i] = static_cast<uint16_t>(i);
+    if (i < nq) {
+      const Query<T> r = q[i];
+      key[i] = static_cast<uint64_t>(ord_key(r.kind == KIND_LEFT ? r.max_dim1 : r.min_dim1));
+      kind[i] = r.kind > KIND_RIGHT ? KIND_RIGHT : r.kind;
+    }
+  }
+  __syncthreads();
+  auto before = [&](uint32_t a, uint32_t b) {
+    if (a >= nq) return false;
+    if (b >= nq) return true;
+    if (kind[a] != kind[b]) return kind[a] < kind[b];
+    if (key[a] != key[b]) return key[a] < key[b];
+    return a < b;
+  };
+  for (uint32_t k = 2; k <= N; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t t = threadIdx.x; t < N / 2; t += blockDim.x) {
+        const uint32_t i = (t / j) * 2 * j + (t % j);
+        const uint32_t l = i + j;
+        const uint16_t a = perm[i], b = perm[l];
+        const bool up = (i & k) == 0;
+        if (up ? before(b, a) : before(a, b)) {
+          perm[i] = b;
+          perm[l] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) gq[i] = i < nq ? perm[i] : ~0u;
+}
+
 __global__ void k_group_pad(uint32_t* __restrict__ gq, uint32_t nq, uint32_t total) {
   const uint32_t i = nq + blockIdx.x * blockDim.x + threadIdx.x;
   if (i < total) gq[i] = ~0u;
@@ -685,6 +734,18 @@ template <typename T>
 Status make_groups(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
   const uint32_t nq = static_cast<uint32_t>(a.nq);
   const uint32_t total = ((nq + kGroup - 1) / kGroup) * kGroup;
+  if (nq <= kSmallSort) {
+    uint32_t N = 2;
+    while (N < nq) N <<= 1;
+    const size_t lds = static_cast<size_t>(N) * (8 + 4 + 2);
+    auto kern = k_group_sort_small<T>;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return hip_status(e, "Error in sizing the query sort");
+    hipLaunchKernelGGL(kern, dim3(1), dim3(kSortBlock), lds, s, static_cast<const Query<T>*>(a.queries), nq, N,
+                       total, sc.groups);
+    return hip_status(hipGetLastError(), "Error in launching query grouping");
+  }
   const unsigned blocks = (nq + 255) / 256;
   auto* keys = static_cast<uint64_t*>(sc.gkeys);
   auto* idx = reinterpret_cast<uint32_t*>(keys + 2 * sc.gcap);

@@ -39,6 +39,7 @@ EXPORTED = [
     "pst_synth_gaussian_volume", "pst_gpu_dtype", "pst_gpu_id_dtype",
     "pst_server_create", "pst_server_destroy", "pst_server_submit", "pst_server_step", "pst_server_poll",
     "pst_server_result", "pst_server_release", "pst_server_outstanding", "pst_server_last_error",
+    "pst_gpu_autotune", "pst_gpu_batch_graph_create", "pst_gpu_batch_graph_launch", "pst_gpu_batch_graph_destroy",
     "pst_interval_parallel_search",
     "pst_interval_parallel_search_into",
 ]
@@ -94,6 +95,10 @@ def lib():
         L.pst_gpu_arr_calc_global_mem_needed.argtypes = [sz, c.c_uint, i32, i32]
         L.pst_gpu_slots_per_tree.restype = sz
         L.pst_gpu_slots_per_tree.argtypes = [vp]
+        L.pst_gpu_autotune.argtypes = [vp, vp, sz, i32, vp, c.POINTER(c.c_int)]
+        L.pst_gpu_batch_graph_create.argtypes = [vp, vp, sz, i32, vp, vp, vp, sz, vp, c.POINTER(vp)]
+        L.pst_gpu_batch_graph_launch.argtypes = [vp, vp]
+        L.pst_gpu_batch_graph_destroy.argtypes = [vp]
         L.pst_gpu_dtype.argtypes = [vp]
         L.pst_gpu_id_dtype.argtypes = [vp]
         L.pst_server_create.argtypes = [c.POINTER(vp), i32, sz, i32, c.POINTER(vp)]
@@ -284,6 +289,24 @@ class StaticPSTGPU:
     def set_tuning(self, rounds=0, budget_small=0, budget_large=0, waves_per_cu=-1):
         _check(lib().pst_gpu_set_search_tuning(self.h, rounds, budget_small, budget_large, waves_per_cu))
 
+    def autotune(self, queries_d, nq, ret=RET_ID, stream=None):
+        """Pick rounds/budgets/waves/grouping for this batch; returns the choice."""
+        _, rp = self._rec(ret)
+        out = (ctypes.c_int * 6)()
+        _check(lib().pst_gpu_autotune(self.h, queries_d.data_ptr(), nq, rp, _stream_ptr(stream), out))
+        return dict(zip(("rounds", "budget_small", "budget_large", "waves_per_cu", "queries_per_walk",
+                         "us_per_batch"), list(out)))
+
+    def batch_graph(self, queries_d, nq, counts_d, offsets_d, out_d, out_capacity, stream, ret=RET_ID):
+        """Capture count + scan + write as a HIP graph on `stream` (not the
+        null stream); returns a BatchGraph whose launch(stream) replays it."""
+        _, rp = self._rec(ret)
+        g = ctypes.c_void_p()
+        _check(lib().pst_gpu_batch_graph_create(self.h, queries_d.data_ptr(), nq, rp, counts_d.data_ptr(),
+                                                offsets_d.data_ptr(), out_d.data_ptr(), out_capacity,
+                                                _stream_ptr(stream), ctypes.byref(g)))
+        return BatchGraph(g)
+
     def set_grouping(self, queries_per_walk: int):
         """Queries per tree walk in batched searches: 8 (default) or 1."""
         _check(lib().pst_gpu_set_query_grouping(self.h, queries_per_walk))
@@ -309,6 +332,19 @@ class StaticPSTGPU:
     @staticmethod
     def calcGlobalMemNeeded(num_elems, dtype="f32", id_dtype="u32"):
         return lib().pst_gpu_calc_global_mem_needed(num_elems, DT[dtype], DT[id_dtype])
+
+
+class BatchGraph:
+    def __init__(self, g):
+        self.g = g
+
+    def launch(self, stream):
+        _check(lib().pst_gpu_batch_graph_launch(self.g, _stream_ptr(stream)))
+
+    def close(self):
+        if getattr(self, "g", None):
+            lib().pst_gpu_batch_graph_destroy(self.g)
+            self.g = None
 
 
 class StaticPSTGPUArr(StaticPSTGPU):
