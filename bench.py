@@ -427,6 +427,14 @@ def main():
     t_write /= reps
     alg_bytes = total * BYTES_PER_ID
     achieved = alg_bytes / (t_write * 1e-3) / 1e9
+    # HBM bytes per launch of the same kernel on this workload, from the
+    # committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated)
+    traffic, traffic_src = None, None
+    tpath = os.path.join(ROOT, "profiles", "k_search_round_write_f32u32_2e26_batch2048.traffic.json")
+    if args.log2n == 26 and args.nq2 == 1024 and args.nq3 == 1024 and os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        traffic, traffic_src = round(tj["traffic_bytes_per_launch"]), tj["source"]
     log(f"[bench] count pass {t_count:.2f} ms, write pass {t_write:.2f} ms, "
         f"write-pass alg {achieved:.0f} GB/s")
 
@@ -558,8 +566,11 @@ def main():
         "count_pass_ms": round(t_count, 3),
         "write_pass_ms": round(t_write, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                     "kernel": "k_search_round (write pass: all rounds of one pass)"},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                     "kernel": "k_search_round<float, unsigned int, false, true, 8> (write pass: 4 launches)",
+                     "launches_per_pass": 4, "avg_launch_ms": round(t_write / 4, 3),
+                     "alg_bytes_per_launch": round(alg_bytes / 4),
+                     "traffic_source": traffic_src},
         "cpu_baseline": cpu,
         "plain_per_query_qps": round(plain_qps, 2),
         "batched_over_plain": round(qps / plain_qps, 2),
