@@ -329,6 +329,49 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish(
 }
 
 // ---------------------------------------------------------------------------
+// Subtree minimum of dim2, one level at a time from the bottom: submin(s) =
+// min(dim2(s), submin(2s+1), submin(2s+2)) over the children that exist.
+// A walk that meets a REPORT_ABOVE subtree with min_dim2 <= submin knows
+// that every node in it is reported. Level t of every tree of a forest is
+// done by one launch (ntrees x 2^t positions).
+// ---------------------------------------------------------------------------
+template <typename T, typename IS>
+__global__ void k_submin_level(const Node<T, IS>* __restrict__ nodes, T* __restrict__ submin, uint32_t t,
+                               uint32_t shift, uint32_t ntrees, uint32_t full_M, uint32_t last_M) {
+  const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t per_tree = 1ull << t;
+  if (k >= per_tree * ntrees) return;
+  const uint32_t tree = static_cast<uint32_t>(k >> t);
+  const uint32_t local = static_cast<uint32_t>((1ull << t) - 1 + (k & (per_tree - 1)));
+  const uint32_t M = tree + 1 < ntrees ? full_M : last_M;
+  const int32_t m = subtree_size32(M, local);
+  if (m < 1) return;
+  const uint64_t base = static_cast<uint64_t>(tree) << shift;
+  T v = nodes[base + local].dim2;
+  if (m >= 2) {
+    const T c = submin[base + 2 * local + 1];
+    v = c < v ? c : v;
+  }
+  if (m >= 3) {
+    const T c = submin[base + 2 * local + 2];
+    v = c < v ? c : v;
+  }
+  submin[base + local] = v;
+}
+
+template <typename T, typename IS>
+Status submin_typed(const void* nodes, void* submin, uint32_t levels, uint32_t shift, uint32_t ntrees, uint32_t full_M,
+                    uint32_t last_M, hipStream_t s) {
+  for (int t = static_cast<int>(levels) - 1; t >= 0; t--) {
+    const uint64_t work = (1ull << t) * ntrees;
+    k_submin_level<T, IS><<<static_cast<unsigned>((work + 255) / 256), 256, 0, s>>>(
+        static_cast<const Node<T, IS>*>(nodes), static_cast<T*>(submin), static_cast<uint32_t>(t), shift, ntrees,
+        full_M, last_M);
+  }
+  return hip_status(hipGetLastError(), "Error in launching subtree minima");
+}
+
+// ---------------------------------------------------------------------------
 // Forest of shallow trees (StaticPSTGPUArr). Tree c holds the input points
 // [c*St, min(n, (c+1)*St)), the reference's consecutive chunks
 // (pst/static-pst-gpu-arr-populate-trees.tu:25-40, pst/static-pst-gpu-arr.tu:
@@ -788,6 +831,35 @@ Status export_forest(int dtype, int id_dtype, size_t n, size_t slots_per_tree, c
       using IS = typename IdStore<decltype(i)>::type;
       return forest_export_typed<decltype(t), IS>(n, slots_per_tree, nodes_d, blob_d, total_slots, o1, o2, om, oi,
                                                   ob, stream);
+    });
+  });
+}
+
+}  // namespace pst
+
+namespace pst {
+
+Status build_submin(int dtype, int id_dtype, size_t n, size_t slots_per_tree, const void* nodes_d, void* submin_d,
+                    hipStream_t stream) {
+  if (n == 0) return Status{};
+  uint32_t shift, ntrees, full_M, last_M, levels = 0;
+  if (slots_per_tree == 0) {
+    shift = 31;
+    ntrees = 1;
+    full_M = last_M = static_cast<uint32_t>(n + 1);
+    while ((1ull << levels) <= n) levels++;
+  } else {
+    shift = 0;
+    while ((1ull << shift) < slots_per_tree + 1) shift++;
+    ntrees = static_cast<uint32_t>((n + slots_per_tree - 1) / slots_per_tree);
+    full_M = static_cast<uint32_t>(slots_per_tree + 1);
+    last_M = static_cast<uint32_t>(n - (ntrees - 1ull) * slots_per_tree + 1);
+    levels = shift;
+  }
+  return with_t(dtype, [&](auto t) {
+    return with_id(id_dtype, [&](auto i) {
+      using IS = typename IdStore<decltype(i)>::type;
+      return submin_typed<decltype(t), IS>(nodes_d, submin_d, levels, shift, ntrees, full_M, last_M, stream);
     });
   });
 }

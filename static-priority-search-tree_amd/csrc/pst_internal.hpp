@@ -56,6 +56,11 @@ Status export_forest(int dtype, int id_dtype, size_t n, size_t slots_per_tree, c
                      size_t total_slots, size_t off_dim1, size_t off_dim2, size_t off_med, size_t off_id,
                      size_t off_bits, hipStream_t stream);
 
+// Subtree minimum of dim2 per slot (T per record slot), for a tree or a
+// forest (slots_per_tree > 0), after construction.
+Status build_submin(int dtype, int id_dtype, size_t n, size_t slots_per_tree, const void* nodes_d, void* submin_d,
+                    hipStream_t stream);
+
 // ---- searches on one deep tree (pst_search.hip) ----
 constexpr int kGroup = 8;  // queries per group walk in batched searches
 struct SearchScratch {
@@ -83,6 +88,7 @@ struct SearchArgs {
   int dtype, id_dtype;
   size_t n;
   const void* nodes;
+  const void* submin;     // subtree minimum of dim2 per slot
   size_t slots_per_tree;  // 0: one deep tree; else a forest of trees of this many slots
   const void* queries;  // device Query<T>[nq]
   size_t nq;
@@ -97,6 +103,7 @@ struct SearchArgs {
   int budget_small, budget_large;
   int waves_per_cu;
   int group;            // 1: one query per walk; otherwise kGroup
+  int copy_min, copy_max;  // write pass: full subtrees of this many points are copied (copy_max 0: never)
   int groups_ready;     // the group table in the scratch already holds this batch
   const void* single_query;  // host Query<T> for one query (queries ignored)
   unsigned long long* debug;  // optional per-round counters (8 per round)

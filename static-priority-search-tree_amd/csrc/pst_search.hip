@@ -37,17 +37,27 @@ namespace {
 
 constexpr int kMaxRounds = 8;
 constexpr uint64_t kEmpty = ~0ull;
-// Per-wave LDS stack; deeper stacks move their oldest kMoveBlock items at a
-// time to a per-wave overflow stack in device memory (the walk pops the
-// newest 64 items per step, so the stack can grow by 64 per tree level).
-// One-query walks (32-bit items) get room for 30 levels in LDS: they are
-// latency bound and a refill from device memory stalls the only wave on a
-// subtree. Group walks (64-bit items) keep a small stack for occupancy.
+// Per-wave LDS stack (64-bit items); deeper stacks move their oldest
+// kMoveBlock items at a time to a per-wave overflow stack in device memory
+// (the walk pops the newest 64 items per step, so the stack can grow by 64
+// per tree level). One-query walks are latency bound (a refill from device
+// memory stalls the only wave on a subtree) and get a deeper LDS stack;
+// group walks keep a small one for occupancy.
 template <int G>
 constexpr int lds_stack() {
-  return G == 1 ? 2048 : 512;
+  return G == 1 ? 1024 : 512;
 }
 constexpr int kMoveBlock = 256;
+
+// Full-subtree shortcut: a REPORT_ABOVE item of at least check_min points
+// loads its subtree's minimum dim2; queries with min_dim2 <= it report the
+// whole subtree. The count pass (check_min = kCheckMinCount) adds the
+// subtree's size; the write pass checks subtrees of at least copy_min
+// points, copies those of at most copy_max points in breadth-first order,
+// and walks larger ones on with the query flagged in the children, so that
+// the check is not repeated (copy_max = 0: no copies). A copy costs a
+// reservation round trip, so small subtrees are cheaper to walk.
+constexpr int32_t kCheckMinCount = 16;
 
 // Spilled work item: the group it belongs to (kEmpty: no item) and the item.
 struct Entry {
@@ -55,34 +65,25 @@ struct Entry {
   uint64_t group;
 };
 
-// Work items. G == 1: slot << 2 | l | h << 1 (a query is active wherever an
-// item exists). G == 8: slot | a << 32 | l << 40 | h << 48.
-template <int G>
-struct Items;
-template <>
-struct Items<1> {
-  using type = uint32_t;
-  __device__ static type make(uint32_t slot, uint32_t, uint32_t l, uint32_t h) { return (slot << 2) | l | (h << 1); }
-  __device__ static uint32_t slot(type it) { return it >> 2; }
-  __device__ static uint32_t a(type) { return 1u; }
-  __device__ static uint32_t l(type it) { return it & 1u; }
-  __device__ static uint32_t h(type it) { return (it >> 1) & 1u; }
-};
-template <>
-struct Items<8> {
+// Work items: slot | a << 32 | l << 40 | h << 48 | f << 56. Bit j of a, l
+// and h is query j's search state at the slot (node_step); bit j of f (a
+// subset of a) says query j reports every node of this subtree.
+struct Items {
   using type = uint64_t;
-  __device__ static type make(uint32_t slot, uint32_t a, uint32_t l, uint32_t h) {
-    return static_cast<uint64_t>(slot) | (static_cast<uint64_t>(a | (l << 8) | (h << 16)) << 32);
+  __device__ static type make(uint32_t slot, uint32_t a, uint32_t l, uint32_t h, uint32_t f) {
+    return static_cast<uint64_t>(slot) | (static_cast<uint64_t>(a | (l << 8) | (h << 16) | (f << 24)) << 32);
   }
   __device__ static uint32_t slot(type it) { return static_cast<uint32_t>(it); }
   __device__ static uint32_t a(type it) { return static_cast<uint32_t>(it >> 32) & 0xffu; }
   __device__ static uint32_t l(type it) { return static_cast<uint32_t>(it >> 40) & 0xffu; }
   __device__ static uint32_t h(type it) { return static_cast<uint32_t>(it >> 48) & 0xffu; }
+  __device__ static uint32_t f(type it) { return static_cast<uint32_t>(it >> 56) & 0xffu; }
 };
 
 template <typename T, typename IS>
 struct Params {
   const Node<T, IS>* nodes;
+  const T* submin;  // subtree minimum of dim2 per slot
   const Query<T>* queries;
   const uint32_t* gq;  // G query indices per group (~0u: none); null when G == 1
   Entry* list[2];
@@ -103,6 +104,7 @@ struct Params {
   // tree_shift 31, ntrees 1, last_M = n + 1.
   uint32_t tree_shift;
   uint32_t ntrees, full_M, last_M;
+  int32_t check_min, copy_max;  // full-subtree shortcut thresholds of this pass
   uint32_t nq;              // query count
   uint32_t ngroups;         // groups: items of other groups are rejected
   void* gstack;              // per-wave overflow stacks, gcap items each
@@ -145,7 +147,7 @@ __global__ void k_search_init(const Query<T>* __restrict__ queries, Query<T> sin
     uint64_t item;
     if constexpr (G == 1) {
       const uint32_t kind = queries ? queries[i].kind : single.kind;
-      item = lo_open(kind) | (hi_open(kind) << 1);
+      item = (1ull << 32) | (static_cast<uint64_t>(lo_open(kind)) << 40) | (static_cast<uint64_t>(hi_open(kind)) << 48);
     } else {
       uint32_t a = 0, l = 0, h = 0;
       for (int j = 0; j < G; j++) {
@@ -159,8 +161,7 @@ __global__ void k_search_init(const Query<T>* __restrict__ queries, Query<T> sin
       }
       item = (static_cast<uint64_t>(a) << 32) | (static_cast<uint64_t>(l) << 40) | (static_cast<uint64_t>(h) << 48);
     }
-    const uint64_t root = tree << tree_shift;
-    item |= (G == 1) ? (root << 2) : root;
+    item |= tree << tree_shift;
     roots[k] = Entry{item, i};
   }
   if (k < nq) counts[k] = 0;
@@ -182,7 +183,7 @@ constexpr int stage_per_query() {
 template <typename T, typename IS, bool RET_POINT, bool WRITE, int G>
 __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_t round, uint32_t last) {
   using Rec = typename OutRec<T, IS, RET_POINT>::type;
-  using It = Items<G>;
+  using It = Items;
   using Item = typename It::type;
   constexpr int kStage = stage_per_query<G, sizeof(Rec)>();
   constexpr int kLdsStack = lds_stack<G>();
@@ -328,6 +329,70 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
         __builtin_amdgcn_wave_barrier();
       }
     };
+    // Copies the msz records of the subtree rooted at slot `root` (tree at
+    // slot tb, shape parameter Mt) to the outputs of the queries in qmask:
+    // one reservation per query (lanes j together), then the subtree's
+    // positions in breadth-first order as one flat range (position x is
+    // level k = floor(log2(x + 1)), local slot (r + 1) 2^k - 1 + x + 1 - 2^k),
+    // kCopyUnroll x 64 positions per pass with all loads issued before the
+    // stores; positions past the tree's last points are skipped.
+    auto copy_subtree = [&](uint32_t root, uint32_t tb, uint32_t Mt, uint32_t msz, uint32_t qmask) {
+      if constexpr (WRITE) {
+        constexpr int U = sizeof(Rec) <= 8 ? 8 : 4;
+        unsigned long long b = 0;
+        uint32_t fit = 0;
+        if (lane < G && ((qmask >> lane) & 1u)) {
+          b = atomicAdd(&P.counts[qid], static_cast<unsigned long long>(msz));
+          const unsigned long long room_q = b < qcap ? qcap - b : 0;
+          const unsigned long long at = qoff + b;
+          const unsigned long long room_o = at < P.out_total ? P.out_total - at : 0;
+          const unsigned long long r = room_q < room_o ? room_q : room_o;
+          fit = static_cast<uint32_t>(r < msz ? r : msz);
+          b = at;
+        }
+        const uint32_t rl = root - tb;
+        const uint32_t levels = 32u - static_cast<uint32_t>(__clz(static_cast<int>(msz)));
+        const uint32_t npos = (1u << levels) - 1u;
+        uint32_t written = 0;
+        for (uint32_t x0 = 0; x0 < npos; x0 += kWave * U) {
+          Rec rec[U];
+          bool ex[U];
+#pragma unroll
+          for (int u = 0; u < U; u++) {
+            const uint32_t x = x0 + u * kWave + lane;
+            ex[u] = false;
+            if (x < npos) {
+              const uint32_t k = 31u - static_cast<uint32_t>(__clz(static_cast<int>(x + 1)));
+              const uint32_t ls = ((rl + 1) << k) + x - (1u << k);
+              ex[u] = subtree_size32(Mt, ls) >= 1;
+              if (ex[u]) {
+                if constexpr (RET_POINT) {
+                  const Node<T, IS> nd2 = P.nodes[tb + ls];
+                  rec[u].dim1_val = nd2.dim1;
+                  rec[u].dim2_val = nd2.dim2;
+                  if constexpr (has_id_v<IS>) rec[u].id = nd2.id;
+                } else {
+                  rec[u] = P.nodes[tb + ls].id;
+                }
+              }
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < U; u++) {
+            const uint64_t em = __ballot(ex[u]);
+            const uint32_t idx = written + lanes_below(em);
+#pragma unroll
+            for (int j = 0; j < G; j++) {
+              if (!((qmask >> j) & 1u)) continue;
+              Rec* out = static_cast<Rec*>(P.out) + read_lane(b, j);
+              if (ex[u] && idx < read_lane(fit, j)) out[idx] = rec[u];
+            }
+            written += __popcll(em);
+          }
+        }
+      }
+    };
+
     // a step adds at most 64 results per query; a full stage triggers a
     // flush of every stage at least this full
     constexpr uint32_t kFlushAll = kStage > kWave ? (kStage - kWave) / 2 + 1 : 1;
@@ -396,16 +461,53 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
           ok = false;
         }
       }
+      uint32_t ia = It::a(item), il = It::l(item), ih = It::h(item), ifull = It::f(item);
+      const uint32_t tM = (slot >> P.tree_shift) + 1 < P.ntrees ? P.full_M : P.last_M;
       Node<T, IS> nd{};
       StepOut so{};
+      // REPORT_ABOVE queries not yet known to report the whole subtree
+      const uint32_t cand = (ok && m >= P.check_min) ? (ia & ~il & ~ih & ~ifull) : 0u;
+      T sm{};
+      if (cand) sm = P.submin[slot];
       if (ok) nd = P.nodes[slot];
+      if (cand) {
+#pragma unroll
+        for (int j = 0; j < G; j++)
+          if (((cand >> j) & 1u) && qb[j].min2 <= sm) ifull |= 1u << j;
+      }
+      // whole subtrees: counted at once, or copied when small enough
+      const uint32_t bulk = (!WRITE || m <= P.copy_max) ? ifull : 0u;
+      if constexpr (!WRITE) {
+        if (__ballot(bulk != 0)) {
+#pragma unroll
+          for (int j = 0; j < G; j++) {
+            uint32_t v = ((bulk >> j) & 1u) ? static_cast<uint32_t>(m) : 0u;
+            for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+            found[j] += uniform(v);
+          }
+        }
+      } else {
+        uint64_t bm = __ballot(bulk != 0);
+        while (bm) {
+          const int src = __builtin_ctzll(bm);
+          bm &= bm - 1;
+          copy_subtree(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(slot), src)),
+                       static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(tbase), src)),
+                       static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(tM), src)),
+                       static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m), src)),
+                       static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(bulk), src)));
+        }
+      }
+      ia &= ~bulk;
+      ifull &= ~bulk;
+      if (ia == 0) ok = false;
       // deep in reported subtrees every item is REPORT_ABOVE for all its
       // queries: then only the dim2 prune is evaluated
-      const bool open = ok && (It::l(item) | It::h(item)) != 0;
+      const bool open = ok && (il | ih) != 0;
       if (__ballot(open) == 0) {
-        if (ok) so = node_step_report_above<T, G>(nd.dim2, m, qb, It::a(item));
+        if (ok) so = node_step_report_above<T, G>(nd.dim2, m, qb, ia);
       } else if (ok) {
-        so = node_step<T, G>(nd.dim1, nd.dim2, nd.med, m, qb, It::a(item), It::l(item), It::h(item));
+        so = node_step<T, G>(nd.dim1, nd.dim2, nd.med, m, qb, ia, il, ih);
       }
       // ---- results, per query of the group
 #pragma unroll
@@ -444,8 +546,8 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
       const uint64_t lm = __ballot(cl), rm = __ballot(cr);
       const uint32_t pos = sp + lanes_below(lm) + lanes_below(rm);
       const uint32_t lchild = tbase + 2 * (slot - tbase) + 1;
-      if (cl) stack[pos] = It::make(lchild, so.al, so.ll, so.hl);
-      if (cr) stack[pos + (cl ? 1u : 0u)] = It::make(lchild + 1, so.ar, so.lr, so.hr);
+      if (cl) stack[pos] = It::make(lchild, so.al, so.ll, so.hl, ifull & so.al);
+      if (cr) stack[pos + (cl ? 1u : 0u)] = It::make(lchild + 1, so.ar, so.lr, so.hr, ifull & so.ar);
       sp += __popcll(lm) + __popcll(rm);
       __builtin_amdgcn_wave_barrier();
       // ---- hand remaining work to the next round when over budget
@@ -649,6 +751,17 @@ template <typename T, typename IS, bool RP, bool W, int G>
 Status launch_rounds(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
   Params<T, IS> P{};
   P.nodes = static_cast<const Node<T, IS>*>(a.nodes);
+  P.submin = static_cast<const T*>(a.submin);
+  if (!W) {
+    P.check_min = kCheckMinCount;
+    P.copy_max = 0;
+  } else if (a.copy_max > 0) {
+    P.check_min = a.copy_min > 1 ? a.copy_min : 1;
+    P.copy_max = a.copy_max;
+  } else {
+    P.check_min = 0x7fffffff;
+    P.copy_max = 0;
+  }
   P.queries = static_cast<const Query<T>*>(a.queries);
   if (a.single_query) P.single = *static_cast<const Query<T>*>(a.single_query);
   P.gq = G == 1 ? nullptr : sc.groups;
@@ -692,7 +805,7 @@ Status launch_rounds(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
   hipError_t e = hipSuccess;
   const uint32_t grid = static_cast<uint32_t>(cus * occ);
   P.total_waves = grid;
-  const size_t gbytes = static_cast<size_t>(grid) * P.gcap * sizeof(typename Items<G>::type);
+  const size_t gbytes = static_cast<size_t>(grid) * P.gcap * sizeof(Items::type);
   if (sc.gstack_bytes < gbytes) {
     if (sc.gstack) hipFree(sc.gstack);
     sc.gstack = nullptr;

@@ -39,7 +39,7 @@ EXPORTED = [
     "pst_synth_gaussian_volume", "pst_gpu_dtype", "pst_gpu_id_dtype",
     "pst_server_create", "pst_server_destroy", "pst_server_submit", "pst_server_step", "pst_server_poll",
     "pst_server_result", "pst_server_release", "pst_server_outstanding", "pst_server_last_error",
-    "pst_gpu_autotune", "pst_gpu_batch_graph_create", "pst_gpu_batch_graph_launch", "pst_gpu_batch_graph_destroy",
+    "pst_gpu_autotune", "pst_gpu_set_subtree_copy", "pst_gpu_batch_graph_create", "pst_gpu_batch_graph_launch", "pst_gpu_batch_graph_destroy",
     "pst_interval_parallel_search",
     "pst_interval_parallel_search_into",
 ]
@@ -96,6 +96,7 @@ def lib():
         L.pst_gpu_slots_per_tree.restype = sz
         L.pst_gpu_slots_per_tree.argtypes = [vp]
         L.pst_gpu_autotune.argtypes = [vp, vp, sz, i32, vp, c.POINTER(c.c_int)]
+        L.pst_gpu_set_subtree_copy.argtypes = [vp, i32, i32]
         L.pst_gpu_batch_graph_create.argtypes = [vp, vp, sz, i32, vp, vp, vp, sz, vp, c.POINTER(vp)]
         L.pst_gpu_batch_graph_launch.argtypes = [vp, vp]
         L.pst_gpu_batch_graph_destroy.argtypes = [vp]
@@ -306,6 +307,10 @@ class StaticPSTGPU:
                                                 offsets_d.data_ptr(), out_d.data_ptr(), out_capacity,
                                                 _stream_ptr(stream), ctypes.byref(g)))
         return BatchGraph(g)
+
+    def set_subtree_copy(self, copy_min: int, copy_max: int):
+        """Write pass: copy fully reported subtrees of copy_min..copy_max points (copy_max 0: off)."""
+        _check(lib().pst_gpu_set_subtree_copy(self.h, copy_min, copy_max))
 
     def set_grouping(self, queries_per_walk: int):
         """Queries per tree walk in batched searches: 8 (default) or 1."""

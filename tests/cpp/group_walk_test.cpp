@@ -73,8 +73,32 @@ std::vector<uint32_t> group_order(const std::vector<Q<T>>& qs) {
   return p;
 }
 
+// The search kernel's full-subtree shortcut (pst_search.hip): thresholds
+// restated here so the walk below takes the same decisions.
+constexpr int64_t kCheckMinCount = 16;  // count pass
+constexpr int64_t kCopyMin = 256;       // write pass defaults
+constexpr int64_t kCopyMax = 2047;
+
+// Subtree minimum of dim2 per slot, from the oracle's tree.
 template <typename T, typename I>
-std::vector<std::vector<long long>> group_walk(const Tree<T, I>& t, const std::vector<Q<T>>& qs, size_t& visits) {
+std::vector<T> subtree_min(const Tree<T, I>& t) {
+  const size_t S = t.tv.L.S;
+  std::vector<T> sm(S);
+  const uint64_t M = t.n + 1;
+  for (size_t s = S; s-- > 0;) {
+    const int64_t m = pst::subtree_size(M, s);
+    if (m < 1) continue;
+    T v = t.tv.dim2()[s];
+    if (m >= 2) v = std::min(v, sm[2 * s + 1]);
+    if (m >= 3) v = std::min(v, sm[2 * s + 2]);
+    sm[s] = v;
+  }
+  return sm;
+}
+
+template <typename T, typename I>
+std::vector<std::vector<long long>> group_walk(const Tree<T, I>& t, const std::vector<Q<T>>& qs, size_t& visits,
+                                               int shortcut = 0) {
   constexpr int G = 8;
   std::vector<std::vector<long long>> res(qs.size());
   if (t.n == 0) return res;
@@ -83,6 +107,7 @@ std::vector<std::vector<long long>> group_walk(const Tree<T, I>& t, const std::v
   const T* D1 = t.tv.dim1();
   const T* D2 = t.tv.dim2();
   const T* MD = t.tv.med();
+  const std::vector<T> sm = shortcut ? subtree_min(t) : std::vector<T>();
   for (size_t g0 = 0; g0 < order.size(); g0 += G) {
     pst::Bounds<T> qb[G] = {};
     uint32_t qid[G], a = 0, l = 0, h = 0;
@@ -98,11 +123,11 @@ std::vector<std::vector<long long>> group_walk(const Tree<T, I>& t, const std::v
     }
     struct Item {
       uint64_t slot;
-      uint32_t a, l, h;
+      uint32_t a, l, h, f;
     };
-    std::vector<Item> st{{0, a, l, h}};
+    std::vector<Item> st{{0, a, l, h, 0}};
     while (!st.empty()) {
-      const Item it = st.back();
+      Item it = st.back();
       st.pop_back();
       visits++;
       const int64_t m = pst::subtree_size(M, it.slot);
@@ -110,6 +135,35 @@ std::vector<std::vector<long long>> group_walk(const Tree<T, I>& t, const std::v
         std::printf("FAIL: walk reached an empty slot %llu\n", (unsigned long long)it.slot);
         failures++;
         continue;
+      }
+      if (shortcut) {
+        // shortcut 1: count pass (every full subtree of >= 16 points at
+        // once); 2: write pass (subtrees of >= kCopyMin points checked, at
+        // most kCopyMax copied); 3: write pass with checks from 16 points
+        const int64_t check_min = shortcut == 1 || shortcut == 3 ? kCheckMinCount : kCopyMin;
+        const uint32_t cand = m >= check_min ? (it.a & ~it.l & ~it.h & ~it.f) : 0u;
+        for (int j = 0; j < G; j++)
+          if (((cand >> j) & 1u) && qb[j].min2 <= sm[it.slot]) it.f |= 1u << j;
+        const uint32_t bulk = (shortcut == 1 || m <= kCopyMax) ? it.f : 0u;
+        if (bulk) {
+          int64_t copied = 0;
+          for (uint32_t k = 0; (1ll << k) <= m; k++)
+            for (uint64_t p = 0; p < (1ull << k); p++) {
+              const uint64_t ls = ((it.slot + 1) << k) - 1 + p;
+              if (pst::subtree_size(M, ls) < 1) continue;
+              copied++;
+              for (int j = 0; j < G; j++)
+                if ((bulk >> j) & 1u) res[qid[j]].push_back(static_cast<long long>(ls));
+            }
+          if (copied != m) {
+            std::printf("FAIL: subtree of slot %llu enumerates %lld points, size %lld\n",
+                        (unsigned long long)it.slot, (long long)copied, (long long)m);
+            failures++;
+          }
+        }
+        it.a &= ~bulk;
+        it.f &= ~bulk;
+        if (it.a == 0) continue;
       }
       const pst::StepOut o =
           pst::node_step<T, G>(D1[it.slot], D2[it.slot], MD[it.slot], m, qb, it.a, it.l, it.h);
@@ -124,8 +178,8 @@ std::vector<std::vector<long long>> group_walk(const Tree<T, I>& t, const std::v
       }
       for (int j = 0; j < G; j++)
         if ((o.rep >> j) & 1u) res[qid[j]].push_back(static_cast<long long>(it.slot));
-      if (o.al) st.push_back({2 * it.slot + 1, o.al, o.ll, o.hl});
-      if (o.ar) st.push_back({2 * it.slot + 2, o.ar, o.lr, o.hr});
+      if (o.al) st.push_back({2 * it.slot + 1, o.al, o.ll, o.hl, it.f & o.al});
+      if (o.ar) st.push_back({2 * it.slot + 2, o.ar, o.lr, o.hr, it.f & o.ar});
     }
   }
   for (auto& r : res) std::sort(r.begin(), r.end());
@@ -197,12 +251,22 @@ void run(const char* name, size_t n, int mode, uint64_t seed, size_t nq) {
   const auto t = build(pts);
   check_shape(t);
   const auto qs = make_queries<T>(rng, nq, mode);
-  size_t visits = 0;
+  size_t visits = 0, v1 = 0, v2 = 0;
   const auto got = group_walk(t, qs, visits);
+  const auto got_count = group_walk(t, qs, v1, 1);
+  const auto got_write = group_walk(t, qs, v2, 2);
+  size_t v3 = 0;
+  const auto got_write16 = group_walk(t, qs, v3, 3);
   size_t reported = 0, single_visits = 0;
   for (size_t i = 0; i < qs.size(); i++) {
     const auto want = oracle_walk(t, qs[i]);
     reported += want.size();
+    if (got_count[i] != want || got_write[i] != want || got_write16[i] != want) {
+      std::printf("FAIL: %s n=%zu mode=%d query %zu kind %u: full-subtree walk differs (%zu / %zu results, oracle %zu)\n",
+                  name, n, mode, i, qs[i].kind, got_count[i].size(), got_write[i].size(), want.size());
+      failures++;
+      return;
+    }
     if (got[i] != want) {
       std::printf("FAIL: %s n=%zu mode=%d query %zu kind %u: %zu results, oracle %zu\n", name, n, mode, i,
                   qs[i].kind, got[i].size(), want.size());
@@ -211,8 +275,8 @@ void run(const char* name, size_t n, int mode, uint64_t seed, size_t nq) {
     }
   }
   (void)single_visits;
-  std::printf("ok %s n=%zu mode=%d queries=%zu reported=%zu group-walk visits=%zu\n", name, n, mode, nq, reported,
-              visits);
+  std::printf("ok %s n=%zu mode=%d queries=%zu reported=%zu group-walk visits=%zu (full-subtree: %zu count, %zu write)\n",
+              name, n, mode, nq, reported, visits, v1, v2);
 }
 
 }  // namespace
@@ -226,6 +290,8 @@ int main() {
       run<int64_t, void>("i64/none", n, mode, 4000 + n + mode, 100);
     }
   }
+  // subtrees above the copy limit: the write pass flags and walks them on
+  run<float, uint32_t>("f32/u32", 200000, 0, 77, 60);
   if (failures) {
     std::printf("%d failures\n", failures);
     return 1;

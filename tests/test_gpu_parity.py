@@ -223,8 +223,8 @@ def test_interval_parallel_search_equals_brute_force(torch, P, t, i):
             assert got == want
 
 
-@pytest.mark.parametrize("group", [8, 1])
-def test_batch_search_interval_workload(torch, P, group):
+@pytest.mark.parametrize("group,copy", [(8, None), (1, None), (8, (0, 0)), (8, (16, 8191)), (1, (64, 65535))])
+def test_batch_search_interval_workload(torch, P, group, copy):
     """One-call count + scan + write (pst_gpu_batch_search) on a 2^20-point
     interval tree with a batch of 515 queries: stabbing queries (both
     sides), three-sided queries of log-uniform width, a duplicate, two
@@ -235,6 +235,8 @@ def test_batch_search_interval_workload(torch, P, group):
     pts = D.interval_points(n, seed=4)
     tr, keep = _tree(P, pts, "f32", "u32")
     tr.set_grouping(group)
+    if copy is not None:
+        tr.set_subtree_copy(*copy)
     qs = D.float_query_mix(128, 384, seed=5)
     # a duplicate query, and two that report every point (deep walk stacks
     # that move to the overflow stack): 515 in all
