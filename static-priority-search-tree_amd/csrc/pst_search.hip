@@ -186,6 +186,7 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
   using It = Items;
   using Item = typename It::type;
   constexpr int kStage = stage_per_query<G, sizeof(Rec)>();
+  static_assert(kStage % kWave == 0, "stages are flushed in whole wave-widths");
   constexpr int kLdsStack = lds_stack<G>();
   __shared__ Item stack[kLdsStack];
   __shared__ Entry pend[kWave];
@@ -295,7 +296,8 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
     }
     // Flushes every query whose stage holds at least `thr` results: lanes
     // 0..G-1 reserve their query's output range together (one atomic
-    // round trip for the group), then the wave copies the stages out.
+    // round trip for the group), then the wave copies the stages out. The
+    // stages are read from LDS while the reservation is in flight.
     auto flush = [&](uint32_t thr) {
       if constexpr (WRITE) {
         uint32_t fl = 0, mine = 0;
@@ -305,6 +307,12 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
           if (lane == static_cast<uint32_t>(j)) mine = nst[j];
         }
         if (fl == 0) return;
+        constexpr int kPer = kStage / kWave;
+        Rec v[G][kPer];
+#pragma unroll
+        for (int j = 0; j < G; j++)
+#pragma unroll
+          for (int k = 0; k < kPer; k++) v[j][k] = stage[j * kStage + k * kWave + lane];
         // lane j: first free index of query j's range after the reservation,
         // and how many of its staged records still fit
         unsigned long long b = 0;
@@ -323,7 +331,11 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
           if (!((fl >> j) & 1u)) continue;
           Rec* out = static_cast<Rec*>(P.out) + read_lane(b, j);
           const uint32_t nfit = read_lane(fit, j);
-          for (uint32_t x = lane; x < nfit; x += kWave) out[x] = stage[j * kStage + x];
+#pragma unroll
+          for (int k = 0; k < kPer; k++) {
+            const uint32_t x = k * kWave + lane;
+            if (x < nfit) out[x] = v[j][k];
+          }
           nst[j] = 0;
         }
         __builtin_amdgcn_wave_barrier();
@@ -334,11 +346,12 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
     // one reservation per query (lanes j together), then the subtree's
     // positions in breadth-first order as one flat range (position x is
     // level k = floor(log2(x + 1)), local slot (r + 1) 2^k - 1 + x + 1 - 2^k),
-    // kCopyUnroll x 64 positions per pass with all loads issued before the
-    // stores; positions past the tree's last points are skipped.
+    // U x 64 positions per pass with every load of the pass issued
+    // before its stores (one memory round trip per pass, 2048 positions for
+    // 4-byte records); positions past the tree's last points are skipped.
     auto copy_subtree = [&](uint32_t root, uint32_t tb, uint32_t Mt, uint32_t msz, uint32_t qmask) {
       if constexpr (WRITE) {
-        constexpr int U = sizeof(Rec) <= 8 ? 8 : 4;
+        constexpr int U = sizeof(Rec) <= 4 ? 32 : (sizeof(Rec) <= 8 ? 16 : 8);
         unsigned long long b = 0;
         uint32_t fit = 0;
         if (lane < G && ((qmask >> lane) & 1u)) {
@@ -353,39 +366,44 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
         const uint32_t rl = root - tb;
         const uint32_t levels = 32u - static_cast<uint32_t>(__clz(static_cast<int>(msz)));
         const uint32_t npos = (1u << levels) - 1u;
+        // a subtree with every position filled needs no per-position check
+        const bool complete = msz == npos;
+        auto local_slot = [&](uint32_t x) {
+          const uint32_t k = 31u - static_cast<uint32_t>(__clz(static_cast<int>(x + 1)));
+          return ((rl + 1) << k) + x - (1u << k);
+        };
+        auto exists = [&](uint32_t x) {
+          return x < npos && (complete || subtree_size32(Mt, local_slot(x)) >= 1);
+        };
         uint32_t written = 0;
         for (uint32_t x0 = 0; x0 < npos; x0 += kWave * U) {
           Rec rec[U];
-          bool ex[U];
 #pragma unroll
           for (int u = 0; u < U; u++) {
             const uint32_t x = x0 + u * kWave + lane;
-            ex[u] = false;
-            if (x < npos) {
-              const uint32_t k = 31u - static_cast<uint32_t>(__clz(static_cast<int>(x + 1)));
-              const uint32_t ls = ((rl + 1) << k) + x - (1u << k);
-              ex[u] = subtree_size32(Mt, ls) >= 1;
-              if (ex[u]) {
-                if constexpr (RET_POINT) {
-                  const Node<T, IS> nd2 = P.nodes[tb + ls];
-                  rec[u].dim1_val = nd2.dim1;
-                  rec[u].dim2_val = nd2.dim2;
-                  if constexpr (has_id_v<IS>) rec[u].id = nd2.id;
-                } else {
-                  rec[u] = P.nodes[tb + ls].id;
-                }
+            if (exists(x)) {
+              const uint32_t ls = local_slot(x);
+              if constexpr (RET_POINT) {
+                const Node<T, IS> nd2 = P.nodes[tb + ls];
+                rec[u].dim1_val = nd2.dim1;
+                rec[u].dim2_val = nd2.dim2;
+                if constexpr (has_id_v<IS>) rec[u].id = nd2.id;
+              } else {
+                rec[u] = P.nodes[tb + ls].id;
               }
             }
           }
 #pragma unroll
           for (int u = 0; u < U; u++) {
-            const uint64_t em = __ballot(ex[u]);
+            const uint32_t x = x0 + u * kWave + lane;
+            const bool ex = exists(x);
+            const uint64_t em = __ballot(ex);
             const uint32_t idx = written + lanes_below(em);
 #pragma unroll
             for (int j = 0; j < G; j++) {
               if (!((qmask >> j) & 1u)) continue;
               Rec* out = static_cast<Rec*>(P.out) + read_lane(b, j);
-              if (ex[u] && idx < read_lane(fit, j)) out[idx] = rec[u];
+              if (ex && idx < read_lane(fit, j)) out[idx] = rec[u];
             }
             written += __popcll(em);
           }
