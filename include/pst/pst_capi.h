@@ -260,9 +260,10 @@ int pst_gpu_batch_graph_launch(pst_graph* graph, pst_stream_t stream);
 int pst_gpu_batch_graph_destroy(pst_graph* graph);
 /* Full-subtree copies of the write pass (no counterpart in the reference):
  * a subtree whose every point a query reports, with copy_min..copy_max
- * points, is written as one block (one reservation, breadth-first loads)
- * instead of being walked; copy_max = 0 turns copies off. Defaults
- * 256..2047. Results are the same either way. */
+ * points, is not walked but recorded as a copy task; after the walk a copy
+ * kernel writes each task's subtree as one block (one reservation per
+ * query, breadth-first loads from a per-slot ID array). copy_max = 0 turns
+ * copies off. Defaults 256..65535. Results are the same either way. */
 int pst_gpu_set_subtree_copy(pst_tree* tree, int copy_min, int copy_max);
 /* Queries per tree walk in batched searches: 8 (default; a batch is sorted
  * by leading dim1 bound and each walk serves 8 neighbouring queries, with

@@ -333,11 +333,13 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish(
 // min(dim2(s), submin(2s+1), submin(2s+2)) over the children that exist.
 // A walk that meets a REPORT_ABOVE subtree with min_dim2 <= submin knows
 // that every node in it is reported. Level t of every tree of a forest is
-// done by one launch (ntrees x 2^t positions).
+// done by one launch (ntrees x 2^t positions). The same pass copies each
+// slot's ID into a dense ID array (null: none), which whole-subtree copies
+// read instead of the full records.
 // ---------------------------------------------------------------------------
 template <typename T, typename IS>
-__global__ void k_submin_level(const Node<T, IS>* __restrict__ nodes, T* __restrict__ submin, uint32_t t,
-                               uint32_t shift, uint32_t ntrees, uint32_t full_M, uint32_t last_M) {
+__global__ void k_submin_level(const Node<T, IS>* __restrict__ nodes, T* __restrict__ submin, IS* __restrict__ ids,
+                               uint32_t t, uint32_t shift, uint32_t ntrees, uint32_t full_M, uint32_t last_M) {
   const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const uint64_t per_tree = 1ull << t;
   if (k >= per_tree * ntrees) return;
@@ -347,7 +349,11 @@ __global__ void k_submin_level(const Node<T, IS>* __restrict__ nodes, T* __restr
   const int32_t m = subtree_size32(M, local);
   if (m < 1) return;
   const uint64_t base = static_cast<uint64_t>(tree) << shift;
-  T v = nodes[base + local].dim2;
+  const Node<T, IS> nd = nodes[base + local];
+  if constexpr (has_id_v<IS>) {
+    if (ids) ids[base + local] = nd.id;
+  }
+  T v = nd.dim2;
   if (m >= 2) {
     const T c = submin[base + 2 * local + 1];
     v = c < v ? c : v;
@@ -360,13 +366,13 @@ __global__ void k_submin_level(const Node<T, IS>* __restrict__ nodes, T* __restr
 }
 
 template <typename T, typename IS>
-Status submin_typed(const void* nodes, void* submin, uint32_t levels, uint32_t shift, uint32_t ntrees, uint32_t full_M,
-                    uint32_t last_M, hipStream_t s) {
+Status submin_typed(const void* nodes, void* submin, void* ids, uint32_t levels, uint32_t shift, uint32_t ntrees,
+                    uint32_t full_M, uint32_t last_M, hipStream_t s) {
   for (int t = static_cast<int>(levels) - 1; t >= 0; t--) {
     const uint64_t work = (1ull << t) * ntrees;
     k_submin_level<T, IS><<<static_cast<unsigned>((work + 255) / 256), 256, 0, s>>>(
-        static_cast<const Node<T, IS>*>(nodes), static_cast<T*>(submin), static_cast<uint32_t>(t), shift, ntrees,
-        full_M, last_M);
+        static_cast<const Node<T, IS>*>(nodes), static_cast<T*>(submin), static_cast<IS*>(ids),
+        static_cast<uint32_t>(t), shift, ntrees, full_M, last_M);
   }
   return hip_status(hipGetLastError(), "Error in launching subtree minima");
 }
@@ -840,7 +846,7 @@ Status export_forest(int dtype, int id_dtype, size_t n, size_t slots_per_tree, c
 namespace pst {
 
 Status build_submin(int dtype, int id_dtype, size_t n, size_t slots_per_tree, const void* nodes_d, void* submin_d,
-                    hipStream_t stream) {
+                    void* ids_d, hipStream_t stream) {
   if (n == 0) return Status{};
   uint32_t shift, ntrees, full_M, last_M, levels = 0;
   if (slots_per_tree == 0) {
@@ -859,7 +865,7 @@ Status build_submin(int dtype, int id_dtype, size_t n, size_t slots_per_tree, co
   return with_t(dtype, [&](auto t) {
     return with_id(id_dtype, [&](auto i) {
       using IS = typename IdStore<decltype(i)>::type;
-      return submin_typed<decltype(t), IS>(nodes_d, submin_d, levels, shift, ntrees, full_M, last_M, stream);
+      return submin_typed<decltype(t), IS>(nodes_d, submin_d, ids_d, levels, shift, ntrees, full_M, last_M, stream);
     });
   });
 }

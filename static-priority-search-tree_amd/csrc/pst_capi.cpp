@@ -19,11 +19,12 @@ struct pst_tree {
   int dev_ind = 0, num_devs = 1, device = 0;
   void* nodes = nullptr;
   void* submin = nullptr;  // subtree minimum of dim2 per record slot
+  void* ids = nullptr;     // ID per record slot (whole-subtree copies read these)
   size_t slots_per_tree = 0;  // forest (StaticPSTGPUArr): slots of a full tree; 0 for one deep tree
   pst::SearchScratch sc;
   int rounds = 4, budget_small = 0, budget_large = 0, waves_per_cu = 0;
   int group = pst::kGroup;  // queries per walk in batched searches
-  int copy_min = 256, copy_max = 2047;  // write pass: full subtrees copied in this size range
+  int copy_min = 256, copy_max = 65535;  // write pass: full subtrees copied in this size range
   hipStream_t stream = nullptr;
   unsigned long long* debug = nullptr;  // per-round search counters (debug builds)
 };
@@ -111,6 +112,7 @@ pst::SearchArgs base_args(pst_tree* t) {
   a.n = t->n;
   a.nodes = t->nodes;
   a.submin = t->submin;
+  a.ids = t->ids;
   a.slots_per_tree = t->slots_per_tree;
   a.rounds = t->rounds;
   a.budget_small = t->budget_small;
@@ -261,6 +263,7 @@ int pst_gpu_create(const void* pt_arr_d, size_t n, int dtype, int id_dtype, int 
   }
   hipError_t e = hipMalloc(&t->nodes, t->S * pst::node_size(dtype, id_dtype));
   if (e == hipSuccess) e = hipMalloc(&t->submin, t->S * pst::dtype_size(dtype));
+  if (e == hipSuccess && id_size(id_dtype) > 0) e = hipMalloc(&t->ids, t->S * id_size(id_dtype));
   if (e != hipSuccess) {
     std::string m = "Error in allocating priority search tree storage array" + dev_suffix(dev_ind, num_devs) +
                     hipGetErrorString(e);
@@ -268,7 +271,7 @@ int pst_gpu_create(const void* pt_arr_d, size_t n, int dtype, int id_dtype, int 
     return fail(PST_ERR_HIP, m);
   }
   Status st = pst::build_tree(dtype, id_dtype, pt_arr_d, n, t->nodes, t->stream, nullptr);
-  if (st.ok()) st = pst::build_submin(dtype, id_dtype, n, 0, t->nodes, t->submin, t->stream);
+  if (st.ok()) st = pst::build_submin(dtype, id_dtype, n, 0, t->nodes, t->submin, t->ids, t->stream);
   if (st.ok()) st = pst::hip_status(hipStreamSynchronize(t->stream), "Error in synchronising after construction");
   if (!st.ok()) {
     pst_gpu_destroy(t);
@@ -284,6 +287,10 @@ int pst_gpu_destroy(pst_tree* t) {
   if (t->nodes) e = hipFree(t->nodes);
   if (t->submin) {
     hipError_t e2 = hipFree(t->submin);
+    if (e == hipSuccess) e = e2;
+  }
+  if (t->ids) {
+    hipError_t e2 = hipFree(t->ids);
     if (e == hipSuccess) e = e2;
   }
   pst::scratch_release(t->sc);
@@ -322,6 +329,7 @@ int pst_gpu_arr_create(const void* pt_arr_d, size_t n, int dtype, int id_dtype, 
   }
   hipError_t e = hipMalloc(&t->nodes, t->S * pst::node_size(dtype, id_dtype));
   if (e == hipSuccess) e = hipMalloc(&t->submin, t->S * pst::dtype_size(dtype));
+  if (e == hipSuccess && id_size(id_dtype) > 0) e = hipMalloc(&t->ids, t->S * id_size(id_dtype));
   if (e != hipSuccess) {
     std::string m = "Error in allocating array of PSTs on device " + std::to_string(dev_ind + 1) +
                     " (1-indexed) of " + std::to_string(num_devs) + ": " + hipGetErrorString(e);
@@ -329,7 +337,8 @@ int pst_gpu_arr_create(const void* pt_arr_d, size_t n, int dtype, int id_dtype, 
     return fail(PST_ERR_HIP, m);
   }
   Status st = pst::build_forest(dtype, id_dtype, pt_arr_d, n, t->slots_per_tree, t->nodes, t->stream);
-  if (st.ok()) st = pst::build_submin(dtype, id_dtype, n, t->slots_per_tree, t->nodes, t->submin, t->stream);
+  if (st.ok())
+    st = pst::build_submin(dtype, id_dtype, n, t->slots_per_tree, t->nodes, t->submin, t->ids, t->stream);
   if (st.ok()) st = pst::hip_status(hipStreamSynchronize(t->stream), "Error in synchronising after construction");
   if (!st.ok()) {
     pst_gpu_destroy(t);
@@ -377,13 +386,14 @@ int pst_gpu_search_into(pst_tree* t, int kind, int ret, const void* min1, const 
 size_t pst_gpu_calc_global_mem_needed(size_t n, int dtype, int id_dtype) {
   if (!valid_t(dtype) || !valid_id(id_dtype)) return 0;
   const size_t S = slots_for(n);
-  const size_t tree = S * (pst::node_size(dtype, id_dtype) + pst::dtype_size(dtype));
+  const size_t tree = S * (pst::node_size(dtype, id_dtype) + pst::dtype_size(dtype) + id_size(id_dtype));
   // construction: two key arrays, two 32-bit value arrays, order and rank
   // arrays, and radix-sort storage (about one more key array)
   const size_t key = (pst::dtype_size(dtype) == 4 && id_size(id_dtype) == 0) ? 4 : 8;
   const size_t construct = n * (3 * key + 4 * 4);
-  // search: a result array of n points plus two 4M-entry work lists
-  const size_t search = n * pst::point_size(dtype, id_dtype) + (2ull << 22) * 16;
+  // search: a result array of n points, two 4M-entry work lists and a
+  // 4M-entry copy task list
+  const size_t search = n * pst::point_size(dtype, id_dtype) + (3ull << 22) * 16;
   return tree + (construct > search ? construct : search);
 }
 
@@ -391,10 +401,11 @@ size_t pst_gpu_arr_calc_global_mem_needed(size_t n, unsigned threads_per_block, 
   if (!valid_t(dtype) || !valid_id(id_dtype) || threads_per_block == 0 || threads_per_block > 1024) return 0;
   const size_t St = forest_slots_per_tree(threads_per_block);
   const size_t ntrees = n ? (n + St - 1) / St : 0;
-  const size_t tree = ntrees * (St + 1) * (pst::node_size(dtype, id_dtype) + pst::dtype_size(dtype));
-  // construction works in LDS; a search needs a result array of n points
-  // and two 4M-entry work lists
-  const size_t search = n * pst::point_size(dtype, id_dtype) + (2ull << 22) * 16;
+  const size_t tree =
+      ntrees * (St + 1) * (pst::node_size(dtype, id_dtype) + pst::dtype_size(dtype) + id_size(id_dtype));
+  // construction works in LDS; a search needs a result array of n points,
+  // two 4M-entry work lists and a 4M-entry copy task list
+  const size_t search = n * pst::point_size(dtype, id_dtype) + (3ull << 22) * 16;
   return tree + search;
 }
 

@@ -56,10 +56,11 @@ Status export_forest(int dtype, int id_dtype, size_t n, size_t slots_per_tree, c
                      size_t total_slots, size_t off_dim1, size_t off_dim2, size_t off_med, size_t off_id,
                      size_t off_bits, hipStream_t stream);
 
-// Subtree minimum of dim2 per slot (T per record slot), for a tree or a
-// forest (slots_per_tree > 0), after construction.
+// Subtree minimum of dim2 per slot (T per record slot) and, when ids_d is
+// not null, each slot's ID (stored ID type per slot), for a tree or a forest
+// (slots_per_tree > 0), after construction.
 Status build_submin(int dtype, int id_dtype, size_t n, size_t slots_per_tree, const void* nodes_d, void* submin_d,
-                    hipStream_t stream);
+                    void* ids_d, hipStream_t stream);
 
 // ---- searches on one deep tree (pst_search.hip) ----
 constexpr int kGroup = 8;  // queries per group walk in batched searches
@@ -76,6 +77,8 @@ struct SearchScratch {
   size_t gcap = 0;  // queries the grouping storage holds (a multiple of kGroup)
   void* gstack = nullptr;  // per-wave overflow stacks of the walks
   size_t gstack_bytes = 0;
+  void* tasks = nullptr;  // copy tasks of write passes (16-byte entries)
+  size_t task_cap = 0;
 };
 // Work lists for nq queries whose walks start from `roots` root items.
 Status scratch_reserve(SearchScratch& sc, size_t nq, size_t roots, hipStream_t stream);
@@ -89,6 +92,7 @@ struct SearchArgs {
   size_t n;
   const void* nodes;
   const void* submin;     // subtree minimum of dim2 per slot
+  const void* ids;        // ID per slot (null for trees without IDs)
   size_t slots_per_tree;  // 0: one deep tree; else a forest of trees of this many slots
   const void* queries;  // device Query<T>[nq]
   size_t nq;

@@ -24,7 +24,10 @@
 //   * load balance: the launch is repeated for a few rounds; in all but the
 //     last, a wave that spent its step budget on one batch pushes its
 //     remaining stack to a shared list that the next round spreads over all
-//     waves. The last round runs every stack to empty.
+//     waves. The last round runs every stack to empty;
+//   * whole subtrees that a query reports are counted at once (count pass)
+//     or recorded as copy tasks that a streaming kernel writes after the
+//     walk (write pass), so the walk stops at their roots.
 // Whether a child exists follows from n alone (pst_common.hpp), so no
 // bitcode is read.
 #include <algorithm>
@@ -53,10 +56,12 @@ constexpr int kMoveBlock = 256;
 // loads its subtree's minimum dim2; queries with min_dim2 <= it report the
 // whole subtree. The count pass (check_min = kCheckMinCount) adds the
 // subtree's size; the write pass checks subtrees of at least copy_min
-// points, copies those of at most copy_max points in breadth-first order,
-// and walks larger ones on with the query flagged in the children, so that
-// the check is not repeated (copy_max = 0: no copies). A copy costs a
-// reservation round trip, so small subtrees are cheaper to walk.
+// points, records those of at most copy_max points as copy tasks (copied in
+// breadth-first order by k_copy_tasks after the last round), and walks
+// larger ones on with the query flagged in the children, so that the check
+// is not repeated (copy_max = 0: no copies). A task costs a list
+// reservation in the walk and an output reservation in the copy, so small
+// subtrees are cheaper to walk.
 constexpr int32_t kCheckMinCount = 16;
 
 // Spilled work item: the group it belongs to (kEmpty: no item) and the item.
@@ -64,6 +69,24 @@ struct Entry {
   uint64_t item;
   uint64_t group;
 };
+
+// A whole subtree that some queries of a group report, found by the write
+// pass's walk and copied to their outputs by k_copy_tasks after the last
+// round (qmask 0: no task).
+struct CopyTask {
+  uint32_t root;   // slot of the subtree's root
+  uint32_t qmask;  // bit j: query j of the group
+  uint64_t group;
+};
+// Copy tasks one write pass can hold; subtrees found beyond it are copied
+// by the walk itself.
+constexpr size_t kCopyTaskCap = size_t{1} << 22;
+constexpr int kCopyWavesPerCu = 32;
+// Loads in flight per lane in a copy pass (4-byte records): 2048 positions
+// per memory round trip in the copy kernel; the walk's own copies (task list
+// full) use narrower passes to keep its register budget.
+constexpr int kCopyPass = 32;
+constexpr int kCopyPassWalk = 8;
 
 // Work items: slot | a << 32 | l << 40 | h << 48 | f << 56. Bit j of a, l
 // and h is query j's search state at the slot (node_step); bit j of f (a
@@ -84,6 +107,7 @@ template <typename T, typename IS>
 struct Params {
   const Node<T, IS>* nodes;
   const T* submin;  // subtree minimum of dim2 per slot
+  const IS* ids;    // ID per slot (whole-subtree copies of IDs read these)
   const Query<T>* queries;
   const uint32_t* gq;  // G query indices per group (~0u: none); null when G == 1
   Entry* list[2];
@@ -109,6 +133,9 @@ struct Params {
   uint32_t ngroups;         // groups: items of other groups are rejected
   void* gstack;              // per-wave overflow stacks, gcap items each
   uint32_t gcap;
+  CopyTask* tasks;           // copy tasks of the write pass (null: copies in the walk)
+  unsigned long long task_cap;
+  unsigned long long* task_ctr;
   uint32_t total_waves;
   uint32_t budget_small, budget_large;
 };
@@ -166,7 +193,10 @@ __global__ void k_search_init(const Query<T>* __restrict__ queries, Query<T> sin
   }
   if (k < nq) counts[k] = 0;
   if (k < nctr) ctr[k] = (k == 0) ? ngroups * ntrees : 0;
-  if (k == 0) *err = 0;
+  if (k == 0) {
+    err[0] = 0;  // error flags
+    err[1] = 0;  // copy tasks recorded
+  }
 }
 
 template <typename T, typename IS, bool RET_POINT>
@@ -178,6 +208,119 @@ struct OutRec {
 template <int G, size_t REC>
 constexpr int stage_per_query() {
   return G == 1 ? 256 : (REC <= 8 ? 128 : 64);
+}
+
+// Copies the msz records of the subtree rooted at slot `root` (tree at
+// slot tb, shape parameter Mt) to the outputs of the queries in qmask (lane
+// j holds query j's index, output offset and capacity): one reservation per
+// query (lanes j together), then the subtree's positions in breadth-first
+// order as one flat range (position x is level k = floor(log2(x + 1)), local
+// slot (r + 1) 2^k - 1 + x + 1 - 2^k), U x 64 positions per pass with every
+// load of the pass issued before its stores (one memory round trip per
+// pass); positions past the tree's last points are skipped. UW is the
+// pass width for 4-byte records (scaled down for larger ones).
+template <typename T, typename IS, bool RET_POINT, int G, int UW>
+__device__ __attribute__((always_inline)) inline void copy_subtree(const Params<T, IS>& P, uint32_t lane,
+                                                                   uint32_t qid, unsigned long long qoff,
+                                                                   unsigned long long qcap, uint32_t root,
+                                                                   uint32_t tb, uint32_t Mt, uint32_t msz,
+                                                                   uint32_t qmask) {
+  using Rec = typename OutRec<T, IS, RET_POINT>::type;
+  constexpr int U = sizeof(Rec) <= 4 ? UW : (sizeof(Rec) <= 8 ? UW / 2 : UW / 4);
+  unsigned long long b = 0;
+  uint32_t fit = 0;
+  if (lane < G && ((qmask >> lane) & 1u)) {
+    b = atomicAdd(&P.counts[qid], static_cast<unsigned long long>(msz));
+    const unsigned long long room_q = b < qcap ? qcap - b : 0;
+    const unsigned long long at = qoff + b;
+    const unsigned long long room_o = at < P.out_total ? P.out_total - at : 0;
+    const unsigned long long r = room_q < room_o ? room_q : room_o;
+    fit = static_cast<uint32_t>(r < msz ? r : msz);
+    b = at;
+  }
+  const uint32_t rl = root - tb;
+  const uint32_t levels = 32u - static_cast<uint32_t>(__clz(static_cast<int>(msz)));
+  const uint32_t npos = (1u << levels) - 1u;
+  // a subtree with every position filled needs no per-position check
+  const bool complete = msz == npos;
+  auto local_slot = [&](uint32_t x) {
+    const uint32_t k = 31u - static_cast<uint32_t>(__clz(static_cast<int>(x + 1)));
+    return ((rl + 1) << k) + x - (1u << k);
+  };
+  auto exists = [&](uint32_t x) { return x < npos && (complete || subtree_size32(Mt, local_slot(x)) >= 1); };
+  uint32_t written = 0;
+  for (uint32_t x0 = 0; x0 < npos; x0 += kWave * U) {
+    Rec rec[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t x = x0 + u * kWave + lane;
+      if (exists(x)) {
+        const uint32_t ls = local_slot(x);
+        if constexpr (RET_POINT) {
+          const Node<T, IS> nd2 = P.nodes[tb + ls];
+          rec[u].dim1_val = nd2.dim1;
+          rec[u].dim2_val = nd2.dim2;
+          if constexpr (has_id_v<IS>) rec[u].id = nd2.id;
+        } else {
+          rec[u] = P.ids[tb + ls];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t x = x0 + u * kWave + lane;
+      const bool ex = exists(x);
+      const uint64_t em = __ballot(ex);
+      const uint32_t idx = written + lanes_below(em);
+#pragma unroll
+      for (int j = 0; j < G; j++) {
+        if (!((qmask >> j) & 1u)) continue;
+        Rec* out = static_cast<Rec*>(P.out) + read_lane(b, j);
+        if (ex && idx < read_lane(fit, j)) out[idx] = rec[u];
+      }
+      written += __popcll(em);
+    }
+  }
+}
+
+// Copies the recorded tasks' subtrees: one task per wave at a time. The
+// walk's tasks were written by the previous launch and are read with
+// device-coherent loads.
+template <typename T, typename IS, bool RET_POINT, int G>
+__global__ __launch_bounds__(kWave) void k_copy_tasks(Params<T, IS> P) {
+  const uint32_t lane = threadIdx.x;
+  const unsigned long long n =
+      min(__hip_atomic_load(P.task_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), P.task_cap);
+  for (unsigned long long t = blockIdx.x; t < n; t += gridDim.x) {
+    const uint64_t group = __hip_atomic_load(&P.tasks[t].group, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t root = __hip_atomic_load(&P.tasks[t].root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t qmask = __hip_atomic_load(&P.tasks[t].qmask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (qmask == 0) continue;
+    if (group >= P.ngroups || root >= P.S || qmask >= (1u << G)) {
+      if (lane == 0) atomicOr(P.err, 2ull);
+      continue;
+    }
+    const uint32_t tree = root >> P.tree_shift;
+    const uint32_t tb = tree << P.tree_shift;
+    const uint32_t Mt = tree + 1 < P.ntrees ? P.full_M : P.last_M;
+    const int32_t msz = subtree_size32(Mt, root - tb);
+    if (msz < 1) {
+      if (lane == 0) atomicOr(P.err, 2ull);
+      continue;
+    }
+    uint32_t qid = ~0u;
+    unsigned long long qoff = 0, qcap = 0;
+    if (lane < G) {
+      qid = (G == 1) ? static_cast<uint32_t>(group) : P.gq[group * G + lane];
+      if (qid < P.nq) {
+        qoff = P.offsets ? P.offsets[qid] : 0;
+        qcap = P.caps ? P.caps[qid] : P.cap_all;
+      }
+    }
+    qmask &= static_cast<uint32_t>(__ballot(lane < G && qid < P.nq));
+    copy_subtree<T, IS, RET_POINT, G, kCopyPass>(P, lane, qid, qoff, qcap, root, tb, Mt, static_cast<uint32_t>(msz),
+                                                 qmask);
+  }
 }
 
 template <typename T, typename IS, bool RET_POINT, bool WRITE, int G>
@@ -341,76 +484,6 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
         __builtin_amdgcn_wave_barrier();
       }
     };
-    // Copies the msz records of the subtree rooted at slot `root` (tree at
-    // slot tb, shape parameter Mt) to the outputs of the queries in qmask:
-    // one reservation per query (lanes j together), then the subtree's
-    // positions in breadth-first order as one flat range (position x is
-    // level k = floor(log2(x + 1)), local slot (r + 1) 2^k - 1 + x + 1 - 2^k),
-    // U x 64 positions per pass with every load of the pass issued
-    // before its stores (one memory round trip per pass, 2048 positions for
-    // 4-byte records); positions past the tree's last points are skipped.
-    auto copy_subtree = [&](uint32_t root, uint32_t tb, uint32_t Mt, uint32_t msz, uint32_t qmask) {
-      if constexpr (WRITE) {
-        constexpr int U = sizeof(Rec) <= 4 ? 32 : (sizeof(Rec) <= 8 ? 16 : 8);
-        unsigned long long b = 0;
-        uint32_t fit = 0;
-        if (lane < G && ((qmask >> lane) & 1u)) {
-          b = atomicAdd(&P.counts[qid], static_cast<unsigned long long>(msz));
-          const unsigned long long room_q = b < qcap ? qcap - b : 0;
-          const unsigned long long at = qoff + b;
-          const unsigned long long room_o = at < P.out_total ? P.out_total - at : 0;
-          const unsigned long long r = room_q < room_o ? room_q : room_o;
-          fit = static_cast<uint32_t>(r < msz ? r : msz);
-          b = at;
-        }
-        const uint32_t rl = root - tb;
-        const uint32_t levels = 32u - static_cast<uint32_t>(__clz(static_cast<int>(msz)));
-        const uint32_t npos = (1u << levels) - 1u;
-        // a subtree with every position filled needs no per-position check
-        const bool complete = msz == npos;
-        auto local_slot = [&](uint32_t x) {
-          const uint32_t k = 31u - static_cast<uint32_t>(__clz(static_cast<int>(x + 1)));
-          return ((rl + 1) << k) + x - (1u << k);
-        };
-        auto exists = [&](uint32_t x) {
-          return x < npos && (complete || subtree_size32(Mt, local_slot(x)) >= 1);
-        };
-        uint32_t written = 0;
-        for (uint32_t x0 = 0; x0 < npos; x0 += kWave * U) {
-          Rec rec[U];
-#pragma unroll
-          for (int u = 0; u < U; u++) {
-            const uint32_t x = x0 + u * kWave + lane;
-            if (exists(x)) {
-              const uint32_t ls = local_slot(x);
-              if constexpr (RET_POINT) {
-                const Node<T, IS> nd2 = P.nodes[tb + ls];
-                rec[u].dim1_val = nd2.dim1;
-                rec[u].dim2_val = nd2.dim2;
-                if constexpr (has_id_v<IS>) rec[u].id = nd2.id;
-              } else {
-                rec[u] = P.nodes[tb + ls].id;
-              }
-            }
-          }
-#pragma unroll
-          for (int u = 0; u < U; u++) {
-            const uint32_t x = x0 + u * kWave + lane;
-            const bool ex = exists(x);
-            const uint64_t em = __ballot(ex);
-            const uint32_t idx = written + lanes_below(em);
-#pragma unroll
-            for (int j = 0; j < G; j++) {
-              if (!((qmask >> j) & 1u)) continue;
-              Rec* out = static_cast<Rec*>(P.out) + read_lane(b, j);
-              if (ex && idx < read_lane(fit, j)) out[idx] = rec[u];
-            }
-            written += __popcll(em);
-          }
-        }
-      }
-    };
-
     // a step adds at most 64 results per query; a full stage triggers a
     // flush of every stage at least this full
     constexpr uint32_t kFlushAll = kStage > kWave ? (kStage - kWave) / 2 + 1 : 1;
@@ -506,14 +579,33 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
         }
       } else {
         uint64_t bm = __ballot(bulk != 0);
-        while (bm) {
-          const int src = __builtin_ctzll(bm);
-          bm &= bm - 1;
-          copy_subtree(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(slot), src)),
-                       static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(tbase), src)),
-                       static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(tM), src)),
-                       static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m), src)),
-                       static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(bulk), src)));
+        if (bm) {
+          // record the subtrees as copy tasks (one list reservation for the
+          // step); a reservation that does not fit marks its in-capacity
+          // part empty and the wave copies the subtrees itself
+          const uint32_t nb = __popcll(bm);
+          bool listed = false;
+          if (P.tasks) {
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(P.task_ctr, static_cast<unsigned long long>(nb));
+            base = __shfl(base, 0, kWave);
+            listed = base < P.task_cap && nb <= P.task_cap - base;
+            if (listed) {
+              if (bulk) P.tasks[base + lanes_below(bm)] = CopyTask{slot, bulk, g};
+            } else if (lane < nb && base + lane < P.task_cap) {
+              P.tasks[base + lane] = CopyTask{0, 0, kEmpty};
+            }
+          }
+          while (!listed && bm) {
+            const int src = __builtin_ctzll(bm);
+            bm &= bm - 1;
+            copy_subtree<T, IS, RET_POINT, G, kCopyPassWalk>(
+                P, lane, qid, qoff, qcap, static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(slot), src)),
+                static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(tbase), src)),
+                static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(tM), src)),
+                static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m), src)),
+                static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(bulk), src)));
+          }
         }
       }
       ia &= ~bulk;
@@ -770,6 +862,8 @@ Status launch_rounds(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
   Params<T, IS> P{};
   P.nodes = static_cast<const Node<T, IS>*>(a.nodes);
   P.submin = static_cast<const T*>(a.submin);
+  P.ids = static_cast<const IS*>(a.ids);
+  if (W && !RP && a.copy_max > 0 && !a.ids) return Status{1, "Error: the tree has no ID array"};
   if (!W) {
     P.check_min = kCheckMinCount;
     P.copy_max = 0;
@@ -833,10 +927,30 @@ Status launch_rounds(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
     sc.gstack_bytes = gbytes;
   }
   P.gstack = sc.gstack;
+  if (W && P.copy_max > 0) {
+    if (sc.task_cap < kCopyTaskCap) {
+      if (sc.tasks) hipFree(sc.tasks);
+      sc.tasks = nullptr;
+      sc.task_cap = 0;
+      e = hipMalloc(&sc.tasks, kCopyTaskCap * sizeof(CopyTask));
+      if (e != hipSuccess) return hip_status(e, "Error in allocating the copy task list");
+      sc.task_cap = kCopyTaskCap;
+    }
+    P.tasks = static_cast<CopyTask*>(sc.tasks);
+    P.task_cap = sc.task_cap;
+    P.task_ctr = sc.ctr + 2 * kMaxRounds + 3;
+  }
   const int R = std::max(1, std::min(a.rounds, kMaxRounds));
   for (int r = 0; r < R; r++) {
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kWave), 0, s, P, static_cast<uint32_t>(r),
                        static_cast<uint32_t>(r == R - 1));
+  }
+  if constexpr (W) {
+    if (P.tasks) {
+      // the copy kernel is a streaming loop with no LDS: fill every CU
+      hipLaunchKernelGGL((k_copy_tasks<T, IS, RP, G>), dim3(static_cast<uint32_t>(cus * kCopyWavesPerCu)),
+                         dim3(kWave), 0, s, P);
+    }
   }
   return hip_status(hipGetLastError(), "Error in launching search kernel");
 }
@@ -950,6 +1064,7 @@ void scratch_release(SearchScratch& sc) {
   if (sc.gkeys) hipFree(sc.gkeys);
   if (sc.gtmp) hipFree(sc.gtmp);
   if (sc.gstack) hipFree(sc.gstack);
+  if (sc.tasks) hipFree(sc.tasks);
   sc = SearchScratch{};
 }
 
