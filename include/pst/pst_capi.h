@@ -263,8 +263,10 @@ int pst_gpu_batch_graph_destroy(pst_graph* graph);
  * points, is not walked but recorded as a copy task; after the walk a copy
  * kernel writes each task's subtree as one block (one reservation per
  * query, breadth-first loads from a per-slot ID array). copy_max = 0 turns
- * copies off. Defaults 256..65535. Results are the same either way. */
-int pst_gpu_set_subtree_copy(pst_tree* tree, int copy_min, int copy_max);
+ * copies off. Defaults 256..65535. max_tasks bounds the task list (0: the
+ * default of 4M entries, also the most); subtrees found once it is full are
+ * copied by the walk itself. Results are the same either way. */
+int pst_gpu_set_subtree_copy(pst_tree* tree, int copy_min, int copy_max, size_t max_tasks);
 /* Queries per tree walk in batched searches: 8 (default; a batch is sorted
  * by leading dim1 bound and each walk serves 8 neighbouring queries, with
  * each query's results exactly those of its own walk) or 1 (one walk per

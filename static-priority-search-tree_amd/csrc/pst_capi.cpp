@@ -25,6 +25,7 @@ struct pst_tree {
   int rounds = 4, budget_small = 0, budget_large = 0, waves_per_cu = 0;
   int group = pst::kGroup;  // queries per walk in batched searches
   int copy_min = 256, copy_max = 65535;  // write pass: full subtrees copied in this size range
+  size_t copy_tasks = 0;                 // copy task list entries (0: library default)
   hipStream_t stream = nullptr;
   unsigned long long* debug = nullptr;  // per-round search counters (debug builds)
 };
@@ -121,6 +122,7 @@ pst::SearchArgs base_args(pst_tree* t) {
   a.group = t->group;
   a.copy_min = t->copy_min;
   a.copy_max = t->copy_max;
+  a.copy_tasks = t->copy_tasks;
   a.debug = t->debug;
   return a;
 }
@@ -681,12 +683,13 @@ int pst_gpu_batch_graph_destroy(pst_graph* g) {
   return PST_OK;
 }
 
-int pst_gpu_set_subtree_copy(pst_tree* t, int copy_min, int copy_max) {
+int pst_gpu_set_subtree_copy(pst_tree* t, int copy_min, int copy_max, size_t max_tasks) {
   if (!t) return fail(PST_ERR_INVALID, "Error: null tree");
   if (copy_max < 0 || copy_min < 0 || (copy_max > 0 && copy_min > copy_max))
     return fail(PST_ERR_INVALID, "Error: need 0 <= copy_min <= copy_max (copy_max 0: off)");
   t->copy_min = copy_min;
   t->copy_max = copy_max;
+  t->copy_tasks = max_tasks;
   return PST_OK;
 }
 

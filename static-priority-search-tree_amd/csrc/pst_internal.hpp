@@ -108,6 +108,7 @@ struct SearchArgs {
   int waves_per_cu;
   int group;            // 1: one query per walk; otherwise kGroup
   int copy_min, copy_max;  // write pass: full subtrees of this many points are copied (copy_max 0: never)
+  size_t copy_tasks;       // copy task list entries (0: default; beyond it the walk copies)
   int groups_ready;     // the group table in the scratch already holds this batch
   const void* single_query;  // host Query<T> for one query (queries ignored)
   unsigned long long* debug;  // optional per-round counters (8 per round)

@@ -241,33 +241,61 @@ __device__ __attribute__((always_inline)) inline void copy_subtree(const Params<
   const uint32_t rl = root - tb;
   const uint32_t levels = 32u - static_cast<uint32_t>(__clz(static_cast<int>(msz)));
   const uint32_t npos = (1u << levels) - 1u;
-  // a subtree with every position filled needs no per-position check
+  // a subtree with every position filled (all but those on the tree's
+  // right edge) needs no per-position check
   const bool complete = msz == npos;
   auto local_slot = [&](uint32_t x) {
     const uint32_t k = 31u - static_cast<uint32_t>(__clz(static_cast<int>(x + 1)));
     return ((rl + 1) << k) + x - (1u << k);
   };
-  auto exists = [&](uint32_t x) { return x < npos && (complete || subtree_size32(Mt, local_slot(x)) >= 1); };
-  uint32_t written = 0;
-  for (uint32_t x0 = 0; x0 < npos; x0 += kWave * U) {
-    Rec rec[U];
+  auto load_rec = [&](uint32_t ls, Rec& r) {
+    if constexpr (RET_POINT) {
+      const Node<T, IS> nd2 = P.nodes[tb + ls];
+      r.dim1_val = nd2.dim1;
+      r.dim2_val = nd2.dim2;
+      if constexpr (has_id_v<IS>) r.id = nd2.id;
+    } else {
+      r = P.ids[tb + ls];
+    }
+  };
+  if (complete) {
+    // position x goes to output index x; positions past the subtree load
+    // the root again (discarded), so every load of a pass is issued
+    for (uint32_t x0 = 0; x0 < npos; x0 += kWave * U) {
+      Rec rec[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const uint32_t x = x0 + u * kWave + lane;
-      if (exists(x)) {
-        const uint32_t ls = local_slot(x);
-        if constexpr (RET_POINT) {
-          const Node<T, IS> nd2 = P.nodes[tb + ls];
-          rec[u].dim1_val = nd2.dim1;
-          rec[u].dim2_val = nd2.dim2;
-          if constexpr (has_id_v<IS>) rec[u].id = nd2.id;
-        } else {
-          rec[u] = P.ids[tb + ls];
+      for (int u = 0; u < U; u++) {
+        const uint32_t x = x0 + u * kWave + lane;
+        load_rec(x < npos ? local_slot(x) : rl, rec[u]);
+      }
+      uint32_t qm = qmask;
+      while (qm) {
+        const int j = __builtin_ctz(qm);
+        qm &= qm - 1;
+        Rec* out = static_cast<Rec*>(P.out) + read_lane(b, j);
+        const uint32_t lim = read_lane(fit, j);  // at most msz = npos
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const uint32_t x = x0 + u * kWave + lane;
+          if (x < lim) out[x] = rec[u];
         }
       }
     }
+    return;
+  }
+  // subtrees on the tree's right edge: narrower passes (rare)
+  constexpr int UE = U >= 8 ? U / 4 : U;
+  auto exists = [&](uint32_t x) { return x < npos && subtree_size32(Mt, local_slot(x)) >= 1; };
+  uint32_t written = 0;
+  for (uint32_t x0 = 0; x0 < npos; x0 += kWave * UE) {
+    Rec rec[UE];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
+    for (int u = 0; u < UE; u++) {
+      const uint32_t x = x0 + u * kWave + lane;
+      if (exists(x)) load_rec(local_slot(x), rec[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < UE; u++) {
       const uint32_t x = x0 + u * kWave + lane;
       const bool ex = exists(x);
       const uint64_t em = __ballot(ex);
@@ -928,16 +956,17 @@ Status launch_rounds(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
   }
   P.gstack = sc.gstack;
   if (W && P.copy_max > 0) {
-    if (sc.task_cap < kCopyTaskCap) {
+    const size_t want = a.copy_tasks > 0 ? std::min(a.copy_tasks, kCopyTaskCap) : kCopyTaskCap;
+    if (sc.task_cap < want) {
       if (sc.tasks) hipFree(sc.tasks);
       sc.tasks = nullptr;
       sc.task_cap = 0;
-      e = hipMalloc(&sc.tasks, kCopyTaskCap * sizeof(CopyTask));
+      e = hipMalloc(&sc.tasks, want * sizeof(CopyTask));
       if (e != hipSuccess) return hip_status(e, "Error in allocating the copy task list");
-      sc.task_cap = kCopyTaskCap;
+      sc.task_cap = want;
     }
     P.tasks = static_cast<CopyTask*>(sc.tasks);
-    P.task_cap = sc.task_cap;
+    P.task_cap = want;
     P.task_ctr = sc.ctr + 2 * kMaxRounds + 3;
   }
   const int R = std::max(1, std::min(a.rounds, kMaxRounds));

@@ -96,7 +96,7 @@ def lib():
         L.pst_gpu_slots_per_tree.restype = sz
         L.pst_gpu_slots_per_tree.argtypes = [vp]
         L.pst_gpu_autotune.argtypes = [vp, vp, sz, i32, vp, c.POINTER(c.c_int)]
-        L.pst_gpu_set_subtree_copy.argtypes = [vp, i32, i32]
+        L.pst_gpu_set_subtree_copy.argtypes = [vp, i32, i32, sz]
         L.pst_gpu_batch_graph_create.argtypes = [vp, vp, sz, i32, vp, vp, vp, sz, vp, c.POINTER(vp)]
         L.pst_gpu_batch_graph_launch.argtypes = [vp, vp]
         L.pst_gpu_batch_graph_destroy.argtypes = [vp]
@@ -308,9 +308,11 @@ class StaticPSTGPU:
                                                 _stream_ptr(stream), ctypes.byref(g)))
         return BatchGraph(g)
 
-    def set_subtree_copy(self, copy_min: int, copy_max: int):
-        """Write pass: copy fully reported subtrees of copy_min..copy_max points (copy_max 0: off)."""
-        _check(lib().pst_gpu_set_subtree_copy(self.h, copy_min, copy_max))
+    def set_subtree_copy(self, copy_min: int, copy_max: int, max_tasks: int = 0):
+        """Write pass: copy fully reported subtrees of copy_min..copy_max points
+        (copy_max 0: off) through a task list of max_tasks entries (0: default);
+        subtrees found beyond it are copied by the walk."""
+        _check(lib().pst_gpu_set_subtree_copy(self.h, copy_min, copy_max, max_tasks))
 
     def set_grouping(self, queries_per_walk: int):
         """Queries per tree walk in batched searches: 8 (default) or 1."""

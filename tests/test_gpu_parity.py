@@ -160,18 +160,22 @@ def test_nan_input_rejected(torch, P):
     assert e.value.code == 3
 
 
-@pytest.mark.parametrize("group", [8, 1])
+@pytest.mark.parametrize("group,copy", [(8, None), (1, None), (8, (16, 65535, 7))])
 @pytest.mark.parametrize("t,i", [("f32", "u32"), ("i32", "u32"), ("f64", None), ("f32", "u64"),
                                  ("i64", "u32")])
-def test_batched_count_write_equal_single(torch, P, t, i, group):
+def test_batched_count_write_equal_single(torch, P, t, i, group, copy):
     """Count pass + exclusive scan + write pass over a mixed batch equals
-    brute force, with grouped (8 queries per walk) and ungrouped walks."""
+    brute force, with grouped (8 queries per walk) and ungrouped walks, and
+    with whole-subtree copies from 16 points on through a 7-entry task list
+    (most copies then run inside the walk)."""
     rng = np.random.default_rng(9)
     n = 30000
     p = _rand_points(rng, t, i, n, dup=False)
     tr, keep = _tree(P, p, t, i)
     tr.set_grouping(group)
-    print(f"batched {t}/{i} group={group}", flush=True)
+    if copy is not None:
+        tr.set_subtree_copy(*copy)
+    print(f"batched {t}/{i} group={group} copy={copy}", flush=True)
     nq = 300
     qd = P.query_dtype(t)
     qs = np.zeros(nq, qd)
@@ -223,7 +227,8 @@ def test_interval_parallel_search_equals_brute_force(torch, P, t, i):
             assert got == want
 
 
-@pytest.mark.parametrize("group,copy", [(8, None), (1, None), (8, (0, 0)), (8, (16, 8191)), (1, (64, 65535))])
+@pytest.mark.parametrize("group,copy", [(8, None), (1, None), (8, (0, 0)), (8, (16, 8191)), (1, (64, 65535)),
+                                        (8, (16, 65535, 5)), (8, (256, 2047, 300))])
 def test_batch_search_interval_workload(torch, P, group, copy):
     """One-call count + scan + write (pst_gpu_batch_search) on a 2^20-point
     interval tree with a batch of 515 queries: stabbing queries (both
