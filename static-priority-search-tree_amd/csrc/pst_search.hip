@@ -342,7 +342,7 @@ __global__ __launch_bounds__(kWave) void k_copy_tasks(Params<T, IS> P) {
       qid = (G == 1) ? static_cast<uint32_t>(group) : P.gq[group * G + lane];
       if (qid < P.nq) {
         qoff = P.offsets ? P.offsets[qid] : 0;
-        qcap = P.caps ? P.caps[qid] : P.cap_all;
+        qcap = P.caps ? P.caps[qid] : read_lane(P.cap_all, 0);
       }
     }
     qmask &= static_cast<uint32_t>(__ballot(lane < G && qid < P.nq));
@@ -441,7 +441,16 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
     Query<T> qv{};
     if (lane < G) {
       qid = (G == 1) ? g : P.gq[static_cast<uint64_t>(g) * G + lane];
-      if (qid < P.nq) qv = P.queries ? P.queries[qid] : P.single;
+      if (qid < P.nq) {
+        if (P.queries) {
+          qv = P.queries[qid];
+        } else {
+          qv.min_dim1 = read_lane(P.single.min_dim1, 0);
+          qv.max_dim1 = read_lane(P.single.max_dim1, 0);
+          qv.min_dim2 = read_lane(P.single.min_dim2, 0);
+          qv.kind = read_lane(P.single.kind, 0);
+        }
+      }
     }
     Bounds<T> qb[G];
 #pragma unroll
@@ -462,7 +471,7 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
     if constexpr (WRITE) {
       if (lane < G && qid < P.nq) {
         qoff = P.offsets ? P.offsets[qid] : 0;
-        qcap = P.caps ? P.caps[qid] : P.cap_all;
+        qcap = P.caps ? P.caps[qid] : read_lane(P.cap_all, 0);
       }
     }
     // Flushes every query whose stage holds at least `thr` results: lanes
@@ -516,83 +525,57 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
     // flush of every stage at least this full
     constexpr uint32_t kFlushAll = kStage > kWave ? (kStage - kWave) / 2 + 1 : 1;
 
-    uint32_t steps = 0;
-    unsigned long long visits = 0;  // items popped (debug counters only)
-    uint32_t gsp = 0;  // items in the wave's global overflow stack
-    while (sp > 0 || gsp > 0) {
-      if (sp == 0) {
-        // refill from the overflow stack (blocks of kMoveBlock, each lane
-        // reads back the entries it wrote)
-        gsp -= kMoveBlock;
-#pragma unroll
-        for (int k = 0; k < kMoveBlock / kWave; k++) stack[lane + k * kWave] = gstack[gsp + lane + k * kWave];
-        sp = kMoveBlock;
-        __builtin_amdgcn_wave_barrier();
-      }
-      if (sp > kLdsStack - kWave) {
-        // a step grows the stack by at most 64: move the oldest kMoveBlock
-        // items to the overflow stack and shift the rest down
-        if (gsp + kMoveBlock > P.gcap) {
-          if (lane == 0) atomicOr(P.err, 1ull);
-          sp = 0;
-          gsp = 0;
-          break;
-        }
-#pragma unroll
-        for (int k = 0; k < kMoveBlock / kWave; k++) gstack[gsp + lane + k * kWave] = stack[lane + k * kWave];
-        gsp += kMoveBlock;
-        // shift in passes of kMoveBlock: each pass reads a block above the
-        // moved one before writing it kMoveBlock lower
-        for (uint32_t b0 = kMoveBlock; b0 < sp; b0 += kMoveBlock) {
-          Item keep[kMoveBlock / kWave];
-#pragma unroll
-          for (int k = 0; k < kMoveBlock / kWave; k++) {
-            const uint32_t x = b0 + lane + k * kWave;
-            keep[k] = x < sp ? stack[x] : Item(0);
-          }
-          __builtin_amdgcn_wave_barrier();
-#pragma unroll
-          for (int k = 0; k < kMoveBlock / kWave; k++) {
-            const uint32_t x = b0 + lane + k * kWave;
-            if (x < sp) stack[x - kMoveBlock] = keep[k];
-          }
-          __builtin_amdgcn_wave_barrier();
-        }
-        sp -= kMoveBlock;
-        __builtin_amdgcn_wave_barrier();
-      }
-      const uint32_t takecnt = sp < kWave ? sp : kWave;
-      sp -= takecnt;
-      visits += takecnt;
-      const bool act = lane < takecnt;
-      const Item item = act ? stack[sp + lane] : Item(0);
-      __builtin_amdgcn_wave_barrier();
-      const uint32_t slot = It::slot(item);
-      int32_t m = 0;
-      bool ok = act;
-      uint32_t tbase = 0;  // first slot of the item's tree
+    // One batch of popped items: prep() decodes the items and issues their
+    // loads; finish() steps the nodes, reports, and pushes the children.
+    struct Half {
+      Node<T, IS> nd;
+      T sm;  // subtree minimum of dim2 (read for candidates only)
+      uint32_t slot, tbase, tM, ia, il, ih, ifull, cand;
+      int32_t m;
+      bool ok, bad;
+    };
+    // tree shape in scalar registers (a per-lane choice between two kernel
+    // arguments would otherwise become a load from the argument block)
+    const uint32_t tshift = uniform(P.tree_shift), ntrees = uniform(P.ntrees);
+    const uint32_t full_M = uniform(P.full_M), last_M = uniform(P.last_M);
+    auto prep = [&](const Item item, const bool act, Half& h) {
+      h.slot = It::slot(item);
+      h.m = 0;
+      h.ok = act;
+      h.bad = false;
+      h.tbase = 0;
       if (act) {
-        const uint32_t tree = slot >> P.tree_shift;
-        tbase = tree << P.tree_shift;
-        m = subtree_size32(tree + 1 < P.ntrees ? P.full_M : P.last_M, slot - tbase);
-        if (slot >= P.S || m < 1) {
-          atomicOr(P.err, 2ull);
-          ok = false;
+        const uint32_t tree = h.slot >> tshift;
+        h.tbase = tree << tshift;
+        h.m = subtree_size32(tree + 1 < ntrees ? full_M : last_M, h.slot - h.tbase);
+        if (h.slot >= P.S || h.m < 1) {
+          h.bad = true;
+          h.ok = false;
         }
       }
-      uint32_t ia = It::a(item), il = It::l(item), ih = It::h(item), ifull = It::f(item);
-      const uint32_t tM = (slot >> P.tree_shift) + 1 < P.ntrees ? P.full_M : P.last_M;
-      Node<T, IS> nd{};
-      StepOut so{};
+      h.ia = It::a(item);
+      h.il = It::l(item);
+      h.ih = It::h(item);
+      h.ifull = It::f(item);
+      h.tM = (h.slot >> tshift) + 1 < ntrees ? full_M : last_M;
       // REPORT_ABOVE queries not yet known to report the whole subtree
-      const uint32_t cand = (ok && m >= P.check_min) ? (ia & ~il & ~ih & ~ifull) : 0u;
-      T sm{};
-      if (cand) sm = P.submin[slot];
-      if (ok) nd = P.nodes[slot];
-      if (cand) {
+      h.cand = (h.ok && h.m >= P.check_min) ? (h.ia & ~h.il & ~h.ih & ~h.ifull) : 0u;
+      h.sm = P.submin[h.cand ? h.slot : 0u];
+      h.nd = P.nodes[h.ok ? h.slot : 0u];
+    };
+    bool saw_bad = false;  // an invalid item was popped (reported after the walk)
+    auto finish = [&](Half& h) {
+      saw_bad |= h.bad;
+      const uint32_t slot = h.slot, tbase = h.tbase, tM = h.tM, il = h.il, ih = h.ih;
+      const int32_t m = h.m;
+      uint32_t ia = h.ia, ifull = h.ifull;
+      bool ok = h.ok;
+      const Node<T, IS>& nd = h.nd;
+      StepOut so{};
+      if (h.cand) {
 #pragma unroll
         for (int j = 0; j < G; j++)
-          if (((cand >> j) & 1u) && qb[j].min2 <= sm) ifull |= 1u << j;
+          if (((h.cand >> j) & 1u) && qb[j].min2 <= h.sm) ifull |= 1u << j;
       }
       // whole subtrees: counted at once, or copied when small enough
       const uint32_t bulk = (!WRITE || m <= P.copy_max) ? ifull : 0u;
@@ -611,16 +594,16 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
           // record the subtrees as copy tasks (one list reservation for the
           // step); a reservation that does not fit marks its in-capacity
           // part empty and the wave copies the subtrees itself
-          const uint32_t nb = __popcll(bm);
+          const uint32_t ntask = __popcll(bm);
           bool listed = false;
           if (P.tasks) {
             unsigned long long base = 0;
-            if (lane == 0) base = atomicAdd(P.task_ctr, static_cast<unsigned long long>(nb));
+            if (lane == 0) base = atomicAdd(P.task_ctr, static_cast<unsigned long long>(ntask));
             base = __shfl(base, 0, kWave);
-            listed = base < P.task_cap && nb <= P.task_cap - base;
+            listed = base < P.task_cap && ntask <= P.task_cap - base;
             if (listed) {
               if (bulk) P.tasks[base + lanes_below(bm)] = CopyTask{slot, bulk, g};
-            } else if (lane < nb && base + lane < P.task_cap) {
+            } else if (lane < ntask && base + lane < P.task_cap) {
               P.tasks[base + lane] = CopyTask{0, 0, kEmpty};
             }
           }
@@ -688,8 +671,74 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
       if (cr) stack[pos + (cl ? 1u : 0u)] = It::make(lchild + 1, so.ar, so.lr, so.hr, ifull & so.ar);
       sp += __popcll(lm) + __popcll(rm);
       __builtin_amdgcn_wave_barrier();
+    };
+    uint32_t steps = 0;
+    unsigned long long visits = 0;  // items popped (debug counters only)
+    uint32_t gsp = 0;  // items in the wave's global overflow stack
+    while (sp > 0 || gsp > 0) {
+      if (sp == 0) {
+        // refill from the overflow stack (blocks of kMoveBlock, each lane
+        // reads back the entries it wrote)
+        gsp -= kMoveBlock;
+#pragma unroll
+        for (int k = 0; k < kMoveBlock / kWave; k++) stack[lane + k * kWave] = gstack[gsp + lane + k * kWave];
+        sp = kMoveBlock;
+        __builtin_amdgcn_wave_barrier();
+      }
+      if (sp > kLdsStack - 2 * kWave) {
+        // a step (two batches) grows the stack by at most 128: move the
+        // oldest kMoveBlock items to the overflow stack and shift the rest
+        // down
+        if (gsp + kMoveBlock > P.gcap) {
+          if (lane == 0) atomicOr(P.err, 1ull);
+          sp = 0;
+          gsp = 0;
+          break;
+        }
+#pragma unroll
+        for (int k = 0; k < kMoveBlock / kWave; k++) gstack[gsp + lane + k * kWave] = stack[lane + k * kWave];
+        gsp += kMoveBlock;
+        // shift in passes of kMoveBlock: each pass reads a block above the
+        // moved one before writing it kMoveBlock lower
+        for (uint32_t b0 = kMoveBlock; b0 < sp; b0 += kMoveBlock) {
+          Item keep[kMoveBlock / kWave];
+#pragma unroll
+          for (int k = 0; k < kMoveBlock / kWave; k++) {
+            const uint32_t x = b0 + lane + k * kWave;
+            keep[k] = x < sp ? stack[x] : Item(0);
+          }
+          __builtin_amdgcn_wave_barrier();
+#pragma unroll
+          for (int k = 0; k < kMoveBlock / kWave; k++) {
+            const uint32_t x = b0 + lane + k * kWave;
+            if (x < sp) stack[x - kMoveBlock] = keep[k];
+          }
+          __builtin_amdgcn_wave_barrier();
+        }
+        sp -= kMoveBlock;
+        __builtin_amdgcn_wave_barrier();
+      }
+      // ---- pop two batches of up to 64 items, A (the newest) and B below
+      // it, and load both batches' nodes before A is processed, so that B's
+      // loads are in flight while A is stepped. Every lane issues both loads
+      // of each batch (lanes without a valid item read slot 0, discarded),
+      // which keeps the number of loads after A's fixed.
+      const uint32_t na = sp < kWave ? sp : kWave;
+      sp -= na;
+      const uint32_t nb = sp < kWave ? sp : kWave;
+      sp -= nb;
+      visits += na + nb;
+      const Item item_a = lane < na ? stack[sp + nb + lane] : Item(0);
+      const Item item_b = lane < nb ? stack[sp + lane] : Item(0);
+      __builtin_amdgcn_wave_barrier();
+      Half A, B;
+      prep(item_a, lane < na, A);
+      prep(item_b, lane < nb, B);
+      finish(A);
+      if (nb) finish(B);
       // ---- hand remaining work to the next round when over budget
-      if (++steps >= budget && sp + gsp > 0) {
+      steps += nb ? 2 : 1;  // batches stepped
+      if (steps >= budget && sp + gsp > 0) {
         // Reservations only grow the counter; one that ends past the list
         // capacity fails, marks its in-capacity part empty (the next round
         // skips empty entries and reads at most list_cap of them) and the
@@ -714,6 +763,7 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
         }
       }
     }
+    if (saw_bad) atomicOr(P.err, 2ull);
     if (P.dbg && lane == 0) {
       atomicAdd(&P.dbg[round * 8 + 6], static_cast<unsigned long long>(steps));
       atomicMax(&P.dbg[round * 8 + 7], static_cast<unsigned long long>(steps));
@@ -926,9 +976,9 @@ Status launch_rounds(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
   P.nq = static_cast<uint32_t>(a.nq);
   P.ngroups = static_cast<uint32_t>((a.nq + G - 1) / G);
   const uint32_t H = sh.levels;
-  // deepest stack: 64 items per level below the first popped batch, plus a
-  // step's growth, in whole move blocks
-  const uint32_t bound = kWave * (H + 2) + 2 * kWave;
+  // deepest stack: 128 items (a step's two batches) per level below the
+  // first popped batch, plus a step's growth, in whole move blocks
+  const uint32_t bound = 2 * kWave * (H + 2) + 4 * kWave;
   P.gcap = ((bound + kMoveBlock - 1) / kMoveBlock) * kMoveBlock;
   P.budget_small = a.budget_small > 0 ? a.budget_small : 6;
   P.budget_large = a.budget_large > 0 ? a.budget_large : 384;
