@@ -29,6 +29,8 @@ sys.path.insert(0, PKG)
 
 HBM_PEAK_GBPS = 8000.0
 BYTES_PER_ID = 21  # 3 x f32 (dim1, dim2, med) + u32 id + 1 B bitcode read, u32 written
+# kernel launches of one write pass: four walk rounds and the subtree copy kernel
+WRITE_LAUNCHES = 5
 
 
 def log(*a):
@@ -430,7 +432,7 @@ def main():
     # HBM bytes per launch of the same kernel on this workload, from the
     # committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated)
     traffic, traffic_src = None, None
-    tpath = os.path.join(ROOT, "profiles", "k_search_round_write_f32u32_2e26_batch2048.traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "write_pass_k_search_round_k_copy_tasks_f32u32_2e26_batch2048.traffic.json")
     if args.log2n == 26 and args.nq2 == 1024 and args.nq3 == 1024 and os.path.exists(tpath):
         with open(tpath) as f:
             tj = json.load(f)
@@ -567,9 +569,10 @@ def main():
         "write_pass_ms": round(t_write, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                     "kernel": "k_search_round<float, unsigned int, false, true, 8> (write pass: 4 launches)",
-                     "launches_per_pass": 4, "avg_launch_ms": round(t_write / 4, 3),
-                     "alg_bytes_per_launch": round(alg_bytes / 4),
+                     "kernel": "write pass: k_search_round<float, unsigned int, false, true, 8> x 4 "
+                               "+ k_copy_tasks<float, unsigned int, false, 8> x 1",
+                     "launches_per_pass": WRITE_LAUNCHES, "avg_launch_ms": round(t_write / WRITE_LAUNCHES, 3),
+                     "alg_bytes_per_launch": round(alg_bytes / WRITE_LAUNCHES),
                      "traffic_source": traffic_src},
         "cpu_baseline": cpu,
         "plain_per_query_qps": round(plain_qps, 2),
