@@ -49,7 +49,8 @@ enum pst_status {
   PST_ERR_HIP = 2,      /* a HIP runtime call failed */
   PST_ERR_NAN = 3,      /* a dimension (or floating ID) is NaN: no order */
   PST_ERR_CAPACITY = 4, /* results exceeded the caller's buffer (counted only) */
-  PST_ERR_INTERNAL = 5
+  PST_ERR_INTERNAL = 5,
+  PST_ERR_IO = 6        /* a file could not be opened, read or written, or is not a valid tree file */
 };
 
 /* Data type codes: T in {F32, F64, I32, I64} (the reference's float,
@@ -180,6 +181,24 @@ size_t pst_gpu_arr_calc_global_mem_needed(size_t num_elems, unsigned threads_per
 size_t pst_gpu_slots_per_tree(const pst_tree* tree);
 
 /* ------------------------------------------------------------------ */
+/* Tree persistence (no counterpart in the reference, which only       */
+/* prints its trees: pst/static-pst-gpu.tu:441-483)                    */
+/* ------------------------------------------------------------------ */
+
+/* Writes a built tree or forest to `path`: a 64-byte header (magic
+ * "PSTAMD", version 1, kind 0 = StaticPSTGPU / 1 = StaticPSTGPUArr, dtype,
+ * id_dtype, n, S, slots per forest tree, checksum) followed by the node
+ * records as SoA sections dim1[S], dim2[S], med[S] and, with IDs, id[S], in
+ * slot order (slots without a point are zero). Blocking. */
+int pst_gpu_save(const pst_tree* tree, const char* path);
+/* Creates a handle from a file written by pst_gpu_save, without the input
+ * points and without rebuilding: a StaticPSTGPU or StaticPSTGPUArr handle
+ * as the file says (pst_gpu_slots_per_tree tells which). The header is
+ * checked against the file size and the sections against the checksum
+ * (PST_ERR_IO on any mismatch). Blocking; searches then run on `stream`. */
+int pst_gpu_load(const char* path, int dev_ind, int num_devs, pst_stream_t stream, pst_tree** out);
+
+/* ------------------------------------------------------------------ */
 /* Metacell formation (helpers/isosurface-data-processing.h:21-137)    */
 /* ------------------------------------------------------------------ */
 
@@ -204,6 +223,20 @@ int pst_form_metacell_tags(const void* vertex_arr_d, int dtype, int id_dtype, co
  * counterpart in the reference. */
 int pst_form_metacells_into(const void* vertex_arr_d, int dtype, int id_dtype, const size_t pt_grid_dims[3],
                             const size_t metacell_dims[3], void* metacell_arr_d, pst_stream_t stream);
+/* Replaces readInVertices<T>(input_filename, num_vertices)
+ * (helpers/isosurface-data-processing.h:205-229): reads num_vertices
+ * values of dtype (native byte order, x fastest) from a raw binary file into
+ * the caller's host array. A file that cannot be opened gives "Failed to
+ * open <file> for reading", one that is too short "Failure: read <have>
+ * bytes, expected <want> bytes" (PST_ERR_IO; the reference prints these and
+ * exits with FILE_OPEN_ERR / INPUT_READ_ERR). */
+int pst_read_vertices(const char* path, int dtype, size_t num_vertices, void* vertex_arr);
+/* The same straight into device memory (vertex_arr_d holds num_vertices
+ * values), through two pinned staging buffers so that reading overlaps the
+ * copy. No counterpart in the reference. Blocking. */
+int pst_read_vertices_to_device(const char* path, int dtype, size_t num_vertices, void* vertex_arr_d,
+                                pst_stream_t stream);
+
 /* Synthetic f32 volume (no counterpart in the reference; a seeded input of
  * the isosurface workload): value = sum_g amp_g * exp(-|p - c_g|^2 /
  * (2 s_g^2)) + noise_std * N(0, 1) at p = (i, j, k) / (dims - 1), x fastest.

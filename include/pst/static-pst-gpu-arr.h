@@ -94,7 +94,24 @@ class StaticPSTGPUArr : public StaticPrioritySearchTree<T, PointStructTemplate, 
   // Beyond the reference: the handle for the batched entry points.
   pst_tree* handle() const { return tree; }
 
+  // Beyond the reference: persistence of the built tree (pst_gpu_save,
+  // pst_gpu_load). load() needs neither the input points nor a rebuild.
+  void save(const std::string& path) const { pst_amd::check(pst_gpu_save(tree, path.c_str())); }
+  static std::unique_ptr<StaticPSTGPUArr> load(const std::string& path, int dev_ind = 0, int num_devs = 1) {
+    pst_tree* h = nullptr;
+    pst_amd::check(pst_gpu_load(path.c_str(), dev_ind, num_devs, nullptr, &h));
+    if (pst_gpu_slots_per_tree(h) == 0 || pst_gpu_dtype(h) != pst_amd::dtype_code<T>() ||
+        pst_gpu_id_dtype(h) != pst_amd::id_code<IDType, num_IDs>()) {
+      pst_gpu_destroy(h);
+      throw std::runtime_error("StaticPSTGPUArr::load: " + path + " holds a different tree type");
+    }
+    return std::unique_ptr<StaticPSTGPUArr>(new StaticPSTGPUArr(h, pst_gpu_num_elems(h), dev_ind, num_devs));
+  }
+
  private:
+  StaticPSTGPUArr(pst_tree* h, size_t num_elems, int dev_ind, int num_devs)
+      : tree(h), num_elems(num_elems), dev_ind(dev_ind), num_devs(num_devs), dev_props{} {}
+
   template <typename RetType>
   static constexpr int ret_code() {
     return std::is_same<RetType, Point>::value ? PST_RET_POINT : PST_RET_ID;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -351,6 +352,73 @@ int pst_gpu_arr_create(const void* pt_arr_d, size_t n, int dtype, int id_dtype, 
 }
 
 size_t pst_gpu_slots_per_tree(const pst_tree* t) { return t ? t->slots_per_tree : 0; }
+
+int pst_gpu_save(const pst_tree* t, const char* path) {
+  if (!t || !path) return fail(PST_ERR_INVALID, "Error: null tree or path");
+  Status st = pst::save_tree_file(path, t->slots_per_tree ? 1 : 0, t->dtype, t->id_dtype, t->n, t->S,
+                                  t->slots_per_tree, t->nodes, t->stream);
+  if (!st.ok()) return fail(st);
+  return PST_OK;
+}
+
+int pst_gpu_load(const char* path, int dev_ind, int num_devs, pst_stream_t stream, pst_tree** out) {
+  if (!out || !path) return fail(PST_ERR_INVALID, "Error: null output handle or path");
+  *out = nullptr;
+  pst::TreeFileInfo info{};
+  Status st = pst::read_tree_header(path, &info);
+  if (!st.ok()) return fail(st);
+  pst_tree* t = new pst_tree();
+  t->dtype = info.dtype;
+  t->id_dtype = info.id_dtype;
+  t->n = info.n;
+  t->S = info.S;
+  t->slots_per_tree = info.slots_per_tree;
+  t->dev_ind = dev_ind;
+  t->num_devs = num_devs;
+  t->stream = reinterpret_cast<hipStream_t>(stream);
+  hipGetDevice(&t->device);
+  if (t->n == 0) {
+    *out = t;
+    return PST_OK;
+  }
+  hipError_t e = hipMalloc(&t->nodes, t->S * pst::node_size(t->dtype, t->id_dtype));
+  if (e == hipSuccess) e = hipMalloc(&t->submin, t->S * pst::dtype_size(t->dtype));
+  if (e == hipSuccess && id_size(t->id_dtype) > 0) e = hipMalloc(&t->ids, t->S * id_size(t->id_dtype));
+  if (e != hipSuccess) {
+    std::string m = "Error in allocating priority search tree storage array" + dev_suffix(dev_ind, num_devs) +
+                    hipGetErrorString(e);
+    pst_gpu_destroy(t);
+    return fail(PST_ERR_HIP, m);
+  }
+  st = pst::load_tree_file(path, info, t->nodes, t->stream);
+  if (st.ok())
+    st = pst::build_submin(t->dtype, t->id_dtype, t->n, t->slots_per_tree, t->nodes, t->submin, t->ids, t->stream);
+  if (st.ok()) st = pst::hip_status(hipStreamSynchronize(t->stream), "Error in synchronising after loading");
+  if (!st.ok()) {
+    pst_gpu_destroy(t);
+    return fail(st);
+  }
+  *out = t;
+  return PST_OK;
+}
+
+int pst_read_vertices(const char* path, int dtype, size_t num_vertices, void* vertex_arr) {
+  if (!path || (!vertex_arr && num_vertices)) return fail(PST_ERR_INVALID, "Error: null path or output");
+  if (!valid_t(dtype)) return fail(PST_ERR_INVALID, "Error: unsupported data type code " + std::to_string(dtype));
+  Status st = pst::read_raw(path, num_vertices * pst::dtype_size(dtype), vertex_arr, nullptr, nullptr);
+  if (!st.ok()) return fail(st);
+  return PST_OK;
+}
+
+int pst_read_vertices_to_device(const char* path, int dtype, size_t num_vertices, void* vertex_arr_d,
+                                pst_stream_t stream) {
+  if (!path || (!vertex_arr_d && num_vertices)) return fail(PST_ERR_INVALID, "Error: null path or output");
+  if (!valid_t(dtype)) return fail(PST_ERR_INVALID, "Error: unsupported data type code " + std::to_string(dtype));
+  Status st = pst::read_raw(path, num_vertices * pst::dtype_size(dtype), nullptr, vertex_arr_d,
+                            reinterpret_cast<hipStream_t>(stream));
+  if (!st.ok()) return fail(st);
+  return PST_OK;
+}
 int pst_gpu_dtype(const pst_tree* t) { return t ? t->dtype : -2; }
 int pst_gpu_id_dtype(const pst_tree* t) { return t ? t->id_dtype : -2; }
 

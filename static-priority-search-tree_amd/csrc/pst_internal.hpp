@@ -137,4 +137,21 @@ Status gauss_volume(float* out_d, const uint32_t dims[3], const float* params_d,
 // helpers
 Status hip_status(hipError_t e, const std::string& what);
 
+// ---- tree files and raw volumes (pst_persist.hip) ----
+struct TreeFileInfo {
+  int kind;  // 0: one deep tree, 1: forest
+  int dtype, id_dtype;
+  size_t n, S, slots_per_tree;
+  uint64_t checksum;
+};
+// Writes the node records of a tree (S slots) to `path`.
+Status save_tree_file(const char* path, int kind, int dtype, int id_dtype, size_t n, size_t S,
+                      size_t slots_per_tree, const void* nodes_d, hipStream_t stream);
+// Reads and validates a tree file's header (no device work).
+Status read_tree_header(const char* path, TreeFileInfo* info);
+// Reads the node records into nodes_d (info.S slots) and checks the checksum.
+Status load_tree_file(const char* path, const TreeFileInfo& info, void* nodes_d, hipStream_t stream);
+// Reads `bytes` bytes of a raw file into host_out, or else into dev_out.
+Status read_raw(const char* path, size_t bytes, void* host_out, void* dev_out, hipStream_t stream);
+
 }  // namespace pst

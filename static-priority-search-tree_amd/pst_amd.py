@@ -42,7 +42,10 @@ EXPORTED = [
     "pst_gpu_autotune", "pst_gpu_set_subtree_copy", "pst_gpu_batch_graph_create", "pst_gpu_batch_graph_launch", "pst_gpu_batch_graph_destroy",
     "pst_interval_parallel_search",
     "pst_interval_parallel_search_into",
+    "pst_gpu_save", "pst_gpu_load", "pst_read_vertices", "pst_read_vertices_to_device",
 ]
+ERR_IO = 6
+DT_NAME = {v: k for k, v in DT.items()}
 
 
 def point_dtype(t: str, i: Optional[str]) -> np.dtype:
@@ -97,6 +100,10 @@ def lib():
         L.pst_gpu_slots_per_tree.argtypes = [vp]
         L.pst_gpu_autotune.argtypes = [vp, vp, sz, i32, vp, c.POINTER(c.c_int)]
         L.pst_gpu_set_subtree_copy.argtypes = [vp, i32, i32, sz]
+        L.pst_gpu_save.argtypes = [vp, c.c_char_p]
+        L.pst_gpu_load.argtypes = [c.c_char_p, i32, i32, vp, c.POINTER(vp)]
+        L.pst_read_vertices.argtypes = [c.c_char_p, i32, sz, vp]
+        L.pst_read_vertices_to_device.argtypes = [c.c_char_p, i32, sz, vp, vp]
         L.pst_gpu_batch_graph_create.argtypes = [vp, vp, sz, i32, vp, vp, vp, sz, vp, c.POINTER(vp)]
         L.pst_gpu_batch_graph_launch.argtypes = [vp, vp]
         L.pst_gpu_batch_graph_destroy.argtypes = [vp]
@@ -203,6 +210,28 @@ class StaticPSTGPU:
         if getattr(self, "h", None):
             _check(lib().pst_gpu_destroy(self.h))
             self.h = None
+
+    # ---- persistence (no counterpart in the reference) ----
+    def save(self, path: str):
+        """Write the built tree (or forest) to `path` (pst_gpu_save)."""
+        _check(lib().pst_gpu_save(self.h, os.fsencode(path)))
+
+    @staticmethod
+    def load(path: str, dev_ind: int = 0, num_devs: int = 1, stream=None):
+        """A tree or forest from a file written by save(), without its input
+        points and without rebuilding (pst_gpu_load)."""
+        h = ctypes.c_void_p()
+        _check(lib().pst_gpu_load(os.fsencode(path), dev_ind, num_devs, _stream_ptr(stream), ctypes.byref(h)))
+        L = lib()
+        spt = L.pst_gpu_slots_per_tree(h)
+        obj = object.__new__(StaticPSTGPUArr if spt else StaticPSTGPU)
+        obj.h = h
+        obj.t = DT_NAME[L.pst_gpu_dtype(h)]
+        obj.i = DT_NAME[L.pst_gpu_id_dtype(h)]
+        obj.n = L.pst_gpu_num_elems(h)
+        if spt:
+            obj.slots_per_tree = spt
+        return obj
 
     def __del__(self):
         try:
@@ -382,6 +411,21 @@ class StaticPSTGPUArr(StaticPSTGPU):
     @staticmethod
     def calcGlobalMemNeeded(num_elems, threads_per_block, dtype="f32", id_dtype="u32"):
         return lib().pst_gpu_arr_calc_global_mem_needed(num_elems, threads_per_block, DT[dtype], DT[id_dtype])
+
+
+def readInVertices(path: str, num_vertices: int, dtype: str = "f32") -> np.ndarray:
+    """Drop-in for the reference's readInVertices<T> (helpers/isosurface-data-
+    processing.h:205-229): num_vertices values of dtype from a raw file."""
+    out = np.empty(num_vertices, dtype=NP[dtype])
+    _check(lib().pst_read_vertices(os.fsencode(path), DT[dtype], num_vertices, out.ctypes.data))
+    return out
+
+
+def read_vertices_to_device(path: str, num_vertices: int, out_d, dtype: str = "f32", stream=None):
+    """Raw volume file straight into device memory (pst_read_vertices_to_device)."""
+    ptr = out_d.data_ptr() if hasattr(out_d, "data_ptr") else out_d
+    _check(lib().pst_read_vertices_to_device(os.fsencode(path), DT[dtype], num_vertices, ptr,
+                                             _stream_ptr(stream)))
 
 
 def intervalParallelSearch(pt_arr_d, num_elems, search_val, dtype="f32", id_dtype="u32",

@@ -5,12 +5,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <fstream>
 #include <random>
 #include <sstream>
 #include <vector>
 
 #include "interval-parallel-search.h"
+#include "isosurface-data-processing.h"
 #include "pst-point.h"
 #include "static-pst-gpu-arr.h"
 #include "static-pst-gpu.h"
@@ -164,6 +167,94 @@ void run_forest(size_t n, unsigned tpb, unsigned seed) {
   (void)hipFree(d);
 }
 
+// StaticPSTGPU::save / load (beyond the reference): the reloaded tree
+// answers every search as the original does.
+void run_persist(size_t n, unsigned seed) {
+  using Pt = PointStruct<float, unsigned, 1>;
+  std::mt19937 rng(seed);
+  std::uniform_real_distribution<float> u(0.0f, 1.0f);
+  std::vector<Pt> pts(n);
+  for (size_t i = 0; i < n; i++) {
+    const float a = u(rng);
+    pts[i] = Pt(a, a + 0.05f * u(rng), static_cast<unsigned>(i));
+  }
+  Pt* d = nullptr;
+  (void)hipMalloc(&d, n * sizeof(Pt));
+  (void)hipMemcpy(d, pts.data(), n * sizeof(Pt), hipMemcpyHostToDevice);
+  const std::string path = "dropin_persist.pst";
+  {
+    StaticPSTGPU<float, PointStruct, unsigned, 1> tree(d, n);
+    tree.save(path);
+    auto lo = StaticPSTGPU<float, PointStruct, unsigned, 1>::load(path);
+    for (int q = 0; q < 8; q++) {
+      const float a = u(rng), c = u(rng);
+      size_t k1 = 0, k2 = 0;
+      unsigned *r1 = nullptr, *r2 = nullptr;
+      tree.template twoSidedRightSearch<unsigned>(k1, r1, a, c);
+      lo->template twoSidedRightSearch<unsigned>(k2, r2, a, c);
+      std::vector<unsigned> g1 = to_host<float>(r1, k1), g2 = to_host<float>(r2, k2);
+      (void)hipFree(r1);
+      (void)hipFree(r2);
+      std::sort(g1.begin(), g1.end());
+      std::sort(g2.begin(), g2.end());
+      EXPECT(g1 == g2, "StaticPSTGPU::load q=%d", q);
+    }
+    bool threw = false;
+    try {
+      StaticPSTGPUArr<float, PointStruct, unsigned, 1>::load(path);
+    } catch (const std::runtime_error&) {
+      threw = true;
+    }
+    EXPECT(threw, "StaticPSTGPUArr::load rejects a StaticPSTGPU file");
+  }
+  std::remove(path.c_str());
+  (void)hipFree(d);
+}
+
+// readInVertices + calcNumMetacells + formMetacellTags as the reference's
+// dataset tester uses them (testers/pst-tester-functions.tu:97-136), against
+// a brute-force min/max per metacell.
+void run_isosurface() {
+  unsigned pt_dims[NUM_DIMS] = {21, 14, 9}, mc_dims[NUM_DIMS] = {4, 4, 3}, mc_grid[NUM_DIMS];
+  const size_t nv = 21 * 14 * 9;
+  std::vector<float> vol(nv);
+  for (size_t v = 0; v < nv; v++) vol[v] = std::sin(0.37f * v) * 10.0f;
+  const std::string path = "dropin_volume.raw";
+  {
+    std::ofstream f(path, std::ios::binary);
+    f.write(reinterpret_cast<const char*>(vol.data()), nv * sizeof(float));
+  }
+  float* host = readInVertices<float>(path, nv);
+  EXPECT(std::equal(vol.begin(), vol.end(), host), "readInVertices values");
+  float* dv = nullptr;
+  (void)hipMalloc(&dv, nv * sizeof(float));
+  (void)hipMemcpy(dv, host, nv * sizeof(float), hipMemcpyHostToDevice);
+  delete[] host;
+  size_t num_metacells = 0;
+  calcNumMetacells(pt_dims, mc_dims, mc_grid, num_metacells);
+  EXPECT(mc_grid[0] == 5 && mc_grid[1] == 4 && mc_grid[2] == 3 && num_metacells == 60, "calcNumMetacells");
+  using Pt = PointStruct<float, unsigned, 1>;
+  Pt* cells = formMetacellTags<Pt>(dv, pt_dims, mc_dims, mc_grid, num_metacells, 0, 1, 64);
+  std::vector<Pt> got = to_host<float>(cells, num_metacells);
+  for (unsigned c = 0; c < mc_grid[2]; c++)
+    for (unsigned b = 0; b < mc_grid[1]; b++)
+      for (unsigned a = 0; a < mc_grid[0]; a++) {
+        float mn = INFINITY, mx = -INFINITY;
+        for (unsigned z = c * 3; z <= std::min(c * 3 + 3, 8u); z++)
+          for (unsigned y = b * 4; y <= std::min(b * 4 + 4, 13u); y++)
+            for (unsigned x = a * 4; x <= std::min(a * 4 + 4, 20u); x++) {
+              const float v = vol[x + (y + z * 14) * 21];
+              mn = std::min(mn, v);
+              mx = std::max(mx, v);
+            }
+        const unsigned id = a + (b + c * mc_grid[1]) * mc_grid[0];
+        EXPECT(got[id].dim1_val == mn && got[id].dim2_val == mx && got[id].id == id, "metacell %u", id);
+      }
+  (void)hipFree(cells);
+  (void)hipFree(dv);
+  std::remove(path.c_str());
+}
+
 int main() {
   run_with_ids<float>(1000, 1);
   run_with_ids<float>(5000, 2);
@@ -172,6 +263,8 @@ int main() {
   run_no_ids();
   run_forest(10000, 64, 5);
   run_forest(3001, 1000, 6);
+  run_persist(20000, 7);
+  run_isosurface();
   if (failures) {
     std::printf("DROPIN FAILED (%d)\n", failures);
     return 1;
