@@ -39,18 +39,29 @@ def log(*a):
 
 def iso_bench(args, P, D, torch, dev, stream):
     """Isosurface form (StaticPSTGPUArr): a side^3 float32 volume (sum of 64
-    Gaussians + N(0, 0.01) noise, seed 1), 4^3-voxel metacells, one tree per
-    64-thread block, 200 isovalues evenly spaced over [vmin, vmax] (both
-    ends included) as twoSidedLeftSearch(q, q)."""
+    Gaussians + N(0, 0.01) noise, seed 1) or a raw float32 dataset file,
+    4^3-voxel metacells, one tree per 64-thread block, 200 isovalues evenly
+    spaced over [vmin, vmax] (both ends included) as twoSidedLeftSearch(q, q)."""
     import numpy as np
-    side = args.iso_side
-    dims = (side, side, side)
-    prm = torch.from_numpy(P.gaussian_volume_params(64, 1).ravel()).to(dev)
-    vol = torch.empty(side ** 3, dtype=torch.float32, device=dev)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
-    ev[0].record(stream)
-    P.synth_gaussian_volume(vol, dims, prm, 64, 0.01, 1, stream=stream)
-    ev[1].record(stream)
+    if args.volume_file:
+        dims = tuple(args.volume_dims) if args.volume_dims else D.DATASET_DIMS[args.dataset]
+        nv = dims[0] * dims[1] * dims[2]
+        vol = torch.empty(nv, dtype=torch.float32, device=dev)
+        ev[0].record(stream)
+        P.read_vertices_to_device(args.volume_file, nv, vol, "f32", stream=stream)
+        ev[1].record(stream)
+        source = f"{os.path.basename(args.volume_file)} ({dims[0]}x{dims[1]}x{dims[2]} f32 raw file)"
+    else:
+        side = args.iso_side
+        dims = (side, side, side)
+        nv = side ** 3
+        prm = torch.from_numpy(P.gaussian_volume_params(64, 1).ravel()).to(dev)
+        vol = torch.empty(nv, dtype=torch.float32, device=dev)
+        ev[0].record(stream)
+        P.synth_gaussian_volume(vol, dims, prm, 64, 0.01, 1, stream=stream)
+        ev[1].record(stream)
+        source = f"{side}^3 f32 volume (64 Gaussians + N(0, 0.01) noise, seed 1)"
     nmc, g = P.calcNumMetacells(dims, (4, 4, 4))
     rec = P.point_dtype("f32", "u32")
     mc = torch.empty(nmc * rec.itemsize, dtype=torch.uint8, device=dev)
@@ -113,16 +124,16 @@ def iso_bench(args, P, D, torch, dev, stream):
     ips_ms = e0.elapsed_time(e1) / len(sel)
     forest.close()
     mc_ms = ev[2].elapsed_time(ev[3])
-    mc_bytes = side ** 3 * 4 + nmc * rec.itemsize
-    log(f"[bench] iso: volume {ev[0].elapsed_time(ev[1]):.1f} ms, metacells {mc_ms:.2f} ms "
+    mc_bytes = nv * 4 + nmc * rec.itemsize
+    log(f"[bench] iso: volume {ev[0].elapsed_time(ev[1]):.1f} ms ({source}), metacells {mc_ms:.2f} ms "
         f"({mc_bytes / (mc_ms * 1e-3) / 1e9:.0f} GB/s), forest {forest_ms:.1f} ms, "
         f"{nq} isovalues batched {batch_ms:.2f} ms ({total} IDs), plain {plain_qps:.0f} q/s, "
         f"IPS {1e3 / ips_ms:.0f} q/s")
     return {
-        "workload": f"{side}^3 f32 volume (64 Gaussians + N(0, 0.01) noise, seed 1), 4^3-voxel metacells "
+        "workload": f"{source}, 4^3-voxel metacells "
                     f"({nmc} intervals), StaticPSTGPUArr with 64 threads per block, {nq} isovalues over "
                     "[vmin, vmax] as twoSidedLeftSearch(q, q)",
-        "volume_synthesis_ms": round(ev[0].elapsed_time(ev[1]), 3),
+        "volume_ms": round(ev[0].elapsed_time(ev[1]), 3),
         "metacell_ms": round(mc_ms, 3),
         "metacell_GBps": round(mc_bytes / (mc_ms * 1e-3) / 1e9, 1),
         "forest_build_ms": round(forest_ms, 2),
@@ -306,6 +317,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=512, help="queries in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--iso-side", type=int, default=1024, help="isosurface volume side (0: skip)")
+    ap.add_argument("--volume-file", default="", help="raw float32 volume for the isosurface leg "
+                    "(default: the synthetic volume)")
+    ap.add_argument("--dataset", default="Vortex", choices=sorted(["Isabel", "Radiation", "TeraShake", "Vortex"]),
+                    help="dataset whose grid dimensions the volume file has")
+    ap.add_argument("--volume-dims", type=int, nargs=3, default=None, help="grid dimensions of the volume file")
     ap.add_argument("--iso-queries", type=int, default=200)
     ap.add_argument("--iso-plain", type=int, default=50, help="isovalues timed on the plain path and IPS")
     ap.add_argument("--int-log2n", type=int, default=27, help="int32 batch workload size (0: skip)")
@@ -539,7 +555,7 @@ def main():
     tree.close()
     del d_pts, out
     torch.cuda.empty_cache()
-    iso = iso_bench(args, P, D, torch, dev, stream) if args.iso_side > 0 else None
+    iso = iso_bench(args, P, D, torch, dev, stream) if (args.iso_side > 0 or args.volume_file) else None
     torch.cuda.empty_cache()
     intb = int_batch_bench(args, P, D, torch, dev) if args.int_log2n > 0 else None
     torch.cuda.empty_cache()

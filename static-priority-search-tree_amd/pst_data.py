@@ -14,6 +14,17 @@ import numpy as np
 M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
 
 
+# Vertex grid dimensions (x, y, z) of the reference's isosurface datasets
+# (testers/pst-dataset-tester-driver.cu:48-52); the raw files hold float32
+# values, x fastest, and are read with readInVertices / read_vertices_to_device.
+DATASET_DIMS = {
+    "Isabel": (500, 500, 100),
+    "Radiation": (600, 248, 248),
+    "TeraShake": (750, 375, 100),
+    "Vortex": (128, 128, 128),
+}
+
+
 def splitmix64(x: np.ndarray) -> np.ndarray:
     with np.errstate(over="ignore"):
         z = x + np.uint64(0x9E3779B97F4A7C15)
