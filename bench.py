@@ -71,6 +71,7 @@ def iso_bench(args, P, D, torch, dev, stream):
     ev[3].record(stream)
     torch.cuda.synchronize()
     vmin, vmax = float(vol.min().item()), float(vol.max().item())
+    vol_keep = vol  # the isosurface extraction below reads the volume again
     del vol
     t0 = time.perf_counter()
     forest = P.StaticPSTGPUArr(mc.data_ptr(), nmc, 64, "f32", "u32")
@@ -122,13 +123,36 @@ def iso_bench(args, P, D, torch, dev, stream):
     e1.record(stream)
     torch.cuda.synchronize()
     ips_ms = e0.elapsed_time(e1) / len(sel)
+    # isosurface of the active metacells of the middle isovalue (marching
+    # tetrahedra): count, scan, write
+    k_mid = sel[len(sel) // 2]
+    q_mid = iso[k_mid]
+    nact = forest.search_into(1, 0, q_mid, q_mid, pout, cap)
+    ids_d = pout[:nact]
+    mcd = (4, 4, 4)
+    icounts = torch.zeros(max(nact, 1), dtype=torch.int64, device=dev)
+    ioffs = torch.zeros(max(nact, 1) + 1, dtype=torch.int64, device=dev)
+    P.isosurface_count(vol_keep, dims, mcd, ids_d, nact, q_mid, icounts, stream=stream)
+    torch.cuda.synchronize()
+    ntri = int(icounts[:nact].sum().item())
+    tri = torch.empty(max(ntri, 1) * 9, dtype=torch.float32, device=dev)
+    ie = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ie[0].record(stream)
+    P.isosurface_count(vol_keep, dims, mcd, ids_d, nact, q_mid, icounts, stream=stream)
+    ioffs[1:nact + 1] = torch.cumsum(icounts[:nact], 0)
+    P.isosurface_write(vol_keep, dims, mcd, ids_d, nact, q_mid, ioffs, tri, ntri, stream=stream)
+    ie[1].record(stream)
+    torch.cuda.synchronize()
+    iso_ms = ie[0].elapsed_time(ie[1])
+    ok &= int(ioffs[nact].item()) == ntri
+    del vol_keep
     forest.close()
     mc_ms = ev[2].elapsed_time(ev[3])
     mc_bytes = nv * 4 + nmc * rec.itemsize
     log(f"[bench] iso: volume {ev[0].elapsed_time(ev[1]):.1f} ms ({source}), metacells {mc_ms:.2f} ms "
         f"({mc_bytes / (mc_ms * 1e-3) / 1e9:.0f} GB/s), forest {forest_ms:.1f} ms, "
         f"{nq} isovalues batched {batch_ms:.2f} ms ({total} IDs), plain {plain_qps:.0f} q/s, "
-        f"IPS {1e3 / ips_ms:.0f} q/s")
+        f"IPS {1e3 / ips_ms:.0f} q/s, isosurface of {nact} active metacells {iso_ms:.2f} ms ({ntri} triangles)")
     return {
         "workload": f"{source}, 4^3-voxel metacells "
                     f"({nmc} intervals), StaticPSTGPUArr with 64 threads per block, {nq} isovalues over "
@@ -142,6 +166,8 @@ def iso_bench(args, P, D, torch, dev, stream):
         "reported_ids": total,
         "plain_qps": round(plain_qps, 1),
         "ips_qps": round(1e3 / ips_ms, 1),
+        "isosurface": {"isovalue": float(q_mid), "active_metacells": nact, "triangles": ntri,
+                       "ms": round(iso_ms, 3), "method": "marching tetrahedra, count + scan + write"},
         "results_checked": bool(ok),
     }
 

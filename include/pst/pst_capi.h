@@ -237,6 +237,37 @@ int pst_read_vertices(const char* path, int dtype, size_t num_vertices, void* ve
 int pst_read_vertices_to_device(const char* path, int dtype, size_t num_vertices, void* vertex_arr_d,
                                 pst_stream_t stream);
 
+/* Isosurface of the active metacells (no counterpart in the reference,
+ * which stops at the metacell IDs of an isovalue search and copies them back
+ * "for on-device marching cubes", testers/pst-tester-functions.tu:230-252).
+ * Marching tetrahedra: each voxel of each listed metacell is cut into the
+ * 6 tetrahedra {0, e_a, e_a + e_b, 7}; a vertex is below when its value is
+ * < isovalue; surface vertices interpolate linearly along grid edges from
+ * the endpoint of lower linear index (shared edges give identical vertices,
+ * so the mesh has no cracks); triangles are wound with their normal toward
+ * values >= isovalue. Triangles are 9 floats (x, y, z of 3 vertices) in grid
+ * coordinates. The metacell grid is that of pst_calc_num_metacells;
+ * metacell_ids_d holds IDs of an ID type of 4 or 8 bytes (the search
+ * results), isovalue points to one value of dtype.
+ *   pst_isosurface_count: counts_d[i] = triangles of metacell ids[i]
+ *                          (an ID past the grid counts 0), asynchronous;
+ *   pst_isosurface_write: metacell ids[i]'s triangles at triangles_d +
+ *                          9 * offsets_d[i], at most `capacity` triangles in
+ *                          all (whole voxels), asynchronous;
+ *   pst_isosurface:       both, blocking, into an array allocated by the
+ *                          call (free with pst_free); PST_ERR_INVALID for an
+ *                          ID past the grid. */
+int pst_isosurface_count(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3],
+                         const size_t metacell_dims[3], const void* metacell_ids_d, int id_dtype, size_t num_ids,
+                         const void* isovalue, unsigned long long* counts_d, pst_stream_t stream);
+int pst_isosurface_write(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3],
+                         const size_t metacell_dims[3], const void* metacell_ids_d, int id_dtype, size_t num_ids,
+                         const void* isovalue, const unsigned long long* offsets_d, float* triangles_d,
+                         size_t capacity, pst_stream_t stream);
+int pst_isosurface(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3], const size_t metacell_dims[3],
+                   const void* metacell_ids_d, int id_dtype, size_t num_ids, const void* isovalue,
+                   float** triangles_d, size_t* num_triangles);
+
 /* Synthetic f32 volume (no counterpart in the reference; a seeded input of
  * the isosurface workload): value = sum_g amp_g * exp(-|p - c_g|^2 /
  * (2 s_g^2)) + noise_std * N(0, 1) at p = (i, j, k) / (dims - 1), x fastest.

@@ -805,6 +805,95 @@ int pst_form_metacells_into(const void* vertex_arr_d, int dtype, int id_dtype, c
   return st.ok() ? PST_OK : fail(st);
 }
 
+namespace {
+int iso_args(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3], const size_t metacell_dims[3],
+             const void* ids_d, int id_dtype, size_t num_ids, const void* isovalue, uint32_t p32[3],
+             uint32_t m32[3]) {
+  if (!vertex_arr_d || !pt_grid_dims || !metacell_dims || !isovalue || (num_ids && !ids_d))
+    return fail(PST_ERR_INVALID, "Error: null argument");
+  if (!valid_t(dtype) || id_dtype == PST_NO_ID || !valid_id(id_dtype))
+    return fail(PST_ERR_INVALID, "Error: unsupported type codes");
+  return metacell_dims_u32(pt_grid_dims, metacell_dims, p32, m32);
+}
+}  // namespace
+
+int pst_isosurface_count(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3],
+                         const size_t metacell_dims[3], const void* metacell_ids_d, int id_dtype, size_t num_ids,
+                         const void* isovalue, unsigned long long* counts_d, pst_stream_t stream) {
+  uint32_t p32[3], m32[3];
+  int rc = iso_args(vertex_arr_d, dtype, pt_grid_dims, metacell_dims, metacell_ids_d, id_dtype, num_ids, isovalue,
+                    p32, m32);
+  if (rc != PST_OK) return rc;
+  if (num_ids && !counts_d) return fail(PST_ERR_INVALID, "Error: null argument");
+  int dev = 0;
+  hipGetDevice(&dev);
+  unsigned long long* err = device_counter(dev);
+  if (!err) return fail(PST_ERR_HIP, "Error in allocating a device counter");
+  Status st = pst::iso_count(dtype, id_dtype, vertex_arr_d, p32, m32, metacell_ids_d, num_ids, isovalue, counts_d,
+                             err, reinterpret_cast<hipStream_t>(stream));
+  return st.ok() ? PST_OK : fail(st);
+}
+
+int pst_isosurface_write(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3],
+                         const size_t metacell_dims[3], const void* metacell_ids_d, int id_dtype, size_t num_ids,
+                         const void* isovalue, const unsigned long long* offsets_d, float* triangles_d,
+                         size_t capacity, pst_stream_t stream) {
+  uint32_t p32[3], m32[3];
+  int rc = iso_args(vertex_arr_d, dtype, pt_grid_dims, metacell_dims, metacell_ids_d, id_dtype, num_ids, isovalue,
+                    p32, m32);
+  if (rc != PST_OK) return rc;
+  if (num_ids && (!offsets_d || (capacity && !triangles_d))) return fail(PST_ERR_INVALID, "Error: null argument");
+  Status st = pst::iso_write(dtype, id_dtype, vertex_arr_d, p32, m32, metacell_ids_d, num_ids, isovalue, offsets_d,
+                             capacity, triangles_d, reinterpret_cast<hipStream_t>(stream));
+  return st.ok() ? PST_OK : fail(st);
+}
+
+int pst_isosurface(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3], const size_t metacell_dims[3],
+                   const void* metacell_ids_d, int id_dtype, size_t num_ids, const void* isovalue,
+                   float** triangles_d, size_t* num_triangles) {
+  if (!triangles_d || !num_triangles) return fail(PST_ERR_INVALID, "Error: null argument");
+  *triangles_d = nullptr;
+  *num_triangles = 0;
+  uint32_t p32[3], m32[3];
+  int rc = iso_args(vertex_arr_d, dtype, pt_grid_dims, metacell_dims, metacell_ids_d, id_dtype, num_ids, isovalue,
+                    p32, m32);
+  if (rc != PST_OK || num_ids == 0) return rc;
+  int dev = 0;
+  hipGetDevice(&dev);
+  unsigned long long* err = device_counter(dev);
+  if (!err) return fail(PST_ERR_HIP, "Error in allocating a device counter");
+  unsigned long long* counts = nullptr;  // counts, then offsets (num_ids + 1)
+  hipError_t e = hipMalloc(&counts, (2 * num_ids + 1) * sizeof(unsigned long long));
+  if (e != hipSuccess) return fail(PST_ERR_HIP, std::string("Error in allocating isosurface counts: ") + hipGetErrorString(e));
+  unsigned long long* offsets = counts + num_ids;
+  Status st = pst::hip_status(hipMemsetAsync(err, 0, 8, nullptr), "Error in clearing the error flag");
+  if (st.ok())
+    st = pst::iso_count(dtype, id_dtype, vertex_arr_d, p32, m32, metacell_ids_d, num_ids, isovalue, counts, err,
+                        nullptr);
+  if (st.ok()) st = pst::exclusive_scan(counts, offsets, num_ids, nullptr);
+  unsigned long long total = 0, bad = 0;
+  if (st.ok())
+    st = pst::hip_status(hipMemcpy(&total, offsets + num_ids, 8, hipMemcpyDeviceToHost),
+                         "Error in reading the triangle count");
+  if (st.ok()) st = pst::hip_status(hipMemcpy(&bad, err, 8, hipMemcpyDeviceToHost), "Error in reading the error flag");
+  if (st.ok() && bad) st = Status{PST_ERR_INVALID, "Error: a metacell ID lies past the metacell grid"};
+  float* tri = nullptr;
+  if (st.ok() && total)
+    st = pst::hip_status(hipMalloc(&tri, total * 9 * sizeof(float)), "Error in allocating the triangle array");
+  if (st.ok() && total)
+    st = pst::iso_write(dtype, id_dtype, vertex_arr_d, p32, m32, metacell_ids_d, num_ids, isovalue, offsets, total,
+                        tri, nullptr);
+  if (st.ok()) st = pst::hip_status(hipStreamSynchronize(nullptr), "Error in synchronising after isosurface extraction");
+  hipFree(counts);
+  if (!st.ok()) {
+    if (tri) hipFree(tri);
+    return fail(st);
+  }
+  *triangles_d = tri;
+  *num_triangles = total;
+  return PST_OK;
+}
+
 int pst_form_metacell_tags(const void* vertex_arr_d, int dtype, int id_dtype, const size_t pt_grid_dims[3],
                            const size_t metacell_dims[3], void** metacell_arr_d, size_t* num_metacells, int dev_ind,
                            int num_devs) {

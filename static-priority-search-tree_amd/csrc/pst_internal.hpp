@@ -137,6 +137,18 @@ Status gauss_volume(float* out_d, const uint32_t dims[3], const float* params_d,
 // helpers
 Status hip_status(hipError_t e, const std::string& what);
 
+// ---- isosurface of active metacells (pst_isosurface.hip) ----
+// counts_d[i]: triangles of metacell ids_d[i]; bit 0 of *err_d is set for an
+// ID past the metacell grid (counted as 0).
+Status iso_count(int dtype, int id_dtype, const void* vertices_d, const uint32_t pt_dims[3],
+                 const uint32_t mc_dims[3], const void* ids_d, size_t nids, const void* isovalue,
+                 unsigned long long* counts_d, unsigned long long* err_d, hipStream_t stream);
+// Triangles of metacell ids_d[i] at tri_d + 9 offsets_d[i] (floats); voxels
+// whose triangles would pass `capacity` triangles are skipped.
+Status iso_write(int dtype, int id_dtype, const void* vertices_d, const uint32_t pt_dims[3],
+                 const uint32_t mc_dims[3], const void* ids_d, size_t nids, const void* isovalue,
+                 const unsigned long long* offsets_d, size_t capacity, float* tri_d, hipStream_t stream);
+
 // ---- tree files and raw volumes (pst_persist.hip) ----
 struct TreeFileInfo {
   int kind;  // 0: one deep tree, 1: forest

@@ -43,6 +43,7 @@ EXPORTED = [
     "pst_interval_parallel_search",
     "pst_interval_parallel_search_into",
     "pst_gpu_save", "pst_gpu_load", "pst_read_vertices", "pst_read_vertices_to_device",
+    "pst_isosurface_count", "pst_isosurface_write", "pst_isosurface",
 ]
 ERR_IO = 6
 DT_NAME = {v: k for k, v in DT.items()}
@@ -120,6 +121,9 @@ def lib():
         L.pst_server_outstanding.argtypes = [vp]
         L.pst_server_last_error.restype = c.c_char_p
         dims3 = c.c_size_t * 3
+        L.pst_isosurface_count.argtypes = [vp, i32, dims3, dims3, vp, i32, sz, vp, vp, vp]
+        L.pst_isosurface_write.argtypes = [vp, i32, dims3, dims3, vp, i32, sz, vp, vp, vp, sz, vp]
+        L.pst_isosurface.argtypes = [vp, i32, dims3, dims3, vp, i32, sz, vp, c.POINTER(vp), c.POINTER(sz)]
         L.pst_calc_num_metacells.restype = sz
         L.pst_calc_num_metacells.argtypes = [dims3, dims3, dims3]
         L.pst_form_metacell_tags.argtypes = [vp, i32, i32, dims3, dims3, c.POINTER(vp), c.POINTER(sz), i32, i32]
@@ -426,6 +430,44 @@ def read_vertices_to_device(path: str, num_vertices: int, out_d, dtype: str = "f
     ptr = out_d.data_ptr() if hasattr(out_d, "data_ptr") else out_d
     _check(lib().pst_read_vertices_to_device(os.fsencode(path), DT[dtype], num_vertices, ptr,
                                              _stream_ptr(stream)))
+
+
+def isosurface(vertex_arr_d, pt_grid_dims, metacell_dims, ids_d, num_ids, isovalue, dtype="f32",
+               id_dtype="u32") -> np.ndarray:
+    """Triangles (n, 3, 3) of the isosurface in the listed metacells
+    (pst_isosurface: marching tetrahedra, grid coordinates)."""
+    vptr = vertex_arr_d.data_ptr() if hasattr(vertex_arr_d, "data_ptr") else vertex_arr_d
+    iptr = ids_d.data_ptr() if hasattr(ids_d, "data_ptr") else ids_d
+    dims3 = ctypes.c_size_t * 3
+    q = _scalar(isovalue, dtype)
+    p, n = ctypes.c_void_p(), ctypes.c_size_t()
+    _check(lib().pst_isosurface(vptr, DT[dtype], dims3(*pt_grid_dims), dims3(*metacell_dims), iptr, DT[id_dtype],
+                                num_ids, q.ctypes.data, ctypes.byref(p), ctypes.byref(n)))
+    if not p.value:
+        return np.zeros((0, 3, 3), np.float32)
+    raw = _d2h(p.value, n.value * 36)
+    _check(lib().pst_free(p.value))
+    return raw.view(np.float32).reshape(-1, 3, 3)
+
+
+def isosurface_count(vertex_arr_d, pt_grid_dims, metacell_dims, ids_d, num_ids, isovalue, counts_d,
+                     dtype="f32", id_dtype="u32", stream=None):
+    """Triangles per listed metacell (pst_isosurface_count), asynchronous."""
+    dims3 = ctypes.c_size_t * 3
+    q = _scalar(isovalue, dtype)
+    _check(lib().pst_isosurface_count(vertex_arr_d.data_ptr(), DT[dtype], dims3(*pt_grid_dims), dims3(*metacell_dims),
+                                      ids_d.data_ptr(), DT[id_dtype], num_ids, q.ctypes.data, counts_d.data_ptr(),
+                                      _stream_ptr(stream)))
+
+
+def isosurface_write(vertex_arr_d, pt_grid_dims, metacell_dims, ids_d, num_ids, isovalue, offsets_d, tri_d,
+                     capacity, dtype="f32", id_dtype="u32", stream=None):
+    """Triangles of the listed metacells at their offsets (pst_isosurface_write)."""
+    dims3 = ctypes.c_size_t * 3
+    q = _scalar(isovalue, dtype)
+    _check(lib().pst_isosurface_write(vertex_arr_d.data_ptr(), DT[dtype], dims3(*pt_grid_dims), dims3(*metacell_dims),
+                                      ids_d.data_ptr(), DT[id_dtype], num_ids, q.ctypes.data, offsets_d.data_ptr(),
+                                      tri_d.data_ptr(), capacity, _stream_ptr(stream)))
 
 
 def intervalParallelSearch(pt_arr_d, num_elems, search_val, dtype="f32", id_dtype="u32",
