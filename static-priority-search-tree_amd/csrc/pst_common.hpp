@@ -211,10 +211,28 @@ struct StepOut {
 //   RIGHT_SEARCH -> med >= lo ? (left RIGHT, right REPORT_ABOVE) : right RIGHT;
 //   THREE_SEARCH -> lo <= med <= hi ? (left RIGHT, right LEFT)
 //                   : hi < med ? left THREE : med < lo ? right THREE : none.
+// Bit j set when query j's lower (upper) dim1 bound is a number. A node's
+// med is never NaN (construction rejects NaN points), so for such a bound
+// md < lo is the complement of md >= lo and hi < md that of md <= hi; a NaN
+// bound fails every comparison.
+struct BoundNum {
+  uint32_t lo, hi;
+};
 template <typename T, int G>
-__host__ __device__ inline StepOut node_step(T d1, T d2, T md, int32_t m, const Bounds<T>* qb, uint32_t a,
-                                             uint32_t l, uint32_t h) {
-  uint32_t ge2 = 0, d1lo = 0, d1hi = 0, mdhi = 0, mdlo = 0, himd = 0, mdltlo = 0;
+__host__ __device__ inline BoundNum bound_num(const Bounds<T>* qb) {
+  BoundNum b{0, 0};
+#pragma unroll
+  for (int j = 0; j < G; j++) {
+    b.lo |= (qb[j].lo == qb[j].lo) ? (1u << j) : 0u;
+    b.hi |= (qb[j].hi == qb[j].hi) ? (1u << j) : 0u;
+  }
+  return b;
+}
+
+template <typename T, int G>
+__host__ __device__ inline StepOut node_step(T d1, T d2, T md, int32_t m, const Bounds<T>* qb, BoundNum bn,
+                                             uint32_t a, uint32_t l, uint32_t h) {
+  uint32_t ge2 = 0, d1lo = 0, d1hi = 0, mdhi = 0, mdlo = 0;
 #pragma unroll
   for (int j = 0; j < G; j++) {
     const uint32_t bit = 1u << j;
@@ -223,9 +241,8 @@ __host__ __device__ inline StepOut node_step(T d1, T d2, T md, int32_t m, const 
     d1hi |= (d1 <= qb[j].hi) ? bit : 0u;
     mdhi |= (md <= qb[j].hi) ? bit : 0u;
     mdlo |= (md >= qb[j].lo) ? bit : 0u;
-    himd |= (qb[j].hi < md) ? bit : 0u;
-    mdltlo |= (md < qb[j].lo) ? bit : 0u;
   }
+  const uint32_t himd = ~mdhi & bn.hi, mdltlo = ~mdlo & bn.lo;
   const uint32_t act = a & ge2;
   const uint32_t ra = act & ~l & ~h, lf = act & h & ~l, rt = act & l & ~h, th = act & l & h;
   const uint32_t split = th & mdlo & mdhi;

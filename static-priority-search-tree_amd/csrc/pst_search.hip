@@ -459,6 +459,9 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
       qb[j].hi = read_lane(qv.max_dim1, j);
       qb[j].min2 = read_lane(qv.min_dim2, j);
     }
+    BoundNum bn;  // queries whose dim1 bounds are numbers (lane j holds query j)
+    bn.lo = static_cast<uint32_t>(__ballot(lane < G && qv.min_dim1 == qv.min_dim1));
+    bn.hi = static_cast<uint32_t>(__ballot(lane < G && qv.max_dim1 == qv.max_dim1));
     uint32_t found[G];  // results of this batch (count mode; a batch reports < 2^32 per query)
     uint32_t nst[G];    // staged results (write mode)
 #pragma unroll
@@ -628,7 +631,7 @@ __global__ __launch_bounds__(kWave) void k_search_round(Params<T, IS> P, uint32_
       if (__ballot(open) == 0) {
         if (ok) so = node_step_report_above<T, G>(nd.dim2, m, qb, ia);
       } else if (ok) {
-        so = node_step<T, G>(nd.dim1, nd.dim2, nd.med, m, qb, ia, il, ih);
+        so = node_step<T, G>(nd.dim1, nd.dim2, nd.med, m, qb, bn, ia, il, ih);
       }
       // ---- results, per query of the group
 #pragma unroll
@@ -981,7 +984,9 @@ Status launch_rounds(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
   const uint32_t bound = 2 * kWave * (H + 2) + 4 * kWave;
   P.gcap = ((bound + kMoveBlock - 1) / kMoveBlock) * kMoveBlock;
   P.budget_small = a.budget_small > 0 ? a.budget_small : 6;
-  P.budget_large = a.budget_large > 0 ? a.budget_large : 384;
+  // a write step costs more than a count step (staging, flushes, copy
+  // tasks), so the write pass hands work on after fewer steps
+  P.budget_large = a.budget_large > 0 ? a.budget_large : (W ? 192 : 384);
   auto kern = k_search_round<T, IS, RP, W, G>;
   int dev = 0;
   hipGetDevice(&dev);

@@ -166,7 +166,8 @@ std::vector<std::vector<long long>> group_walk(const Tree<T, I>& t, const std::v
         if (it.a == 0) continue;
       }
       const pst::StepOut o =
-          pst::node_step<T, G>(D1[it.slot], D2[it.slot], MD[it.slot], m, qb, it.a, it.l, it.h);
+          pst::node_step<T, G>(D1[it.slot], D2[it.slot], MD[it.slot], m, qb, pst::bound_num<T, G>(qb), it.a, it.l,
+                               it.h);
       if (it.l == 0 && it.h == 0) {
         // the report-above shortcut must agree with the full transition
         const pst::StepOut f = pst::node_step_report_above<T, G>(D2[it.slot], m, qb, it.a);
