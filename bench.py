@@ -388,14 +388,24 @@ def main():
     torch.cuda.synchronize()
 
     # ---- construction
+    # the first build of a process also loads the construction kernels and
+    # maps fresh memory; later builds show the steady construction time
     log("[bench] building tree")
     t0 = time.perf_counter()
     tree = P.StaticPSTGPU(d_pts, n, "f32", "u32")
     torch.cuda.synchronize()
-    build_ms = (time.perf_counter() - t0) * 1e3
+    cold_build_ms = (time.perf_counter() - t0) * 1e3
+    warm = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        again = P.StaticPSTGPU(d_pts, n, "f32", "u32")
+        torch.cuda.synchronize()
+        warm.append((time.perf_counter() - t0) * 1e3)
+        again.close()
+    build_ms = float(np.median(warm))
     S = (1 << (args.log2n + 1)) - 1
     build_bytes = n * 12 + S * 17
-    log(f"[bench] build {build_ms:.1f} ms")
+    log(f"[bench] build {build_ms:.1f} ms (median of 3; first build of the process {cold_build_ms:.1f} ms)")
 
     # ---- queries
     qs = D.float_query_mix(args.nq3, args.nq2, seed=0)
@@ -625,6 +635,7 @@ def main():
         "ips_qps": round(1e3 / ips_ms, 2),
         "ips_GBps": round(float(ips_gbps), 1),
         "build_ms": round(build_ms, 2),
+        "first_build_ms": round(cold_build_ms, 2),
         "build_GBps_io_floor": round(build_bytes / (build_ms * 1e-3) / 1e9, 1),
         "results_checked": bool(ok and (iso is None or iso["results_checked"])
                                 and (intb is None or intb["results_checked"])),

@@ -8,11 +8,16 @@
 // (helpers/linearise-id.h:11-19), with g = ceil((p-1)/m) metacells per axis
 // (calcNumMetacells, isosurface-data-processing.h:21-36).
 //
-// Here one workgroup takes one (b, c) row of metacells: it streams the
-// row's (my+1)(mz+1) vertex rows once with coalesced loads, reducing every
-// vertex column in registers, then reduces each metacell's mx+1 columns from
-// LDS. The volume is read about (1 + 1/my)(1 + 1/mz) times (shared boundary
-// planes); the bound is HBM bandwidth.
+// Here a workgroup owns a block of vertex columns (a thread per column), one
+// z slab of metacells (mz + 1 vertex planes) and a run of metacell rows, and
+// sweeps down y: for each vertex row it reduces the slab's planes in
+// registers, folds the row into the current metacell row, and carries the
+// boundary row (the last of one metacell row, the first of the next) instead
+// of reading it again; at the end of each metacell row it reduces every
+// metacell's mx + 1 columns from LDS. The volume is read about 1 + 1/mz
+// times (the shared boundary planes of neighbouring slabs); the bound is HBM
+// bandwidth. Metacells of 256 or more voxels along x use k_metacells, one
+// workgroup per metacell row with an LDS column tile.
 #include <cfloat>
 #include <climits>
 
@@ -75,6 +80,100 @@ __global__ __launch_bounds__(kMcBlock) void k_metacells(const T* __restrict__ vo
       Point<T, I> r;
       r.dim1_val = mn;
       r.dim2_val = mxv;
+      if constexpr (has_id_v<I>) r.id = static_cast<I>(id);
+      out[id] = r;
+    }
+    __syncthreads();
+  }
+}
+
+constexpr int kSweepBlock = 256;
+constexpr uint32_t kSweepRows = 16;  // metacell rows per workgroup
+constexpr uint32_t kRows = 4;        // vertex rows loaded together
+constexpr uint32_t kPlanes = 5;      // vertex planes loaded together
+
+template <typename T, typename I>
+__global__ __launch_bounds__(kSweepBlock) void k_metacells_sweep(const T* __restrict__ vol, uint32_t px, uint32_t py,
+                                                                 uint32_t pz, uint32_t mx, uint32_t my, uint32_t mz,
+                                                                 uint32_t gx, uint32_t gy,
+                                                                 Point<T, I>* __restrict__ out) {
+  __shared__ T cmin[kSweepBlock];
+  __shared__ T cmax[kSweepBlock];
+  const uint32_t cpb = (kSweepBlock - 1) / mx;  // metacells per block along x
+  const uint32_t a0 = blockIdx.x * cpb;
+  const uint32_t c = blockIdx.y;
+  const uint32_t b0 = blockIdx.z * kSweepRows, b1 = min(gy, b0 + kSweepRows);
+  const uint32_t tid = threadIdx.x;
+  const uint32_t x = a0 * mx + tid;
+  const uint32_t xlast = min(min(gx, a0 + cpb) * mx, px - 1);  // last vertex column of the block
+  const bool col = x <= xlast;
+  const uint32_t z0 = c * mz, z1 = min((c + 1) * mz, pz - 1);
+  const size_t plane = static_cast<size_t>(py) * px;
+  const uint32_t nz = z1 - z0;  // planes of the slab, less one
+  const T* colp = vol + x + static_cast<size_t>(z0) * plane;
+  // min and max over the slab's planes of vertex rows y..y+kRows-1 (rows
+  // past `ylast` and planes past the slab repeat the last one: min and max
+  // do not change), every load issued before the first use
+  auto rows = [&](uint32_t y, uint32_t ylast, T lo[kRows], T hi[kRows]) {
+    for (uint32_t zb = 0; zb <= nz; zb += kPlanes) {
+      T v[kRows][kPlanes];
+#pragma unroll
+      for (uint32_t r = 0; r < kRows; r++)
+#pragma unroll
+        for (uint32_t k = 0; k < kPlanes; k++)
+          v[r][k] = colp[static_cast<size_t>(min(y + r, ylast)) * px + static_cast<size_t>(min(zb + k, nz)) * plane];
+#pragma unroll
+      for (uint32_t r = 0; r < kRows; r++)
+#pragma unroll
+        for (uint32_t k = 0; k < kPlanes; k++) {
+          lo[r] = (zb == 0 && k == 0) ? v[r][k] : vmin(lo[r], v[r][k]);
+          hi[r] = (zb == 0 && k == 0) ? v[r][k] : vmax(hi[r], v[r][k]);
+        }
+    }
+  };
+  T clo{}, chi{};  // the carried boundary row
+  bool carry = false;
+  for (uint32_t b = b0; b < b1; b++) {
+    const uint32_t y0 = b * my, y1 = min((b + 1) * my, py - 1);
+    T mn{}, mxv{};
+    if (col) {
+      if (carry) {
+        mn = clo;
+        mxv = chi;
+      } else {
+        T lo[kRows], hi[kRows];
+        rows(y0, y0, lo, hi);
+        mn = lo[0];
+        mxv = hi[0];
+      }
+      for (uint32_t y = y0 + 1; y <= y1; y += kRows) {
+        T lo[kRows], hi[kRows];
+        rows(y, y1, lo, hi);
+#pragma unroll
+        for (uint32_t r = 0; r < kRows; r++) {
+          mn = vmin(mn, lo[r]);
+          mxv = vmax(mxv, hi[r]);
+        }
+        clo = lo[kRows - 1];  // row min(y + kRows - 1, y1): y1 on the last pass
+        chi = hi[kRows - 1];
+      }
+      cmin[tid] = mn;
+      cmax[tid] = mxv;
+    }
+    carry = true;  // row y1 starts the next metacell row
+    __syncthreads();
+    const uint32_t a = a0 + tid;
+    if (tid < cpb && a < gx) {
+      const uint32_t xs = tid * mx, xe = min((a + 1) * mx, px - 1) - a0 * mx;
+      T lo = cmin[xs], hi = cmax[xs];
+      for (uint32_t k = xs + 1; k <= xe; k++) {
+        lo = vmin(lo, cmin[k]);
+        hi = vmax(hi, cmax[k]);
+      }
+      const uint64_t id = a + (static_cast<uint64_t>(b) + static_cast<uint64_t>(c) * gy) * gx;
+      Point<T, I> r;
+      r.dim1_val = lo;
+      r.dim2_val = hi;
       if constexpr (has_id_v<I>) r.id = static_cast<I>(id);
       out[id] = r;
     }
@@ -172,9 +271,17 @@ Status form_metacells(int dtype, int id_dtype, const void* vertices_d, const uin
     return with_id(id_dtype, [&](auto i) {
       using T = decltype(t);
       using I = decltype(i);
-      k_metacells<T, I><<<dim3(g[1], g[2]), kMcBlock, 0, stream>>>(
-          static_cast<const T*>(vertices_d), pt_dims[0], pt_dims[1], pt_dims[2], mc_dims[0], mc_dims[1], mc_dims[2],
-          g[0], g[1], static_cast<Point<T, I>*>(out_pts_d));
+      if (mc_dims[0] < static_cast<uint32_t>(kSweepBlock)) {
+        const uint32_t cpb = (kSweepBlock - 1) / mc_dims[0];
+        const dim3 grid((g[0] + cpb - 1) / cpb, g[2], (g[1] + kSweepRows - 1) / kSweepRows);
+        k_metacells_sweep<T, I><<<grid, kSweepBlock, 0, stream>>>(
+            static_cast<const T*>(vertices_d), pt_dims[0], pt_dims[1], pt_dims[2], mc_dims[0], mc_dims[1],
+            mc_dims[2], g[0], g[1], static_cast<Point<T, I>*>(out_pts_d));
+      } else {
+        k_metacells<T, I><<<dim3(g[1], g[2]), kMcBlock, 0, stream>>>(
+            static_cast<const T*>(vertices_d), pt_dims[0], pt_dims[1], pt_dims[2], mc_dims[0], mc_dims[1],
+            mc_dims[2], g[0], g[1], static_cast<Point<T, I>*>(out_pts_d));
+      }
       return hip_status(hipGetLastError(), "Error in launching metacell formation");
     });
   });
