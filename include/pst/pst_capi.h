@@ -252,8 +252,9 @@ int pst_read_vertices_to_device(const char* path, int dtype, size_t num_vertices
  *   pst_isosurface_count: counts_d[i] = triangles of metacell ids[i]
  *                          (an ID past the grid counts 0), asynchronous;
  *   pst_isosurface_write: metacell ids[i]'s triangles at triangles_d +
- *                          9 * offsets_d[i], at most `capacity` triangles in
- *                          all (whole voxels), asynchronous;
+ *                          9 * offsets_d[i] (in the order: 64-voxel chunk,
+ *                          tetrahedron, voxel), triangles at or past index
+ *                          `capacity` not written, asynchronous;
  *   pst_isosurface:       both, blocking, into an array allocated by the
  *                          call (free with pst_free); PST_ERR_INVALID for an
  *                          ID past the grid. */

@@ -143,8 +143,8 @@ Status hip_status(hipError_t e, const std::string& what);
 Status iso_count(int dtype, int id_dtype, const void* vertices_d, const uint32_t pt_dims[3],
                  const uint32_t mc_dims[3], const void* ids_d, size_t nids, const void* isovalue,
                  unsigned long long* counts_d, unsigned long long* err_d, hipStream_t stream);
-// Triangles of metacell ids_d[i] at tri_d + 9 offsets_d[i] (floats); voxels
-// whose triangles would pass `capacity` triangles are skipped.
+// Triangles of metacell ids_d[i] at tri_d + 9 offsets_d[i] (floats);
+// triangles at or past index `capacity` are not written.
 Status iso_write(int dtype, int id_dtype, const void* vertices_d, const uint32_t pt_dims[3],
                  const uint32_t mc_dims[3], const void* ids_d, size_t nids, const void* isovalue,
                  const unsigned long long* offsets_d, size_t capacity, float* tri_d, hipStream_t stream);

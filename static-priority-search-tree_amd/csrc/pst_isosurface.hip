@@ -19,9 +19,10 @@
 // x in [a mx, min((a + 1) mx, px - 1)) (likewise in y and z); its ID is
 // a + (b + c gy) gx, as formMetacellTags numbers them. Count pass: one wave
 // per metacell, a lane per voxel, the wave's total per metacell. Write pass:
-// the same walk, each lane's triangles at its wave-prefix offset after the
-// metacell's offset. Positions are grid coordinates (x, y, z) as float, 9
-// floats per triangle.
+// the same walk; for each tetrahedron index the lanes' triangles are staged
+// in LDS and stored as one contiguous run after the metacell's offset
+// (order: 64-voxel chunk, tetrahedron, voxel). Positions are grid
+// coordinates (x, y, z) as float, 9 floats per triangle.
 #include <cstring>
 
 #include "pst_internal.hpp"
@@ -107,34 +108,38 @@ __device__ inline void put_tri(float* __restrict__ out, float3 a, float3 b, floa
   out[6] = c.x, out[7] = c.y, out[8] = c.z;
 }
 
+// Triangles of tetrahedron t of voxel v (0, 1 or 2), written at `out`.
 template <typename T>
-__device__ inline void voxel_write(const Voxel& v, const T f[8], T q, float* __restrict__ out) {
-  int w = 0;
+__device__ inline void tet_write(const Voxel& v, const T f[8], T q, int t, float* out) {
+  int below[4], above[4], nb = 0, na = 0;
 #pragma unroll
-  for (int t = 0; t < 6; t++) {
-    int below[4], above[4], nb = 0, na = 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int c = kTets[t][k];
-      if (f[c] < q) below[nb++] = c;
-      else above[na++] = c;
-    }
-    if (nb == 0 || na == 0) continue;
-    // a direction from below to above q across this tetrahedron
-    const float3 pb = corner_pos(v, below[0]), pa = corner_pos(v, above[0]);
-    const float3 up = make_float3(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z);
-    if (nb == 1 || na == 1) {
-      const int s = nb == 1 ? below[0] : above[0];
-      const int* o = nb == 1 ? above : below;
-      put_tri(out + 9 * w++, edge_point(v, f, s, o[0], q), edge_point(v, f, s, o[1], q),
-              edge_point(v, f, s, o[2], q), up);
-    } else {
-      const float3 p0 = edge_point(v, f, below[0], above[0], q), p1 = edge_point(v, f, below[0], above[1], q);
-      const float3 p2 = edge_point(v, f, below[1], above[1], q), p3 = edge_point(v, f, below[1], above[0], q);
-      put_tri(out + 9 * w++, p0, p1, p2, up);
-      put_tri(out + 9 * w++, p0, p2, p3, up);
-    }
+  for (int k = 0; k < 4; k++) {
+    const int c = kTets[t][k];
+    if (f[c] < q) below[nb++] = c;
+    else above[na++] = c;
   }
+  if (nb == 0 || na == 0) return;
+  // a direction from below to above q across this tetrahedron
+  const float3 pb = corner_pos(v, below[0]), pa = corner_pos(v, above[0]);
+  const float3 up = make_float3(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z);
+  if (nb == 1 || na == 1) {
+    const int s = nb == 1 ? below[0] : above[0];
+    const int* o = nb == 1 ? above : below;
+    put_tri(out, edge_point(v, f, s, o[0], q), edge_point(v, f, s, o[1], q), edge_point(v, f, s, o[2], q), up);
+  } else {
+    const float3 p0 = edge_point(v, f, below[0], above[0], q), p1 = edge_point(v, f, below[0], above[1], q);
+    const float3 p2 = edge_point(v, f, below[1], above[1], q), p3 = edge_point(v, f, below[1], above[0], q);
+    put_tri(out, p0, p1, p2, up);
+    put_tri(out + 9, p0, p2, p3, up);
+  }
+}
+
+template <typename T>
+__device__ inline int tet_count(const T f[8], T q, int t) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) m |= (f[kTets[t][k]] < q ? 1u : 0u) << k;
+  return tet_tris(m);
 }
 
 struct IsoGrid {
@@ -186,35 +191,46 @@ __global__ __launch_bounds__(kIsoWave) void k_iso_count(const T* __restrict__ vo
   }
 }
 
+// One wave per metacell; per chunk of 64 voxels and per tetrahedron index,
+// the lanes' triangles (at most 128) are staged in LDS at their prefix
+// positions and written out as one contiguous run of floats, so the stores
+// are coalesced. Order within a metacell: voxel chunk, tetrahedron, voxel.
 template <typename T, typename I>
 __global__ __launch_bounds__(kIsoWave) void k_iso_write(const T* __restrict__ vol, IsoGrid g,
                                                          const I* __restrict__ ids, uint64_t nids, T q,
                                                          const unsigned long long* __restrict__ offsets,
                                                          unsigned long long capacity, float* __restrict__ out) {
+  __shared__ float stage[2 * kIsoWave * 9];
   const uint32_t lane = threadIdx.x;
   const uint64_t ncells = static_cast<uint64_t>(g.gx) * g.gy * g.gz;
   for (uint64_t i = blockIdx.x; i < nids; i += gridDim.x) {
     const uint64_t id = static_cast<uint64_t>(ids[i]);
     if (id >= ncells) continue;
-    unsigned long long base = offsets[i];
+    unsigned long long base = offsets[i];  // triangles written so far, as an output index
     uint32_t nvox = 1;
     for (uint32_t j0 = 0; j0 < nvox; j0 += kIsoWave) {
       Voxel v;
       T f[8];
-      uint32_t n = 0;
       const bool have = metacell_voxel(g, id, j0 + lane, v, nvox);
-      if (have) {
-        load_corners(vol, g.px, g.py, v, f);
-        n = static_cast<uint32_t>(voxel_count(f, q));
+      if (have) load_corners(vol, g.px, g.py, v, f);
+      for (int t = 0; t < 6; t++) {
+        const uint32_t n = have ? static_cast<uint32_t>(tet_count(f, q, t)) : 0u;
+        uint32_t incl = n;
+        for (int o = 1; o < kIsoWave; o <<= 1) {
+          const uint32_t u = __shfl_up(incl, o, kIsoWave);
+          if (static_cast<int>(lane) >= o) incl += u;
+        }
+        const uint32_t total = __shfl(incl, kIsoWave - 1, kIsoWave);
+        if (total == 0) continue;
+        if (n) tet_write(v, f, q, t, stage + 9 * (incl - n));
+        __syncthreads();
+        for (uint32_t k = lane; k < 9 * total; k += kIsoWave) {
+          const unsigned long long tri = base + k / 9;
+          if (tri < capacity) out[9 * base + k] = stage[k];
+        }
+        __syncthreads();
+        base += total;
       }
-      uint32_t incl = n;
-      for (int o = 1; o < kIsoWave; o <<= 1) {
-        const uint32_t t = __shfl_up(incl, o, kIsoWave);
-        if (static_cast<int>(lane) >= o) incl += t;
-      }
-      const unsigned long long at = base + (incl - n);
-      if (have && n > 0 && at + n <= capacity) voxel_write(v, f, q, out + 9 * at);
-      base += __shfl(incl, kIsoWave - 1, kIsoWave);
     }
   }
 }

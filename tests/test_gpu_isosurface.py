@@ -35,8 +35,9 @@ def P():
 
 
 def ref_metacell(vol, mc, cell, q):
-    """Triangles of one metacell, in the order the kernel writes them
-    (voxels x fastest, tetrahedra in TETS order)."""
+    """Triangles of one metacell, in the order the kernel writes them: chunks
+    of 64 voxels (voxels x fastest), within a chunk tetrahedron by
+    tetrahedron (TETS order), within that voxel by voxel."""
     pz, py, px = vol.shape
     mx, my, mz = mc
     gx = -(-(px - 1) // mx)
@@ -44,10 +45,13 @@ def ref_metacell(vol, mc, cell, q):
     a, b, c = cell % gx, (cell // gx) % gy, cell // (gx * gy)
     acc = np.float32 if vol.dtype == np.float32 else np.float64
     qa = acc(q)
+    voxels = [(x, y, z) for z in range(c * mz, min((c + 1) * mz, pz - 1))
+              for y in range(b * my, min((b + 1) * my, py - 1))
+              for x in range(a * mx, min((a + 1) * mx, px - 1))]
     out = []
-    for z in range(c * mz, min((c + 1) * mz, pz - 1)):
-        for y in range(b * my, min((b + 1) * my, py - 1)):
-            for x in range(a * mx, min((a + 1) * mx, px - 1)):
+    for j0 in range(0, len(voxels), 64):
+        for tet in TETS:
+            for (x, y, z) in voxels[j0:j0 + 64]:
                 f = [vol[z + (k >> 2), y + ((k >> 1) & 1), x + (k & 1)] for k in range(8)]
                 pos = [np.array([x + (k & 1), y + ((k >> 1) & 1), z + (k >> 2)], np.float32) for k in range(8)]
 
@@ -56,25 +60,24 @@ def ref_metacell(vol, mc, cell, q):
                     t = (qa - acc(f[u])) / (acc(f[w]) - acc(f[u]))
                     return pos[u] + np.float32(t) * (pos[w] - pos[u])
 
-                for tet in TETS:
-                    below = [k for k in tet if f[k] < q]
-                    above = [k for k in tet if not f[k] < q]
-                    if not below or not above:
-                        continue
-                    up = pos[above[0]] - pos[below[0]]
-                    if len(below) == 1 or len(above) == 1:
-                        s = below[0] if len(below) == 1 else above[0]
-                        o = above if len(below) == 1 else below
-                        tris = [(edge(s, o[0]), edge(s, o[1]), edge(s, o[2]))]
-                    else:
-                        p0, p1 = edge(below[0], above[0]), edge(below[0], above[1])
-                        p2, p3 = edge(below[1], above[1]), edge(below[1], above[0])
-                        tris = [(p0, p1, p2), (p0, p2, p3)]
-                    for ta, tb, tc in tris:
-                        n = np.cross(tb - ta, tc - ta)
-                        if float(np.dot(n, up)) < 0:
-                            tb, tc = tc, tb
-                        out.append(np.array((ta, tb, tc), np.float32))
+                below = [k for k in tet if f[k] < q]
+                above = [k for k in tet if not f[k] < q]
+                if not below or not above:
+                    continue
+                up = pos[above[0]] - pos[below[0]]
+                if len(below) == 1 or len(above) == 1:
+                    s = below[0] if len(below) == 1 else above[0]
+                    o = above if len(below) == 1 else below
+                    tris = [(edge(s, o[0]), edge(s, o[1]), edge(s, o[2]))]
+                else:
+                    p0, p1 = edge(below[0], above[0]), edge(below[0], above[1])
+                    p2, p3 = edge(below[1], above[1]), edge(below[1], above[0])
+                    tris = [(p0, p1, p2), (p0, p2, p3)]
+                for ta, tb, tc in tris:
+                    n = np.cross(tb - ta, tc - ta)
+                    if float(np.dot(n, up)) < 0:
+                        tb, tc = tc, tb
+                    out.append(np.array((ta, tb, tc), np.float32))
     return out
 
 
