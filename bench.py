@@ -29,8 +29,10 @@ sys.path.insert(0, PKG)
 
 HBM_PEAK_GBPS = 8000.0
 BYTES_PER_ID = 21  # 3 x f32 (dim1, dim2, med) + u32 id + 1 B bitcode read, u32 written
-# kernel launches of one write pass: four walk rounds and the subtree copy kernel
+# kernel launches of one write pass of the walk: four walk rounds and the subtree copy kernel
 WRITE_LAUNCHES = 5
+# level index write pass: each reported u32 ID is read once from a sorted run and written once
+COPY_BYTES_PER_ID = 8
 
 
 def log(*a):
@@ -373,6 +375,8 @@ def main():
                          "(the library groups queries itself)")
     ap.add_argument("--ungrouped", type=int, default=1,
                     help="also time the batch with one walk per query (grouping off)")
+    ap.add_argument("--level-index", type=int, default=1,
+                    help="1: the headline batch runs through the level index (the walk is timed beside it)")
     ap.add_argument("--profile-mode", action="store_true",
                     help="build + one batched step + a 1 GiB copy (for counter scaling), no JSON")
     args = ap.parse_args()
@@ -449,6 +453,8 @@ def main():
         ok &= want == int(host_counts[k])
 
     if args.profile_mode:
+        if args.level_index:
+            tree.set_level_index(True)
         tree.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
         torch.cuda.synchronize()
         # plain copy of a known size: the scale for HBM byte counters
@@ -460,47 +466,57 @@ def main():
         tree.close()
         return
 
-    # ---- timed steps: batched count + scan + write
-    for _ in range(args.warmup):
-        tree.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        tree.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ok &= int(offsets[nq].item()) == total
-    ms_per_step = elapsed * 1e3 / args.steps
-    qps = nq * args.steps / elapsed
-    ids_per_s = total * args.steps / elapsed
-
-    # ---- per-pass timing with events on the launch stream (roofline figure)
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-    t_count = t_write = 0.0
-    reps = max(1, args.steps)
-    for _ in range(reps):
-        ev[0].record(stream)
-        tree.batch_count(d_q, nq, counts, stream=stream)
-        ev[1].record(stream)
-        tree.batch_write(d_q, nq, offsets, out, caps_d=counts, stream=stream)
-        ev[2].record(stream)
+    def timed_steps():
+        for _ in range(args.warmup):
+            tree.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
         torch.cuda.synchronize()
-        t_count += ev[0].elapsed_time(ev[1])
-        t_write += ev[1].elapsed_time(ev[2])
-    t_count /= reps
-    t_write /= reps
-    alg_bytes = total * BYTES_PER_ID
-    achieved = alg_bytes / (t_write * 1e-3) / 1e9
-    # HBM bytes per launch of the same kernel on this workload, from the
-    # committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated)
-    traffic, traffic_src = None, None
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            tree.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    def pass_times():
+        # per-pass timing with events on the launch stream (roofline figure)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        tc = tw = 0.0
+        reps = max(1, args.steps)
+        for _ in range(reps):
+            ev[0].record(stream)
+            tree.batch_count(d_q, nq, counts, stream=stream)
+            ev[1].record(stream)
+            tree.batch_write(d_q, nq, offsets, out, caps_d=counts, stream=stream)
+            ev[2].record(stream)
+            torch.cuda.synchronize()
+            tc += ev[0].elapsed_time(ev[1])
+            tw += ev[1].elapsed_time(ev[2])
+        return tc / reps, tw / reps
+
+    def sample_results(ks):
+        c, o = counts.cpu().numpy(), offsets.cpu().numpy()
+        return {k: np.sort(out[o[k]:o[k] + c[k]].cpu().numpy()) for k in ks}
+
+    # ---- timed steps on the tree walk: batched count + scan + write
+    elapsed = timed_steps()
+    ok &= int(offsets[nq].item()) == total
+    walk_ms = elapsed * 1e3 / args.steps
+    w_count, w_write = pass_times()
+    sample_ks = sorted({0, 1, args.nq2 // 2, args.nq2 - 1, args.nq2, args.nq2 + args.nq3 // 2, nq - 1})
+    walk_ref = sample_results(sample_ks)
+    log(f"[bench] walk: {walk_ms:.2f} ms per batch ({nq / (walk_ms * 1e-3):.0f} q/s), count pass {w_count:.2f} ms, "
+        f"write pass {w_write:.2f} ms")
+    # HBM bytes per launch of the walk's write pass on this workload, from
+    # the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated)
+    walk_traffic = None
     tpath = os.path.join(ROOT, "profiles", "write_pass_k_search_round_k_copy_tasks_f32u32_2e26_batch2048.traffic.json")
     if args.log2n == 26 and args.nq2 == 1024 and args.nq3 == 1024 and os.path.exists(tpath):
         with open(tpath) as f:
-            tj = json.load(f)
-        traffic, traffic_src = round(tj["traffic_bytes_per_launch"]), tj["source"]
-    log(f"[bench] count pass {t_count:.2f} ms, write pass {t_write:.2f} ms, "
-        f"write-pass alg {achieved:.0f} GB/s")
+            walk_traffic = round(json.load(f)["traffic_bytes_per_launch"])
+    walk = {"ms_per_step": round(walk_ms, 3), "qps": round(nq / (walk_ms * 1e-3), 2),
+            "count_pass_ms": round(w_count, 3), "write_pass_ms": round(w_write, 3),
+            "write_pass_alg_GBps": round(total * BYTES_PER_ID / (w_write * 1e-3) / 1e9, 1),
+            "write_pass_traffic_bytes_per_launch": walk_traffic,
+            "write_launches": WRITE_LAUNCHES}
 
     # ---- the same batch with one walk per query (grouping off), for comparison
     ungrouped_qps = None
@@ -515,6 +531,57 @@ def main():
         ok &= int(offsets[nq].item()) == total
         tree.set_grouping(8)
         log(f"[bench] ungrouped batch {ungrouped_qps:.0f} q/s")
+
+    # ---- level index: build it, then the same batch through it
+    index = None
+    if args.level_index:
+        t0 = time.perf_counter()
+        tree.set_level_index(True)
+        torch.cuda.synchronize()
+        first_index_ms = (time.perf_counter() - t0) * 1e3
+        iw = []
+        for _ in range(2):
+            tree.set_level_index(False)
+            t0 = time.perf_counter()
+            tree.set_level_index(True)
+            torch.cuda.synchronize()
+            iw.append((time.perf_counter() - t0) * 1e3)
+        out.fill_(-1)
+        elapsed = timed_steps()
+        ok &= int(offsets[nq].item()) == total
+        got = sample_results(sample_ks)
+        same = all(np.array_equal(got[k], walk_ref[k]) for k in sample_ks)
+        ok &= same
+        t_count, t_write = pass_times()
+        index = {"build_ms": round(min(iw), 2), "first_build_ms": round(first_index_ms, 2),
+                 "bytes": P.StaticPSTGPU.level_index_bytes(n, "f32", "u32"),
+                 "same_results_as_walk": bool(same)}
+        log(f"[bench] level index: build {min(iw):.1f} ms (first {first_index_ms:.1f} ms), "
+            f"{index['bytes'] / 2**30:.1f} GiB, results as the walk's: {same}")
+    else:
+        t_count, t_write = w_count, w_write
+    ms_per_step = elapsed * 1e3 / args.steps
+    qps = nq * args.steps / elapsed
+    ids_per_s = total * args.steps / elapsed
+    if args.level_index:
+        alg_bytes = total * COPY_BYTES_PER_ID
+        launches = 1
+        kernel = ("write pass: k_lv_plan<float, unsigned int, true> + offset scan + k_lv_mark + "
+                  "k_lv_copy<unsigned int> (the copy streams every ID; the pass time is its upper bound)")
+        tpath = os.path.join(ROOT, "profiles", "write_pass_k_lv_copy_f32u32_2e26_batch2048.traffic.json")
+    else:
+        alg_bytes = total * BYTES_PER_ID
+        launches = WRITE_LAUNCHES
+        kernel = ("write pass: k_search_round<float, unsigned int, false, true, 8> x 4 "
+                  "+ k_copy_tasks<float, unsigned int, false, 8> x 1")
+    achieved = alg_bytes / (t_write * 1e-3) / 1e9
+    traffic, traffic_src = None, None
+    if args.log2n == 26 and args.nq2 == 1024 and args.nq3 == 1024 and os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        traffic, traffic_src = round(tj["traffic_bytes_per_launch"]), tj["source"]
+    log(f"[bench] count pass {t_count:.2f} ms, write pass {t_write:.2f} ms, "
+        f"write-pass alg {achieved:.0f} GB/s")
 
     # ---- plain path: one blocking search per query, preallocated output
     np_plain = min(args.plain, nq)
@@ -627,15 +694,18 @@ def main():
         },
         "reported_ids_per_s": round(ids_per_s, 1),
         "alg_GBps_step": round(ids_per_s * BYTES_PER_ID / 1e9, 1),
+        "path": "level index" if args.level_index else "tree walk",
         "count_pass_ms": round(t_count, 3),
         "write_pass_ms": round(t_write, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                     "kernel": "write pass: k_search_round<float, unsigned int, false, true, 8> x 4 "
-                               "+ k_copy_tasks<float, unsigned int, false, 8> x 1",
-                     "launches_per_pass": WRITE_LAUNCHES, "avg_launch_ms": round(t_write / WRITE_LAUNCHES, 3),
-                     "alg_bytes_per_launch": round(alg_bytes / WRITE_LAUNCHES),
+                     "kernel": kernel, "launches_per_pass": launches,
+                     "avg_launch_ms": round(t_write / launches, 3),
+                     "alg_bytes_per_launch": round(alg_bytes / launches),
+                     "alg_bytes_per_id": COPY_BYTES_PER_ID if args.level_index else BYTES_PER_ID,
                      "traffic_source": traffic_src},
+        "level_index": index,
+        "walk": walk,
         "cpu_baseline": cpu,
         "plain_per_query_qps": round(plain_qps, 2),
         "batched_over_plain": round(qps / plain_qps, 2),

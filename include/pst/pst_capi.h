@@ -337,6 +337,22 @@ int pst_gpu_set_subtree_copy(pst_tree* tree, int copy_min, int copy_max, size_t 
  * each query's results exactly those of its own walk) or 1 (one walk per
  * query). No counterpart in the reference. */
 int pst_gpu_set_query_grouping(pst_tree* tree, int queries_per_walk);
+/* Level index of a deep tree with IDs (no counterpart in the reference):
+ * on = 1 builds it (blocking; a no-op when built), on = 0 frees it. For
+ * every depth t it holds the points of each subtree rooted at depth t
+ * sorted by dim2 descending, so a subtree whose dim1 range lies inside a
+ * query's bounds reports a prefix of its sorted run: one binary search and
+ * one contiguous copy. While it exists, pst_gpu_batch_count and the ID
+ * returning pst_gpu_batch_write / pst_gpu_batch_search (and the batch
+ * graphs made from them) walk only the query's boundary paths and stream
+ * the IDs; results are the same sets as without it (order within a
+ * query's results differs). Trees of fewer than 2^31 points; not for a
+ * forest or a tree without IDs (PST_ERR_INVALID). Device memory:
+ * pst_gpu_level_index_bytes. */
+int pst_gpu_set_level_index(pst_tree* tree, int on);
+/* Bytes of the level index for n points (bit_length(n) levels of n
+ * key/ID pairs); 0 for n = 0 or a tree without IDs. */
+size_t pst_gpu_level_index_bytes(size_t n, int dtype, int id_dtype);
 
 /* ------------------------------------------------------------------ */
 /* Query server (no counterpart in the reference)                      */

@@ -27,6 +27,9 @@ struct pst_tree {
   int group = pst::kGroup;  // queries per walk in batched searches
   int copy_min = 256, copy_max = 65535;  // write pass: full subtrees copied in this size range
   size_t copy_tasks = 0;                 // copy task list entries (0: library default)
+  void* lv_key = nullptr;  // level index (deep trees with IDs, on request): sorted dim2 keys per level
+  void* lv_id = nullptr;   // and the IDs in the same order
+  pst::LevelScratch lsc;
   hipStream_t stream = nullptr;
   unsigned long long* debug = nullptr;  // per-round search counters (debug builds)
 };
@@ -297,6 +300,9 @@ int pst_gpu_destroy(pst_tree* t) {
     if (e == hipSuccess) e = e2;
   }
   pst::scratch_release(t->sc);
+  if (t->lv_key) hipFree(t->lv_key);
+  if (t->lv_id) hipFree(t->lv_id);
+  pst::level_scratch_release(t->lsc);
   const int di = t->dev_ind, nd = t->num_devs;
   delete t;
   if (e != hipSuccess)
@@ -544,7 +550,7 @@ int pst_gpu_batch_count(pst_tree* t, const void* queries_d, size_t nq, unsigned 
   a.mode = 0;
   a.ret_point = 0;
   a.counts = counts_d;
-  Status st = pst::run_search(a, t->sc, s);
+  Status st = t->lv_key ? pst::level_search(a, t->lv_key, t->lv_id, t->lsc, s) : pst::run_search(a, t->sc, s);
   return st.ok() ? PST_OK : fail(st);
 }
 
@@ -568,7 +574,8 @@ int pst_gpu_batch_write(pst_tree* t, const void* queries_d, size_t nq, int ret, 
   a.caps = caps_d;
   a.cap_all = ~0ull;
   a.out = out_d;
-  st = pst::run_search(a, t->sc, s);
+  st = (t->lv_key && ret != PST_RET_POINT) ? pst::level_search(a, t->lv_key, t->lv_id, t->lsc, s)
+                                           : pst::run_search(a, t->sc, s);
   return st.ok() ? PST_OK : fail(st);
 }
 
@@ -598,7 +605,11 @@ int pst_gpu_batch_search(pst_tree* t, const void* queries_d, size_t nq, int ret,
   a.cap_all = ~0ull;
   a.out = out_d;
   a.out_total = out_capacity;
-  a.groups_ready = 1;  // the count pass above grouped this batch
+  if (t->lv_key && ret != PST_RET_POINT) {
+    st = pst::level_search(a, t->lv_key, t->lv_id, t->lsc, s);
+    return st.ok() ? PST_OK : fail(st);
+  }
+  a.groups_ready = !t->lv_key;  // the count pass above grouped this batch (unless the index counted it)
   st = pst::run_search(a, t->sc, s);
   return st.ok() ? PST_OK : fail(st);
 }
@@ -758,6 +769,43 @@ int pst_gpu_set_subtree_copy(pst_tree* t, int copy_min, int copy_max, size_t max
   t->copy_min = copy_min;
   t->copy_max = copy_max;
   t->copy_tasks = max_tasks;
+  return PST_OK;
+}
+
+size_t pst_gpu_level_index_bytes(size_t n, int dtype, int id_dtype) {
+  if (!valid_t(dtype) || id_dtype == PST_NO_ID || !valid_id(id_dtype)) return 0;
+  return pst::level_index_bytes(dtype, id_dtype, n);
+}
+
+int pst_gpu_set_level_index(pst_tree* t, int on) {
+  if (!t) return fail(PST_ERR_INVALID, "Error: null tree");
+  if (!on) {
+    hipSetDevice(t->device);
+    if (t->lv_key) hipFree(t->lv_key);
+    if (t->lv_id) hipFree(t->lv_id);
+    t->lv_key = t->lv_id = nullptr;
+    pst::level_scratch_release(t->lsc);
+    return PST_OK;
+  }
+  if (t->slots_per_tree) return fail(PST_ERR_INVALID, "Error: the level index is for one deep tree, not a forest");
+  if (t->id_dtype == PST_NO_ID) return fail(PST_ERR_INVALID, "Error: the level index needs a tree with IDs");
+  if (t->lv_key || t->n == 0) return PST_OK;
+  if (t->n >= (1ull << 31)) return fail(PST_ERR_INVALID, "Error: the level index holds trees of fewer than 2^31 points");
+  hipSetDevice(t->device);
+  const size_t levels = pst::level_index_bytes(t->dtype, t->id_dtype, t->n) /
+                        (t->n * (pst::dtype_size(t->dtype) + id_size(t->id_dtype)));
+  void *k = nullptr, *i = nullptr;
+  hipError_t e = hipMalloc(&k, levels * t->n * pst::dtype_size(t->dtype));
+  if (e == hipSuccess) e = hipMalloc(&i, levels * t->n * id_size(t->id_dtype));
+  Status st = pst::hip_status(e, "Error in allocating the level index" + dev_suffix(t->dev_ind, t->num_devs));
+  if (st.ok()) st = pst::build_level_index(t->dtype, t->id_dtype, t->n, t->nodes, t->ids, k, i, t->stream);
+  if (!st.ok()) {
+    if (k) hipFree(k);
+    if (i) hipFree(i);
+    return fail(st);
+  }
+  t->lv_key = k;
+  t->lv_id = i;
   return PST_OK;
 }
 

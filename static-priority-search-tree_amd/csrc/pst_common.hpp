@@ -194,6 +194,13 @@ struct Bounds {
   T lo, hi, min2;
 };
 
+// Initial state bits of a query of the given kind (codes other than 0..2
+// walk as right searches).
+__host__ __device__ inline uint32_t lo_open(uint32_t kind) { return kind != KIND_LEFT ? 1u : 0u; }
+__host__ __device__ inline uint32_t hi_open(uint32_t kind) {
+  return (kind == KIND_THREE || kind == KIND_LEFT) ? 1u : 0u;
+}
+
 struct StepOut {
   uint32_t rep;         // queries that report the node
   uint32_t al, ll, hl;  // state of the left child (al == 0: not visited)

@@ -119,6 +119,28 @@ unsigned long long* search_error_flag(SearchScratch& sc);
 // offsets[q] = sum of in[0..q), offsets[n] = total (one workgroup).
 Status exclusive_scan(const unsigned long long* in, unsigned long long* offsets, size_t n, hipStream_t s);
 
+// ---- level index of a deep tree (pst_levels.hip) ----
+// Device bytes of the index: bit_length(n) levels of n (key, ID) pairs.
+size_t level_index_bytes(int dtype, int id_dtype, size_t n);
+// Builds the index of a deep tree with IDs (ids_d: ID per slot) into key_d
+// (unsigned keys of the tree's dtype width) and id_d (blocking).
+Status build_level_index(int dtype, int id_dtype, size_t n, const void* nodes_d, const void* ids_d, void* key_d,
+                         void* id_d, hipStream_t stream);
+struct LevelScratch {
+  uint64_t* tsrc = nullptr;   // per query, 128 ranges: first index in the level arrays
+  uint32_t* tpre = nullptr;   // per query, 128: output position of each range, then the total
+  uint32_t* ntask = nullptr;  // ranges per query
+  unsigned long long* wq = nullptr;    // IDs written per query
+  unsigned long long* woff = nullptr;  // their exclusive prefix (nq + 1)
+  uint32_t* chunk_task = nullptr;      // copy chunk -> first range
+  size_t cap = 0;                      // queries the buffers hold
+};
+Status level_scratch_reserve(LevelScratch& sc, size_t nq);
+void level_scratch_release(LevelScratch& sc);
+// Batched count (a.mode 0) or ID write (a.mode 1) with the semantics of
+// run_search, through the level index (asynchronous).
+Status level_search(const SearchArgs& a, const void* key_d, const void* id_d, LevelScratch& sc, hipStream_t stream);
+
 // ---- interval parallel search (pst_ips.hip) ----
 // counter_d must be zero on entry; results past `cap` are counted only.
 Status ips_search(int dtype, int id_dtype, const void* pts_d, size_t n, int ret_point,

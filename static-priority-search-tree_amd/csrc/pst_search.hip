@@ -154,12 +154,6 @@ __device__ inline V read_lane(V v, int j) {
   }
 }
 
-// Initial state bits of a query of the given kind (codes other than 0..2
-// walk as right searches).
-__host__ __device__ inline uint32_t lo_open(uint32_t kind) { return kind != KIND_LEFT ? 1u : 0u; }
-__host__ __device__ inline uint32_t hi_open(uint32_t kind) {
-  return (kind == KIND_THREE || kind == KIND_LEFT) ? 1u : 0u;
-}
 
 template <typename T, int G>
 __global__ void k_search_init(const Query<T>* __restrict__ queries, Query<T> single, const uint32_t* gq,

@@ -43,6 +43,7 @@ EXPORTED = [
     "pst_interval_parallel_search",
     "pst_interval_parallel_search_into",
     "pst_gpu_save", "pst_gpu_load", "pst_read_vertices", "pst_read_vertices_to_device",
+    "pst_gpu_set_level_index", "pst_gpu_level_index_bytes",
     "pst_isosurface_count", "pst_isosurface_write", "pst_isosurface",
 ]
 ERR_IO = 6
@@ -145,6 +146,9 @@ def lib():
         L.pst_gpu_batch_search.argtypes = [vp, vp, sz, i32, vp, vp, vp, sz, vp]
         L.pst_gpu_set_search_tuning.argtypes = [vp, i32, i32, i32, i32]
         L.pst_gpu_set_query_grouping.argtypes = [vp, i32]
+        L.pst_gpu_set_level_index.argtypes = [vp, i32]
+        L.pst_gpu_level_index_bytes.restype = sz
+        L.pst_gpu_level_index_bytes.argtypes = [sz, i32, i32]
         L.pst_interval_parallel_search.argtypes = [vp, sz, i32, i32, i32, c.POINTER(vp),
                                                    c.POINTER(sz), vp, i32, i32, i32,
                                                    c.c_uint, c.c_uint]
@@ -346,6 +350,16 @@ class StaticPSTGPU:
         (copy_max 0: off) through a task list of max_tasks entries (0: default);
         subtrees found beyond it are copied by the walk."""
         _check(lib().pst_gpu_set_subtree_copy(self.h, copy_min, copy_max, max_tasks))
+
+    def set_level_index(self, on: bool = True):
+        """Build (on) or free (off) the level index: per depth, each subtree's
+        points sorted by dim2 descending. While built, batched counts and ID
+        writes walk only the boundary paths and copy sorted runs."""
+        _check(lib().pst_gpu_set_level_index(self.h, 1 if on else 0))
+
+    @staticmethod
+    def level_index_bytes(num_elems, dtype="f32", id_dtype="u32"):
+        return lib().pst_gpu_level_index_bytes(num_elems, DT[dtype], DT[id_dtype])
 
     def set_grouping(self, queries_per_walk: int):
         """Queries per tree walk in batched searches: 8 (default) or 1."""
