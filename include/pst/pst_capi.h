@@ -350,8 +350,9 @@ int pst_gpu_set_query_grouping(pst_tree* tree, int queries_per_walk);
  * forest or a tree without IDs (PST_ERR_INVALID). Device memory:
  * pst_gpu_level_index_bytes. */
 int pst_gpu_set_level_index(pst_tree* tree, int on);
-/* Bytes of the level index for n points (bit_length(n) levels of n
- * key/ID pairs); 0 for n = 0 or a tree without IDs. */
+/* Bytes of the level index for n points (bit_length(n) levels of n key/ID
+ * pairs, each level padded to a multiple of 64); 0 for n = 0 or a tree
+ * without IDs. */
 size_t pst_gpu_level_index_bytes(size_t n, int dtype, int id_dtype);
 
 /* ------------------------------------------------------------------ */

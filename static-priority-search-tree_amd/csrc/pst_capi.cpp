@@ -792,11 +792,11 @@ int pst_gpu_set_level_index(pst_tree* t, int on) {
   if (t->lv_key || t->n == 0) return PST_OK;
   if (t->n >= (1ull << 31)) return fail(PST_ERR_INVALID, "Error: the level index holds trees of fewer than 2^31 points");
   hipSetDevice(t->device);
-  const size_t levels = pst::level_index_bytes(t->dtype, t->id_dtype, t->n) /
-                        (t->n * (pst::dtype_size(t->dtype) + id_size(t->id_dtype)));
+  const size_t elems = pst::level_index_bytes(t->dtype, t->id_dtype, t->n) /
+                       (pst::dtype_size(t->dtype) + id_size(t->id_dtype));
   void *k = nullptr, *i = nullptr;
-  hipError_t e = hipMalloc(&k, levels * t->n * pst::dtype_size(t->dtype));
-  if (e == hipSuccess) e = hipMalloc(&i, levels * t->n * id_size(t->id_dtype));
+  hipError_t e = hipMalloc(&k, elems * pst::dtype_size(t->dtype));
+  if (e == hipSuccess) e = hipMalloc(&i, elems * id_size(t->id_dtype));
   Status st = pst::hip_status(e, "Error in allocating the level index" + dev_suffix(t->dev_ind, t->num_devs));
   if (st.ok()) st = pst::build_level_index(t->dtype, t->id_dtype, t->n, t->nodes, t->ids, k, i, t->stream);
   if (!st.ok()) {

@@ -120,7 +120,8 @@ unsigned long long* search_error_flag(SearchScratch& sc);
 Status exclusive_scan(const unsigned long long* in, unsigned long long* offsets, size_t n, hipStream_t s);
 
 // ---- level index of a deep tree (pst_levels.hip) ----
-// Device bytes of the index: bit_length(n) levels of n (key, ID) pairs.
+// Device bytes of the index: bit_length(n) levels of n (key, ID) pairs,
+// each level padded to a multiple of 64 pairs.
 size_t level_index_bytes(int dtype, int id_dtype, size_t n);
 // Builds the index of a deep tree with IDs (ids_d: ID per slot) into key_d
 // (unsigned keys of the tree's dtype width) and id_d (blocking).

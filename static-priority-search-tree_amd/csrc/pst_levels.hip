@@ -4,8 +4,9 @@
 // index holds the points of all subtrees rooted at depth t, subtree by
 // subtree in position order, each subtree's points sorted by dim2
 // descending:
-//   key[t * n + k] = ~ord_key(dim2)   (ascending along a subtree's run)
-//   id [t * n + k] = the point's ID
+//   key[t * ns + k] = ~ord_key(dim2)   (ascending along a subtree's run)
+//   id [t * ns + k] = the point's ID
+// (level stride ns = n rounded up to 64, so every level starts aligned).
 // The run of level-t position j starts at level_start(M, t, j) and holds
 // level_size(M, t, j) points (M = n + 1, the closed-form tree shape).
 //
@@ -53,6 +54,28 @@ constexpr int kCopyBytes = 256;  // bytes of loads in flight per lane in the cop
 
 template <typename T>
 using key_of = decltype(ord_key(T{}));
+
+__host__ __device__ inline uint64_t level_stride(uint64_t n) { return (n + 63) & ~63ull; }
+
+// The kLvPer consecutive elements from a[base] (zeros past nt): aligned
+// 16-byte loads when all are in range (base is a multiple of kLvPer and
+// every level starts 256-byte aligned).
+template <typename V>
+__device__ inline void load_run(const V* __restrict__ a, uint32_t base, uint32_t nt, V (&v)[kLvPer]) {
+  if (base + kLvPer <= nt) {
+    const uint4* q = reinterpret_cast<const uint4*>(a + base);
+    constexpr int kVec = sizeof(V) * kLvPer / 16;
+    constexpr int kPer = 16 / sizeof(V);
+#pragma unroll
+    for (int w = 0; w < kVec; w++) {
+      const uint4 x = q[w];
+      __builtin_memcpy(&v[w * kPer], &x, 16);
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < kLvPer; u++) v[u] = base + u < nt ? a[base + u] : V{};
+  }
+}
 
 // #{ j' < j : brev_t(j') < r } (positions are t-bit numbers; r may exceed
 // 2^t - 1, counting every position). Positions j' < j are grouped by the
@@ -123,12 +146,13 @@ __global__ __launch_bounds__(kLvBlock) void k_lv_tile_counts(const uint32_t* __r
                                                              uint32_t t, unsigned long long* __restrict__ agg) {
   __shared__ unsigned long long wsum[kLvBlock / kWave];
   const uint32_t base = blockIdx.x * kLvTile + threadIdx.x * kLvPer;
+  uint32_t sl[kLvPer];
+  load_run(slots, base, nt, sl);
   unsigned long long c = 0;
 #pragma unroll
   for (int u = 0; u < kLvPer; u++) {
-    const uint32_t p = base + u;
-    if (p < nt) {
-      const uint32_t sd = lv_side(slots[p], t);
+    if (base + u < nt) {
+      const uint32_t sd = lv_side(sl[u], t);
       c += sd == 0 ? 1ull : (sd == 1 ? (1ull << 32) : 0ull);
     }
   }
@@ -154,12 +178,15 @@ __global__ __launch_bounds__(kLvBlock) void k_lv_scatter(const uint32_t* __restr
   const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const uint32_t base = blockIdx.x * kLvTile + tid * kLvPer;
   uint32_t s[kLvPer];
+  K kv[kLvPer];
+  IS iv[kLvPer];
+  load_run(slots_in, base, nt, s);
+  load_run(key_in, base, nt, kv);
+  load_run(id_in, base, nt, iv);
   unsigned long long loc = 0;
 #pragma unroll
   for (int u = 0; u < kLvPer; u++) {
-    const uint32_t p = base + u;
-    s[u] = p < nt ? slots_in[p] : ~0u;
-    if (p < nt) {
+    if (base + u < nt) {
       const uint32_t sd = lv_side(s[u], t);
       loc += sd == 0 ? 1ull : (sd == 1 ? (1ull << 32) : 0ull);
     }
@@ -173,27 +200,35 @@ __global__ __launch_bounds__(kLvBlock) void k_lv_scatter(const uint32_t* __restr
   __syncthreads();
   unsigned long long run = tile_pre[blockIdx.x] + incl - loc;
   for (uint32_t w = 0; w < wave; w++) run += wsum[w];
-  uint32_t cj = ~0u;
-  uint64_t cl = 0, cr = 0, sl = 0, sr = 0;
+  // A thread's points lie in runs cj, cj + 1, ... (every level-t run
+  // holds at least its root), so the closed-form counts of the first run
+  // are advanced by the child run sizes from then on.
+  uint32_t cj = 0;
+  uint64_t cl = 0, cr = 0;
+  bool have = false;
 #pragma unroll
   for (int u = 0; u < kLvPer; u++) {
-    const uint32_t p = base + u;
-    if (p >= nt) continue;
-    const uint32_t sd = lv_side(s[u], t);
-    if (sd == 2) continue;
+    if (base + u >= nt) continue;
     const uint32_t d = depth_of(s[u]);
     const uint32_t j = ((s[u] + 1u) >> (d - t)) - (1u << t);
-    if (j != cj) {
+    if (!have) {
       cj = j;
       cl = side_before(M, t, j, 0);
       cr = side_before(M, t, j, 1);
-      sl = static_cast<uint64_t>(level_start(M, t + 1, 2 * j));
-      sr = static_cast<uint64_t>(level_start(M, t + 1, 2 * j + 1));
+      have = true;
     }
+    while (cj < j) {
+      cl += static_cast<uint64_t>(level_size(M, t + 1, 2 * cj));
+      cr += static_cast<uint64_t>(level_size(M, t + 1, 2 * cj + 1));
+      cj++;
+    }
+    if (d == t) continue;
+    const uint32_t sd = ((s[u] + 1u) >> (d - t - 1u)) & 1u;
+    const uint64_t start = sd == 0 ? cl + cr : cl + cr + static_cast<uint64_t>(level_size(M, t + 1, 2 * cj));
     const uint64_t rank = sd == 0 ? (run & 0xffffffffull) - cl : (run >> 32) - cr;
-    const uint64_t dst = (sd == 0 ? sl : sr) + rank;
-    key_out[dst] = key_in[p];
-    id_out[dst] = id_in[p];
+    const uint64_t dst = start + rank;
+    key_out[dst] = kv[u];
+    id_out[dst] = iv[u];
     slots_out[dst] = s[u];
     run += sd == 0 ? 1ull : (1ull << 32);
   }
@@ -205,6 +240,7 @@ struct LvParams {
   const key_of<T>* key;
   const IS* id;
   uint64_t n;
+  uint64_t ns;  // level stride
   uint32_t M, H;
   const Query<T>* queries;
   Query<T> single;
@@ -233,7 +269,7 @@ __global__ __launch_bounds__(kWave) void k_lv_plan(LvParams<T, IS> P) {
   const Query<T> Q = P.use_single ? P.single : P.queries[q];
   const Bounds<T> qb{Q.min_dim1, Q.max_dim1, Q.min_dim2};
   const BoundNum bn = bound_num<T, 1>(&qb);
-  const uint64_t n = P.n;
+  const uint64_t ns = P.ns;
   // a path node: slot | l << 32 | h << 33 | valid << 34
   constexpr uint64_t kValid = 1ull << 34;
   uint64_t mine = lane == 0 ? (kValid | (static_cast<uint64_t>(lo_open(Q.kind)) << 32) |
@@ -256,7 +292,7 @@ __global__ __launch_bounds__(kWave) void k_lv_plan(LvParams<T, IS> P) {
       const uint32_t j = slot + 1u - (1u << t);
       if (o.rep) {
         emit |= 1u;
-        own_src = static_cast<uint64_t>(t) * n + static_cast<uint64_t>(level_start(P.M, t, j));
+        own_src = static_cast<uint64_t>(t) * ns + static_cast<uint64_t>(level_start(P.M, t, j));
       }
       auto child = [&](uint32_t a, uint32_t cl, uint32_t ch, uint32_t side) {
         if (!a) return;
@@ -264,7 +300,7 @@ __global__ __launch_bounds__(kWave) void k_lv_plan(LvParams<T, IS> P) {
         if (!cl && !ch) {
           emit |= 2u;
           const uint32_t cjj = 2u * j + side;
-          ra_src = static_cast<uint64_t>(t + 1) * n + static_cast<uint64_t>(level_start(P.M, t + 1, cjj));
+          ra_src = static_cast<uint64_t>(t + 1) * ns + static_cast<uint64_t>(level_start(P.M, t + 1, cjj));
           ra_size = static_cast<uint32_t>(level_size(P.M, t + 1, cjj));
         } else {
           const uint64_t e = kValid | cs | (static_cast<uint64_t>(cl) << 32) | (static_cast<uint64_t>(ch) << 33);
@@ -495,7 +531,7 @@ uint32_t bit_length(uint64_t n) { return n ? 64u - static_cast<uint32_t>(__built
 size_t level_index_bytes(int dtype, int id_dtype, size_t n) {
   if (n == 0) return 0;
   const size_t per = dtype_size(dtype) + (id_dtype == F64 || id_dtype == I64 || id_dtype == U64 ? 8 : 4);
-  return static_cast<size_t>(bit_length(n)) * n * per;
+  return static_cast<size_t>(bit_length(n)) * level_stride(n) * per;
 }
 
 Status build_level_index(int dtype, int id_dtype, size_t n, const void* nodes_d, const void* ids_d, void* key_d,
@@ -505,6 +541,7 @@ Status build_level_index(int dtype, int id_dtype, size_t n, const void* nodes_d,
   if (!ids_d) return Status{1, "Error: the level index needs a tree with IDs"};
   const uint32_t H = bit_length(n);
   const uint64_t M = n + 1;
+  const size_t ns = level_stride(n);
   return lv_with_t(dtype, [&](auto tv) -> Status {
     using T = decltype(tv);
     using K = key_of<T>;
@@ -551,8 +588,8 @@ Status build_level_index(int dtype, int id_dtype, size_t n, const void* nodes_d,
           st = exclusive_scan(agg, pre, tiles, s);
           if (!st.ok()) break;
           hipLaunchKernelGGL((k_lv_scatter<K, IS>), dim3(tiles), dim3(kLvBlock), 0, s, sa, sb,
-                             key + static_cast<size_t>(t) * n, key + static_cast<size_t>(t + 1) * n,
-                             id + static_cast<size_t>(t) * n, id + static_cast<size_t>(t + 1) * n, pre, nt, t, M);
+                             key + static_cast<size_t>(t) * ns, key + static_cast<size_t>(t + 1) * ns,
+                             id + static_cast<size_t>(t) * ns, id + static_cast<size_t>(t + 1) * ns, pre, nt, t, M);
           std::swap(sa, sb);
         }
         if (st.ok()) st = hip_status(hipGetLastError(), "Error in launching level index construction");
@@ -614,6 +651,7 @@ Status level_search(const SearchArgs& a, const void* key_d, const void* id_d, Le
       P.key = static_cast<const K*>(key_d);
       P.id = static_cast<const IS*>(id_d);
       P.n = a.n;
+      P.ns = level_stride(a.n);
       P.M = static_cast<uint32_t>(a.n + 1);
       P.H = bit_length(a.n);
       P.queries = static_cast<const Query<T>*>(a.queries);

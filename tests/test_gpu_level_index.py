@@ -155,6 +155,7 @@ def test_batch_search_interval_workload_and_toggle(torch, P):
     exp = _expect(pts, qs)
     total = sum(len(e) for e in exp)
     assert P.StaticPSTGPU.level_index_bytes(n, "f32", "u32") == 21 * n * 8
+    assert P.StaticPSTGPU.level_index_bytes(1000, "f64", "u32") == 10 * 1024 * 12
     for on in (True, False, True):
         print(f"batch_search level index on={on}", flush=True)
         tr.set_level_index(on)
