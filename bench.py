@@ -213,6 +213,12 @@ def serve_bench(args, P, D, torch, dev):
         mixes.append(qs)
         sizes.append(n)
     torch.cuda.empty_cache()
+    index_bytes = 0
+    if args.level_index:
+        for tr, n in zip(trees, sizes):
+            tr.set_level_index(True)
+            index_bytes += P.StaticPSTGPU.level_index_bytes(n, tr.t, "u32")
+        torch.cuda.synchronize()
     srv = P.Server(trees, args.serve_budget_gb << 30, max_batch=args.serve_max_batch)
     rng = np.random.default_rng(args.serve_seed)
     nq = int(args.serve_qps * args.serve_seconds)
@@ -261,7 +267,9 @@ def serve_bench(args, P, D, torch, dev):
     return {
         "workload": "8 resident trees 2^20..2^27 (alternating f32 intervals / int32 geometric widths), "
                     f"Poisson arrivals at {args.serve_qps} q/s for {args.serve_seconds} s, tree chosen with "
-                    "probability proportional to size, 32 GB output budget, IDs returned",
+                    "probability proportional to size, 32 GB output budget, IDs returned"
+                    + (", level index on every tree" if args.level_index else ""),
+        "level_index_bytes": index_bytes,
         "offered_qps": args.serve_qps, "completed": done, "submitted": nq,
         "served_qps": round(done / elapsed, 1),
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
@@ -323,6 +331,40 @@ def int_batch_bench(args, P, D, torch, dev):
     s.synchronize()
     eager_ms = e0.elapsed_time(e1) / reps
     g.close()
+    index = None
+    if args.level_index:
+        ks = (0, 1, nq // 2, nq - 1)
+        oh, ch = offs.cpu().numpy(), counts.cpu().numpy()
+        ref = {k: np.sort(out[oh[k]:oh[k] + ch[k]].cpu().numpy()) for k in ks}
+        t0 = time.perf_counter()
+        tree.set_level_index(True)
+        torch.cuda.synchronize()
+        ib_ms = (time.perf_counter() - t0) * 1e3
+        g = tree.batch_graph(dq, nq, counts, offs, out, total + 1, stream=s)
+        g.launch(s)
+        s.synchronize()
+        e0.record(s)
+        for _ in range(reps):
+            g.launch(s)
+        e1.record(s)
+        s.synchronize()
+        ig_ms = e0.elapsed_time(e1) / reps
+        ok &= int(offs[nq].item()) == total
+        oh = offs.cpu().numpy()
+        ok &= all(np.array_equal(np.sort(out[oh[k]:oh[k] + ch[k]].cpu().numpy()), ref[k]) for k in ks)
+        e0.record(s)
+        for _ in range(reps):
+            tree.batch_search(dq, nq, counts, offs, out, total + 1, stream=s)
+        e1.record(s)
+        s.synchronize()
+        ie_ms = e0.elapsed_time(e1) / reps
+        g.close()
+        index = {"build_ms": round(ib_ms, 2), "graph_ms_per_batch": round(ig_ms, 3),
+                 "graph_qps": round(nq / (ig_ms * 1e-3), 1), "eager_ms_per_batch": round(ie_ms, 3),
+                 "reported_ids_per_s": round(total / (ig_ms * 1e-3), 1),
+                 "copy_GBps": round(total * COPY_BYTES_PER_ID / (ig_ms * 1e-3) / 1e9, 1)}
+        log(f"[bench] int32 level index: build {ib_ms:.1f} ms, graph {ig_ms:.2f} ms "
+            f"({nq / (ig_ms * 1e-3):.0f} q/s), eager {ie_ms:.2f} ms")
     tree.close()
     log(f"[bench] int32 2^{args.int_log2n}: build {build_ms:.1f} ms, {nq} queries, {total} IDs, graph "
         f"{graph_ms:.2f} ms ({nq / (graph_ms * 1e-3):.0f} q/s), eager {eager_ms:.2f} ms, tuned {tuned}")
@@ -338,6 +380,7 @@ def int_batch_bench(args, P, D, torch, dev):
         "reported_ids_per_s": round(total / (graph_ms * 1e-3), 1),
         "alg_GBps": round(total * BYTES_PER_ID / (graph_ms * 1e-3) / 1e9, 1),
         "autotuned": tuned,
+        "level_index": index,
         "results_checked": bool(ok),
     }
 

@@ -342,11 +342,11 @@ int pst_gpu_set_query_grouping(pst_tree* tree, int queries_per_walk);
  * every depth t it holds the points of each subtree rooted at depth t
  * sorted by dim2 descending, so a subtree whose dim1 range lies inside a
  * query's bounds reports a prefix of its sorted run: one binary search and
- * one contiguous copy. While it exists, pst_gpu_batch_count and the ID
+ * one contiguous copy. While it exists, pst_gpu_batch_count, and the ID
  * returning pst_gpu_batch_write / pst_gpu_batch_search (and the batch
- * graphs made from them) walk only the query's boundary paths and stream
- * the IDs; results are the same sets as without it (order within a
- * query's results differs). Trees of fewer than 2^31 points; not for a
+ * graphs made from them), single searches and pst_gpu_search_into walk
+ * only the query's boundary paths and stream the IDs; results are the same
+ * sets as without it (order within a query's results differs). Trees of fewer than 2^31 points; not for a
  * forest or a tree without IDs (PST_ERR_INVALID). Device memory:
  * pst_gpu_level_index_bytes. */
 int pst_gpu_set_level_index(pst_tree* tree, int on);

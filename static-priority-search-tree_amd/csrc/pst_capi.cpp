@@ -146,11 +146,13 @@ int single_search(pst_tree* t, int kind, int ret, const void* a, const void* b, 
   args.counts = t->sc.cursor;
   args.cap_all = cap;
   args.out = out;
-  st = pst::run_search(args, t->sc, s);
+  const bool via_index = t->lv_key && ret != PST_RET_POINT;
+  st = via_index ? pst::level_search(args, t->lv_key, t->lv_id, t->lsc, s) : pst::run_search(args, t->sc, s);
   if (!st.ok()) return fail(st);
   unsigned long long host[2] = {0, 0};
   hipError_t e = hipMemcpyAsync(&host[0], t->sc.cursor, 8, hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(&host[1], pst::search_error_flag(t->sc), 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && !via_index)
+    e = hipMemcpyAsync(&host[1], pst::search_error_flag(t->sc), 8, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess)
     return fail(PST_ERR_HIP, "Error in copying global result array final index from device" +

@@ -416,20 +416,20 @@ __global__ __launch_bounds__(kLvBlock) void k_lv_mark(const unsigned long long* 
                                                       const uint32_t* __restrict__ tpre,
                                                       const uint32_t* __restrict__ ntask, uint32_t nq,
                                                       uint32_t* __restrict__ chunk_task) {
-  const uint32_t q = blockIdx.x * kLvBlock + threadIdx.x;
-  if (q >= nq) return;
+  // one thread per (query, range)
+  const uint64_t g = static_cast<uint64_t>(blockIdx.x) * kLvBlock + threadIdx.x;
+  const uint32_t q = static_cast<uint32_t>(g / kTaskCap), i = static_cast<uint32_t>(g % kTaskCap);
+  if (q >= nq || i >= ntask[q]) return;
   const uint64_t L = chunk_len(woff[nq]);
   const uint64_t base = woff[q];
-  const uint32_t nt = ntask[q];
   const uint64_t qb = static_cast<uint64_t>(q) * kTaskCap;
-  for (uint32_t i = 0; i < nt; i++) {
-    const uint64_t a = base + tpre[qb + i], b = base + tpre[qb + i + 1];
-    for (uint64_t c = (a + L - 1) / L; c * L < b; c++) chunk_task[c] = (q << 7) | i;
-  }
+  const uint64_t a = base + tpre[qb + i], b = base + tpre[qb + i + 1];
+  for (uint64_t c = (a + L - 1) / L; c * L < b; c++) chunk_task[c] = (q << 7) | i;
 }
 
-// All lanes of the workgroup copy len IDs, kCopyBytes per lane in flight
-// (every load of a pass issued before its stores).
+// All lanes of the workgroup copy len IDs: full passes with kCopyBytes of
+// loads in flight per lane (every load of a pass issued before its stores),
+// then passes of 8 IDs per lane for the rest.
 template <typename IS>
 __device__ inline void copy_run(const IS* __restrict__ src, IS* __restrict__ dst, uint64_t len, uint32_t tid) {
   constexpr int U = kCopyBytes / sizeof(IS);
@@ -442,15 +442,16 @@ __device__ inline void copy_run(const IS* __restrict__ src, IS* __restrict__ dst
 #pragma unroll
     for (int u = 0; u < U; u++) dst[k0 + u * kCopyBlock + tid] = v[u];
   }
-  if (k0 < len) {
-    IS v[U];
+  constexpr int V = 8;
+  for (; k0 < len; k0 += static_cast<uint64_t>(kCopyBlock) * V) {
+    IS v[V];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
+    for (int u = 0; u < V; u++) {
       const uint64_t k = k0 + u * kCopyBlock + tid;
       if (k < len) v[u] = src[k];
     }
 #pragma unroll
-    for (int u = 0; u < U; u++) {
+    for (int u = 0; u < V; u++) {
       const uint64_t k = k0 + u * kCopyBlock + tid;
       if (k < len) dst[k] = v[u];
     }
@@ -459,7 +460,9 @@ __device__ inline void copy_run(const IS* __restrict__ src, IS* __restrict__ dst
 
 // Streams the batch's ranges to the outputs, a chunk of IDs per workgroup
 // pass: the ranges a chunk overlaps are copied in turn, all lanes on one
-// range.
+// range. The current query's range table (output positions and sources)
+// is staged in LDS, so moving from range to range costs no memory round
+// trip.
 template <typename IS>
 __global__ __launch_bounds__(kCopyBlock) void k_lv_copy(const IS* __restrict__ ids, const uint64_t* __restrict__ tsrc,
                                                         const uint32_t* __restrict__ tpre,
@@ -468,6 +471,8 @@ __global__ __launch_bounds__(kCopyBlock) void k_lv_copy(const IS* __restrict__ i
                                                         const unsigned long long* __restrict__ offsets,
                                                         const uint32_t* __restrict__ chunk_task, uint32_t nq,
                                                         IS* __restrict__ out) {
+  __shared__ uint32_t s_pre[kTaskCap + 1];
+  __shared__ uint64_t s_src[kTaskCap];
   const uint64_t W = woff[nq];
   const uint64_t L = chunk_len(W);
   const uint64_t nch = (W + L - 1) / L;
@@ -477,10 +482,21 @@ __global__ __launch_bounds__(kCopyBlock) void k_lv_copy(const IS* __restrict__ i
     uint32_t q = e >> 7, i = e & 127u;
     uint64_t x = c * L;
     const uint64_t x1 = x + L < W ? x + L : W;
+    uint32_t staged = ~0u, nt = 0;
+    uint64_t wbase = 0, obase = 0;
     while (x < x1) {
-      const uint64_t qb = static_cast<uint64_t>(q) * kTaskCap;
-      const uint64_t wbase = woff[q];
-      if (i >= ntask[q]) {
+      if (q != staged) {
+        __syncthreads();  // every lane is done with the previous table
+        nt = ntask[q];
+        wbase = woff[q];
+        obase = offsets ? offsets[q] : 0ull;
+        const uint64_t qb = static_cast<uint64_t>(q) * kTaskCap;
+        if (tid <= nt) s_pre[tid] = tpre[qb + tid];
+        if (tid < nt) s_src[tid] = tsrc[qb + tid];
+        __syncthreads();
+        staged = q;
+      }
+      if (i >= nt) {
         // the next query with IDs: the last q' with woff[q'] <= x
         uint32_t lo = q + 1, hi = nq - 1;
         while (lo < hi) {
@@ -493,12 +509,10 @@ __global__ __launch_bounds__(kCopyBlock) void k_lv_copy(const IS* __restrict__ i
         i = 0;
         continue;
       }
-      const uint64_t a = wbase + tpre[qb + i], b = wbase + tpre[qb + i + 1];
+      const uint64_t a = wbase + s_pre[i], b = wbase + s_pre[i + 1];
       if (x < a || x >= b) break;  // not a range of this batch's plan
       const uint64_t hi = b < x1 ? b : x1;
-      const IS* src = ids + tsrc[qb + i] + (x - a);
-      IS* dst = out + (offsets ? offsets[q] : 0ull) + (x - wbase);
-      copy_run(src, dst, hi - x, tid);
+      copy_run(ids + s_src[i] + (x - a), out + obase + (x - wbase), hi - x, tid);
       x = hi;
       if (x == b) i++;
     }
@@ -676,8 +690,9 @@ Status level_search(const SearchArgs& a, const void* key_d, const void* id_d, Le
       hipLaunchKernelGGL((k_lv_plan<T, IS, true>), dim3(P.nq), dim3(kWave), 0, s, P);
       Status st = exclusive_scan(sc.wq, sc.woff, a.nq, s);
       if (!st.ok()) return st;
-      hipLaunchKernelGGL(k_lv_mark, dim3((P.nq + kLvBlock - 1) / kLvBlock), dim3(kLvBlock), 0, s, sc.woff, sc.tpre,
-                         sc.ntask, P.nq, sc.chunk_task);
+      const uint64_t mark_threads = static_cast<uint64_t>(P.nq) * kTaskCap;
+      hipLaunchKernelGGL(k_lv_mark, dim3(static_cast<unsigned>((mark_threads + kLvBlock - 1) / kLvBlock)),
+                         dim3(kLvBlock), 0, s, sc.woff, sc.tpre, sc.ntask, P.nq, sc.chunk_task);
       int dev = 0, cus = 0;
       hipGetDevice(&dev);
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);

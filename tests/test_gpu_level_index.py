@@ -220,3 +220,35 @@ def test_rejected_without_ids_and_for_forests(torch, P):
     assert e.value.code == 1 and "forest" in str(e.value)
     f.close()
     assert P.StaticPSTGPU.level_index_bytes(1000, "f32", None) == 0
+
+
+@pytest.mark.parametrize("t,i", [("f32", "u32"), ("i64", "u64"), ("f64", "i32")])
+def test_single_searches_through_the_index(torch, P, t, i):
+    """threeSidedSearch / twoSidedLeftSearch / twoSidedRightSearch and
+    search_into (with a capacity that is too small: the count is still
+    returned, at most capacity IDs written) through the index equal brute
+    force; points are still returned by the walk."""
+    rng = np.random.default_rng(len(t) * 5 + len(i))
+    n = 20011
+    p = _points(rng, t, i, n, t != "f32")
+    tr = P.StaticPSTGPU(P.to_device(p).data_ptr(), n, t, i)
+    tr.set_level_index(True)
+    qs = _queries(rng, P, t, p, 24)
+    out = torch.full((n + 16,), -1, dtype=torch.int64, device="cuda")
+    for q in qs:
+        kind, a, b, c = int(q["kind"]), q["min_dim1"], q["max_dim1"], q["min_dim2"]
+        exp = p[O.brute_force(p, kind, a, b, c)]
+        got = tr.search(kind, a, b, c)
+        assert np.array_equal(np.sort(got), np.sort(exp["id"]))
+        pts = tr.search(kind, a, b, c, ret=P.RET_POINT)
+        assert G.canon(pts, True, True) == G.canon(exp, True, True)
+        cap = max(0, len(exp) // 2)
+        out.fill_(-1)
+        num = tr.search_into(kind, a, b, c, out, cap)
+        assert num == len(exp)
+        r = out.cpu().numpy().view(O.NP[i])
+        isz = np.dtype(O.NP[i]).itemsize
+        w = r[:cap]
+        assert len(np.unique(w)) == cap and np.all(np.isin(w, exp["id"]))
+        assert np.all(out.cpu().numpy().view(np.uint8)[cap * isz:] == 0xFF)
+    tr.close()
