@@ -152,6 +152,9 @@ class StaticPSTGPU : public StaticPrioritySearchTree<T, PointStructTemplate, IDT
   // Beyond the reference: persistence of the built tree (pst_gpu_save,
   // pst_gpu_load). load() needs neither the input points nor a rebuild.
   void save(const std::string& path) const { pst_amd::check(pst_gpu_save(tree, path.c_str())); }
+  // Level index for ID searches (pst_gpu_set_level_index; no counterpart in
+  // the reference).
+  void setLevelIndex(bool on) { pst_amd::check(pst_gpu_set_level_index(tree, on ? 1 : 0)); }
   static std::unique_ptr<StaticPSTGPU> load(const std::string& path, int dev_ind = 0, int num_devs = 1) {
     pst_tree* h = nullptr;
     pst_amd::check(pst_gpu_load(path.c_str(), dev_ind, num_devs, nullptr, &h));

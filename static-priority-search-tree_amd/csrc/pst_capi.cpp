@@ -588,9 +588,31 @@ int pst_gpu_batch_search(pst_tree* t, const void* queries_d, size_t nq, int ret,
   if (nq >= (1ull << 24)) return fail(PST_ERR_INVALID, "Error: batches hold fewer than 2^24 queries");
   if (ret != PST_RET_POINT && t->id_dtype == PST_NO_ID)
     return fail(PST_ERR_INVALID, "Error: IDs requested from a tree built without IDs");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (t->lv_key && ret != PST_RET_POINT && t->n > 0) {
+    // through the level index: the count pass keeps its ranges and the
+    // write step only clips them
+    Status st = pst::scratch_reserve(t->sc, nq, 0, s);
+    pst::SearchArgs a = base_args(t);
+    a.queries = queries_d;
+    a.nq = nq;
+    a.mode = 0;
+    a.counts = counts_d;
+    if (st.ok()) st = pst::level_search(a, t->lv_key, t->lv_id, t->lsc, s, 1);
+    if (st.ok()) st = pst::exclusive_scan(counts_d, offsets_d, nq, s);
+    a.mode = 1;
+    a.ret_point = ret;
+    a.counts = t->sc.cursor;
+    a.offsets = offsets_d;
+    a.caps = counts_d;
+    a.cap_all = ~0ull;
+    a.out = out_d;
+    a.out_total = out_capacity;
+    if (st.ok()) st = pst::level_search(a, t->lv_key, t->lv_id, t->lsc, s, 2);
+    return st.ok() ? PST_OK : fail(st);
+  }
   int rc = pst_gpu_batch_count(t, queries_d, nq, counts_d, stream);
   if (rc != PST_OK) return rc;
-  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   Status st = pst::exclusive_scan(counts_d, offsets_d, nq, s);
   if (!st.ok()) return fail(st);
   if (t->n == 0) return PST_OK;
@@ -607,10 +629,6 @@ int pst_gpu_batch_search(pst_tree* t, const void* queries_d, size_t nq, int ret,
   a.cap_all = ~0ull;
   a.out = out_d;
   a.out_total = out_capacity;
-  if (t->lv_key && ret != PST_RET_POINT) {
-    st = pst::level_search(a, t->lv_key, t->lv_id, t->lsc, s);
-    return st.ok() ? PST_OK : fail(st);
-  }
   a.groups_ready = !t->lv_key;  // the count pass above grouped this batch (unless the index counted it)
   st = pst::run_search(a, t->sc, s);
   return st.ok() ? PST_OK : fail(st);

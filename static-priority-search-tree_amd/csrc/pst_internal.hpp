@@ -139,8 +139,11 @@ struct LevelScratch {
 Status level_scratch_reserve(LevelScratch& sc, size_t nq);
 void level_scratch_release(LevelScratch& sc);
 // Batched count (a.mode 0) or ID write (a.mode 1) with the semantics of
-// run_search, through the level index (asynchronous).
-Status level_search(const SearchArgs& a, const void* key_d, const void* id_d, LevelScratch& sc, hipStream_t stream);
+// run_search, through the level index (asynchronous). plan 1 (count): also
+// keep the batch's ranges in the scratch; plan 2 (write): write from the
+// ranges kept by the count pass of the same batch instead of planning again.
+Status level_search(const SearchArgs& a, const void* key_d, const void* id_d, LevelScratch& sc, hipStream_t stream,
+                    int plan = 0);
 
 // ---- interval parallel search (pst_ips.hip) ----
 // counter_d must be zero on entry; results past `cap` are counted only.

@@ -245,6 +245,7 @@ struct LvParams {
   const Query<T>* queries;
   Query<T> single;
   uint32_t use_single;
+  uint32_t keep;  // count mode: also keep the ranges (unclipped) for a later write from them
   uint32_t nq;
   unsigned long long* counts;  // count mode: per-query counts; write mode: per-query totals
   const unsigned long long* offsets;
@@ -369,15 +370,17 @@ __global__ __launch_bounds__(kWave) void k_lv_plan(LvParams<T, IS> P) {
   const uint32_t tot0 = __shfl(inc0, kWave - 1, kWave);
   inc1 += tot0;
   const uint32_t total = __shfl(inc1, kWave - 1, kWave);
-  if constexpr (!W) {
+  if (!W && !P.keep) {
     if (lane == 0) P.counts[q] = total;
   } else {
-    const unsigned long long off = P.offsets ? P.offsets[q] : 0ull;
-    const unsigned long long cap = P.caps ? P.caps[q] : P.cap_all;
-    const unsigned long long room = off < P.out_total ? P.out_total - off : 0ull;
     unsigned long long clip = total;
-    if (cap < clip) clip = cap;
-    if (room < clip) clip = room;
+    if constexpr (W) {
+      const unsigned long long off = P.offsets ? P.offsets[q] : 0ull;
+      const unsigned long long cap = P.caps ? P.caps[q] : P.cap_all;
+      const unsigned long long room = off < P.out_total ? P.out_total - off : 0ull;
+      if (cap < clip) clip = cap;
+      if (room < clip) clip = room;
+    }
     const uint32_t cw = static_cast<uint32_t>(clip);
     // keep the ranges that start before the clip point and hold IDs
     const uint32_t pre0 = inc0 - cnt[0], pre1 = inc1 - cnt[1];
@@ -398,10 +401,39 @@ __global__ __launch_bounds__(kWave) void k_lv_plan(LvParams<T, IS> P) {
       const uint32_t nt = static_cast<uint32_t>(__builtin_popcountll(b0) + __builtin_popcountll(b1));
       P.tpre[qb0 + nt] = cw;
       P.ntask[q] = nt;
-      P.wq[q] = clip;
+      if constexpr (W) P.wq[q] = clip;
       P.counts[q] = total;
     }
   }
+}
+
+// Write step from ranges kept by a count pass: clip each query's ranges to
+// its capacity and the output's size (a thread per query).
+__global__ __launch_bounds__(kLvBlock) void k_lv_clip(uint32_t* __restrict__ tpre, uint32_t* __restrict__ ntask,
+                                                      unsigned long long* __restrict__ wq,
+                                                      unsigned long long* __restrict__ totals,
+                                                      const unsigned long long* __restrict__ offsets,
+                                                      const unsigned long long* __restrict__ caps,
+                                                      unsigned long long cap_all, unsigned long long out_total,
+                                                      uint32_t nq) {
+  const uint32_t q = blockIdx.x * kLvBlock + threadIdx.x;
+  if (q >= nq) return;
+  const uint64_t qb = static_cast<uint64_t>(q) * kTaskCap;
+  uint32_t nt = ntask[q];
+  const uint32_t total = tpre[qb + nt];
+  const unsigned long long off = offsets ? offsets[q] : 0ull;
+  const unsigned long long cap = caps ? caps[q] : cap_all;
+  const unsigned long long room = off < out_total ? out_total - off : 0ull;
+  unsigned long long clip = total;
+  if (cap < clip) clip = cap;
+  if (room < clip) clip = room;
+  if (clip < total) {
+    while (nt > 0 && tpre[qb + nt - 1] >= clip) nt--;
+    ntask[q] = nt;
+    tpre[qb + nt] = static_cast<uint32_t>(clip);
+  }
+  wq[q] = clip;
+  totals[q] = total;
 }
 
 __host__ __device__ inline uint64_t chunk_len(uint64_t W) {
@@ -648,10 +680,13 @@ void level_scratch_release(LevelScratch& sc) {
   sc = LevelScratch{};
 }
 
-Status level_search(const SearchArgs& a, const void* key_d, const void* id_d, LevelScratch& sc, hipStream_t s) {
+Status level_search(const SearchArgs& a, const void* key_d, const void* id_d, LevelScratch& sc, hipStream_t s,
+                    int plan) {
   if (a.nq == 0 || a.n == 0) return Status{};
   if (a.mode == 1 && a.ret_point) return Status{1, "Error: the level index returns IDs only"};
-  if (a.mode == 1) {
+  if (plan == 2 && (a.mode != 1 || sc.cap < a.nq || !sc.tsrc))
+    return Status{1, "Error: no kept ranges for this batch"};
+  if (a.mode == 1 || plan == 1) {
     Status st = level_scratch_reserve(sc, a.nq);
     if (!st.ok()) return st;
   }
@@ -684,10 +719,16 @@ Status level_search(const SearchArgs& a, const void* key_d, const void* id_d, Le
       P.ntask = sc.ntask;
       P.wq = sc.wq;
       if (a.mode == 0) {
+        P.keep = plan == 1;
         hipLaunchKernelGGL((k_lv_plan<T, IS, false>), dim3(P.nq), dim3(kWave), 0, s, P);
         return hip_status(hipGetLastError(), "Error in launching level search");
       }
-      hipLaunchKernelGGL((k_lv_plan<T, IS, true>), dim3(P.nq), dim3(kWave), 0, s, P);
+      if (plan == 2) {
+        hipLaunchKernelGGL(k_lv_clip, dim3((P.nq + kLvBlock - 1) / kLvBlock), dim3(kLvBlock), 0, s, sc.tpre, sc.ntask,
+                           sc.wq, a.counts, a.offsets, a.caps, a.cap_all, P.out_total, P.nq);
+      } else {
+        hipLaunchKernelGGL((k_lv_plan<T, IS, true>), dim3(P.nq), dim3(kWave), 0, s, P);
+      }
       Status st = exclusive_scan(sc.wq, sc.woff, a.nq, s);
       if (!st.ok()) return st;
       const uint64_t mark_threads = static_cast<uint64_t>(P.nq) * kTaskCap;

@@ -51,7 +51,9 @@ void run_with_ids(size_t n, unsigned seed) {
   hipDeviceProp_t props{};
   (void)hipGetDeviceProperties(&props, 0);
   StaticPSTGPU<T, PointStruct, unsigned, 1> tree(d, n, 1, 0, 1, props);
-  for (int q = 0; q < 20; q++) {
+  // the second half of the queries runs with the level index built
+  for (int q = 0; q < 40; q++) {
+    if (q == 20) tree.setLevelIndex(true);
     T a = static_cast<T>(u(rng)), b = a + static_cast<T>(u(rng) % 20), c = static_cast<T>(u(rng));
     size_t k = 0;
     unsigned* ids = nullptr;
