@@ -167,6 +167,11 @@ __global__ __launch_bounds__(kLvBlock) void k_lv_tile_counts(const uint32_t* __r
 }
 
 // Level t -> t+1: every point but its run's root moves to its child run.
+// Phase 1 (a thread per 16 consecutive points): each point's destination,
+// into LDS. Phase 2 (a thread per point, 256 consecutive points per pass):
+// the moves, so that a wave's stores fall in at most a few contiguous
+// stretches of the child runs and leave as whole lines.
+constexpr uint32_t kDstPad = kLvPer + 1;  // LDS row of a thread's 16 destinations, padded against bank conflicts
 template <typename K, typename IS>
 __global__ __launch_bounds__(kLvBlock) void k_lv_scatter(const uint32_t* __restrict__ slots_in,
                                                          uint32_t* __restrict__ slots_out,
@@ -175,14 +180,12 @@ __global__ __launch_bounds__(kLvBlock) void k_lv_scatter(const uint32_t* __restr
                                                          const unsigned long long* __restrict__ tile_pre,
                                                          uint32_t nt, uint32_t t, uint64_t M) {
   __shared__ unsigned long long wsum[kLvBlock / kWave];
+  __shared__ uint32_t s_dst[kLvBlock * kDstPad];
   const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  const uint32_t base = blockIdx.x * kLvTile + tid * kLvPer;
+  const uint32_t tile = blockIdx.x * kLvTile;
+  const uint32_t base = tile + tid * kLvPer;
   uint32_t s[kLvPer];
-  K kv[kLvPer];
-  IS iv[kLvPer];
   load_run(slots_in, base, nt, s);
-  load_run(key_in, base, nt, kv);
-  load_run(id_in, base, nt, iv);
   unsigned long long loc = 0;
 #pragma unroll
   for (int u = 0; u < kLvPer; u++) {
@@ -208,29 +211,42 @@ __global__ __launch_bounds__(kLvBlock) void k_lv_scatter(const uint32_t* __restr
   bool have = false;
 #pragma unroll
   for (int u = 0; u < kLvPer; u++) {
-    if (base + u >= nt) continue;
-    const uint32_t d = depth_of(s[u]);
-    const uint32_t j = ((s[u] + 1u) >> (d - t)) - (1u << t);
-    if (!have) {
-      cj = j;
-      cl = side_before(M, t, j, 0);
-      cr = side_before(M, t, j, 1);
-      have = true;
+    uint32_t dst = ~0u;  // the run's root (dropped) or past the level
+    if (base + u < nt) {
+      const uint32_t d = depth_of(s[u]);
+      const uint32_t j = ((s[u] + 1u) >> (d - t)) - (1u << t);
+      if (!have) {
+        cj = j;
+        cl = side_before(M, t, j, 0);
+        cr = side_before(M, t, j, 1);
+        have = true;
+      }
+      while (cj < j) {
+        cl += static_cast<uint64_t>(level_size(M, t + 1, 2 * cj));
+        cr += static_cast<uint64_t>(level_size(M, t + 1, 2 * cj + 1));
+        cj++;
+      }
+      if (d != t) {
+        const uint32_t sd = ((s[u] + 1u) >> (d - t - 1u)) & 1u;
+        const uint64_t start = sd == 0 ? cl + cr : cl + cr + static_cast<uint64_t>(level_size(M, t + 1, 2 * cj));
+        const uint64_t rank = sd == 0 ? (run & 0xffffffffull) - cl : (run >> 32) - cr;
+        dst = static_cast<uint32_t>(start + rank);
+        run += sd == 0 ? 1ull : (1ull << 32);
+      }
     }
-    while (cj < j) {
-      cl += static_cast<uint64_t>(level_size(M, t + 1, 2 * cj));
-      cr += static_cast<uint64_t>(level_size(M, t + 1, 2 * cj + 1));
-      cj++;
+    s_dst[tid * kDstPad + u] = dst;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kLvPer; u++) {
+    const uint32_t r = u * kLvBlock + tid;  // point within the tile
+    const uint32_t p = tile + r;
+    const uint32_t dst = s_dst[(r / kLvPer) * kDstPad + r % kLvPer];
+    if (p < nt && dst != ~0u) {
+      key_out[dst] = key_in[p];
+      id_out[dst] = id_in[p];
+      slots_out[dst] = slots_in[p];
     }
-    if (d == t) continue;
-    const uint32_t sd = ((s[u] + 1u) >> (d - t - 1u)) & 1u;
-    const uint64_t start = sd == 0 ? cl + cr : cl + cr + static_cast<uint64_t>(level_size(M, t + 1, 2 * cj));
-    const uint64_t rank = sd == 0 ? (run & 0xffffffffull) - cl : (run >> 32) - cr;
-    const uint64_t dst = start + rank;
-    key_out[dst] = kv[u];
-    id_out[dst] = iv[u];
-    slots_out[dst] = s[u];
-    run += sd == 0 ? 1ull : (1ull << 32);
   }
 }
 
@@ -737,7 +753,9 @@ Status level_search(const SearchArgs& a, const void* key_d, const void* id_d, Le
       int dev = 0, cus = 0;
       hipGetDevice(&dev);
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-      hipLaunchKernelGGL((k_lv_copy<IS>), dim3(static_cast<unsigned>(cus * 8)), dim3(kCopyBlock), 0, s, P.id,
+      // three workgroups per CU: measured on the 2^26 batch, 2-4 beat 8
+      // (fewer waves, each with 256 B of loads in flight)
+      hipLaunchKernelGGL((k_lv_copy<IS>), dim3(static_cast<unsigned>(cus * 3)), dim3(kCopyBlock), 0, s, P.id,
                          sc.tsrc, sc.tpre, sc.ntask, sc.woff, a.offsets, sc.chunk_task, P.nq,
                          static_cast<IS*>(a.out));
       return hip_status(hipGetLastError(), "Error in launching level search");
