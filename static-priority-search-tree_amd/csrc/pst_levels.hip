@@ -51,6 +51,7 @@ constexpr uint64_t kMinChunk = 32768;            // IDs per copy chunk (at least
 constexpr uint64_t kChunkCap = 1u << 20;         // chunk table entries
 constexpr int kCopyBlock = 256;
 constexpr int kCopyBytes = 256;  // bytes of loads in flight per lane in the copy
+constexpr uint32_t kFlat = 4096;  // positions per flat pass of the copy (16 per lane)
 
 template <typename T>
 using key_of = decltype(ord_key(T{}));
@@ -475,41 +476,28 @@ __global__ __launch_bounds__(kLvBlock) void k_lv_mark(const unsigned long long* 
   for (uint64_t c = (a + L - 1) / L; c * L < b; c++) chunk_task[c] = (q << 7) | i;
 }
 
-// All lanes of the workgroup copy len IDs: full passes with kCopyBytes of
-// loads in flight per lane (every load of a pass issued before its stores),
-// then passes of 8 IDs per lane for the rest.
+// All lanes of the workgroup copy len IDs, a multiple of the pass size,
+// with kCopyBytes of loads in flight per lane (every load of a pass issued
+// before its stores).
 template <typename IS>
-__device__ inline void copy_run(const IS* __restrict__ src, IS* __restrict__ dst, uint64_t len, uint32_t tid) {
+__device__ inline void copy_passes(const IS* __restrict__ src, IS* __restrict__ dst, uint64_t len, uint32_t tid) {
   constexpr int U = kCopyBytes / sizeof(IS);
   constexpr uint64_t kPass = static_cast<uint64_t>(kCopyBlock) * U;
-  uint64_t k0 = 0;
-  for (; k0 + kPass <= len; k0 += kPass) {
+  for (uint64_t k0 = 0; k0 < len; k0 += kPass) {
     IS v[U];
 #pragma unroll
     for (int u = 0; u < U; u++) v[u] = src[k0 + u * kCopyBlock + tid];
 #pragma unroll
     for (int u = 0; u < U; u++) dst[k0 + u * kCopyBlock + tid] = v[u];
   }
-  constexpr int V = 8;
-  for (; k0 < len; k0 += static_cast<uint64_t>(kCopyBlock) * V) {
-    IS v[V];
-#pragma unroll
-    for (int u = 0; u < V; u++) {
-      const uint64_t k = k0 + u * kCopyBlock + tid;
-      if (k < len) v[u] = src[k];
-    }
-#pragma unroll
-    for (int u = 0; u < V; u++) {
-      const uint64_t k = k0 + u * kCopyBlock + tid;
-      if (k < len) dst[k] = v[u];
-    }
-  }
 }
 
 // Streams the batch's ranges to the outputs, a chunk of IDs per workgroup
-// pass: the ranges a chunk overlaps are copied in turn, all lanes on one
-// range. The current query's range table (output positions and sources)
-// is staged in LDS, so moving from range to range costs no memory round
+// pass. The current query's range table (output positions and sources) is
+// staged in LDS. A range's whole passes are copied with all lanes on it;
+// what is left of it, and the ranges after it in the same query and chunk,
+// go in flat passes of kFlat positions where each lane finds its
+// position's range in the table, so short ranges share a memory round
 // trip.
 template <typename IS>
 __global__ __launch_bounds__(kCopyBlock) void k_lv_copy(const IS* __restrict__ ids, const uint64_t* __restrict__ tsrc,
@@ -560,9 +548,39 @@ __global__ __launch_bounds__(kCopyBlock) void k_lv_copy(const IS* __restrict__ i
       const uint64_t a = wbase + s_pre[i], b = wbase + s_pre[i + 1];
       if (x < a || x >= b) break;  // not a range of this batch's plan
       const uint64_t hi = b < x1 ? b : x1;
-      copy_run(ids + s_src[i] + (x - a), out + obase + (x - wbase), hi - x, tid);
-      x = hi;
-      if (x == b) i++;
+      constexpr uint64_t kPass = static_cast<uint64_t>(kCopyBlock) * (kCopyBytes / sizeof(IS));
+      const uint64_t bulk = ((hi - x) / kPass) * kPass;
+      if (bulk) {
+        copy_passes(ids + s_src[i] + (x - a), out + obase + (x - wbase), bulk, tid);
+        x += bulk;
+        if (x == b) i++;
+        continue;
+      }
+      // a flat pass from x to the query's, the chunk's or the pass's end
+      const uint64_t qend = wbase + s_pre[nt];
+      uint64_t fend = x + kFlat;
+      if (fend > qend) fend = qend;
+      if (fend > x1) fend = x1;
+      {
+        constexpr int V = kFlat / kCopyBlock;
+        IS v[V];
+        uint32_t loc[V];
+        uint32_t ii = i;
+#pragma unroll
+        for (int u = 0; u < V; u++) {
+          const uint64_t pos = x + u * kCopyBlock + tid;
+          loc[u] = static_cast<uint32_t>(pos - wbase);
+          if (pos < fend) {
+            while (s_pre[ii + 1] <= loc[u]) ii++;
+            v[u] = ids[s_src[ii] + (loc[u] - s_pre[ii])];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < V; u++)
+          if (x + u * kCopyBlock + tid < fend) out[obase + loc[u]] = v[u];
+      }
+      x = fend;
+      while (i < nt && wbase + s_pre[i + 1] <= x) i++;
     }
   }
 }
