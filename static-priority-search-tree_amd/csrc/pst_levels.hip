@@ -251,6 +251,12 @@ __global__ __launch_bounds__(kLvBlock) void k_lv_scatter(const uint32_t* __restr
   }
 }
 
+__host__ __device__ inline uint64_t chunk_len(uint64_t W) {
+  uint64_t c = (W + kChunkCap - 1) / kChunkCap;
+  c = (c + 255) & ~255ull;
+  return c > kMinChunk ? c : kMinChunk;
+}
+
 template <typename T, typename IS>
 struct LvParams {
   const Node<T, IS>* nodes;
@@ -273,6 +279,8 @@ struct LvParams {
   uint32_t* tpre;   // per query, kTaskCap: output position of each range (then the query's total)
   uint32_t* ntask;  // ranges per query
   unsigned long long* wq;  // IDs written per query
+  unsigned long long* woff;  // one query (use_single, write mode): its work offsets, written by the plan
+  uint32_t* chunk_task;      // and its chunk table
 };
 
 // One workgroup (one wave) per query: the open paths, level by level, then
@@ -421,6 +429,23 @@ __global__ __launch_bounds__(kWave) void k_lv_plan(LvParams<T, IS> P) {
       if constexpr (W) P.wq[q] = clip;
       P.counts[q] = total;
     }
+    if constexpr (W) {
+      if (P.use_single) {
+        // one query: its work offsets and chunk table directly (no scan or
+        // marking launch); a kept range ends where the next one starts
+        if (lane == 0) {
+          P.woff[0] = 0;
+          P.woff[1] = clip;
+        }
+        const uint64_t L = chunk_len(clip);
+        auto mark = [&](uint32_t at, uint32_t pre, uint32_t c) {
+          const uint64_t a = pre, b = pre + c < cw ? pre + c : cw;
+          for (uint64_t ch = (a + L - 1) / L; ch * L < b; ch++) P.chunk_task[ch] = at;
+        };
+        if (k0) mark(at0, pre0, cnt[0]);
+        if (k1) mark(at1, pre1, cnt[1]);
+      }
+    }
   }
 }
 
@@ -451,12 +476,6 @@ __global__ __launch_bounds__(kLvBlock) void k_lv_clip(uint32_t* __restrict__ tpr
   }
   wq[q] = clip;
   totals[q] = total;
-}
-
-__host__ __device__ inline uint64_t chunk_len(uint64_t W) {
-  uint64_t c = (W + kChunkCap - 1) / kChunkCap;
-  c = (c + 255) & ~255ull;
-  return c > kMinChunk ? c : kMinChunk;
 }
 
 // Chunk table: chunk c (IDs [c L, (c+1) L) of the batch's concatenated
@@ -752,6 +771,8 @@ Status level_search(const SearchArgs& a, const void* key_d, const void* id_d, Le
       P.tpre = sc.tpre;
       P.ntask = sc.ntask;
       P.wq = sc.wq;
+      P.woff = sc.woff;
+      P.chunk_task = sc.chunk_task;
       if (a.mode == 0) {
         P.keep = plan == 1;
         hipLaunchKernelGGL((k_lv_plan<T, IS, false>), dim3(P.nq), dim3(kWave), 0, s, P);
@@ -763,11 +784,13 @@ Status level_search(const SearchArgs& a, const void* key_d, const void* id_d, Le
       } else {
         hipLaunchKernelGGL((k_lv_plan<T, IS, true>), dim3(P.nq), dim3(kWave), 0, s, P);
       }
-      Status st = exclusive_scan(sc.wq, sc.woff, a.nq, s);
-      if (!st.ok()) return st;
-      const uint64_t mark_threads = static_cast<uint64_t>(P.nq) * kTaskCap;
-      hipLaunchKernelGGL(k_lv_mark, dim3(static_cast<unsigned>((mark_threads + kLvBlock - 1) / kLvBlock)),
-                         dim3(kLvBlock), 0, s, sc.woff, sc.tpre, sc.ntask, P.nq, sc.chunk_task);
+      if (!(P.use_single && plan != 2)) {  // a single query's plan wrote these itself
+        Status st = exclusive_scan(sc.wq, sc.woff, a.nq, s);
+        if (!st.ok()) return st;
+        const uint64_t mark_threads = static_cast<uint64_t>(P.nq) * kTaskCap;
+        hipLaunchKernelGGL(k_lv_mark, dim3(static_cast<unsigned>((mark_threads + kLvBlock - 1) / kLvBlock)),
+                           dim3(kLvBlock), 0, s, sc.woff, sc.tpre, sc.ntask, P.nq, sc.chunk_task);
+      }
       int dev = 0, cus = 0;
       hipGetDevice(&dev);
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
