@@ -252,3 +252,50 @@ def test_single_searches_through_the_index(torch, P, t, i):
         assert len(np.unique(w)) == cap and np.all(np.isin(w, exp["id"]))
         assert np.all(out.cpu().numpy().view(np.uint8)[cap * isz:] == 0xFF)
     tr.close()
+
+
+@pytest.mark.parametrize("index", [True, False])
+def test_gapped_offsets_and_short_output(torch, P, index):
+    """Writes land at the caller's offsets (gaps between queries untouched),
+    and a one-call batch whose output holds fewer IDs than the batch
+    reports writes nothing at or past the capacity, while the queries that
+    fit are complete; the same with the index and with the walk."""
+    rng = np.random.default_rng(31)
+    n = 40000
+    p = _points(rng, "f32", "u32", n, False)
+    tr = P.StaticPSTGPU(P.to_device(p).data_ptr(), n, "f32", "u32")
+    tr.set_level_index(index)
+    qs = _queries(rng, P, "f32", p, 48)
+    nq = len(qs)
+    qd = P.to_device(qs)
+    exp = _expect(p, qs)
+    cnt = np.array([len(e) for e in exp], np.int64)
+    offs = (np.concatenate([[0], np.cumsum(cnt)[:-1]]) + 7 * np.arange(nq)).astype(np.int64)
+    total = int(offs[-1] + cnt[-1] + 7)
+    out = torch.full((total,), -5, dtype=torch.int32, device="cuda")
+    tr.batch_write(qd, nq, torch.from_numpy(offs).cuda(), out, ret=P.RET_ID)
+    torch.cuda.synchronize()
+    r = out.cpu().numpy()
+    mask = np.ones(total, bool)
+    for k in range(nq):
+        assert np.array_equal(np.sort(r[offs[k]:offs[k] + cnt[k]].view(np.uint32)), exp[k]), k
+        mask[offs[k]:offs[k] + cnt[k]] = False
+    assert np.all(r[mask] == -5)
+    # one call with a short output: the last queries are cut at the capacity
+    full = int(cnt.sum())
+    cap = full - int(cnt[-1]) // 2 - 3
+    counts = torch.zeros(nq, dtype=torch.int64, device="cuda")
+    offd = torch.zeros(nq + 1, dtype=torch.int64, device="cuda")
+    out2 = torch.full((full + 64,), -5, dtype=torch.int32, device="cuda")
+    tr.batch_search(qd, nq, counts, offd, out2, cap)
+    torch.cuda.synchronize()
+    c, o, r2 = counts.cpu().numpy(), offd.cpu().numpy(), out2.cpu().numpy()
+    assert np.array_equal(c, cnt) and o[nq] == full
+    assert np.all(r2[cap:] == -5)
+    for k in range(nq):
+        if o[k] + c[k] <= cap:
+            assert np.array_equal(np.sort(r2[o[k]:o[k] + c[k]].view(np.uint32)), exp[k]), k
+        else:
+            seg = r2[o[k]:cap].view(np.uint32)
+            assert len(np.unique(seg)) == len(seg) and np.all(np.isin(seg, exp[k])), k
+    tr.close()
