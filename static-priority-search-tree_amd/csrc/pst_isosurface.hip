@@ -32,7 +32,7 @@ namespace pst {
 namespace {
 
 constexpr int kIsoWave = 64;
-constexpr int kIsoWavesPerCu = 16;
+constexpr int kIsoWavesPerCu = 24;  // measured on 1.6M metacells: 24 beats 16 and 32
 
 // interpolation precision: float for float volumes, double otherwise
 template <typename T>
