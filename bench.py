@@ -93,13 +93,19 @@ def iso_bench(args, P, D, torch, dev, stream):
     out = torch.empty(total + 1, dtype=torch.int32, device=dev)
     forest.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
     torch.cuda.synchronize()
+    # the batch on a second handle with autotuned walk settings (the first
+    # keeps the defaults for the plain path below)
+    tf = P.StaticPSTGPUArr(mc.data_ptr(), nmc, 64, "f32", "u32")
+    tuned = tf.autotune(d_q, nq, stream=stream)
+    tf.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
     reps = 3
     ev[4].record(stream)
     for _ in range(reps):
-        forest.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
+        tf.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
     ev[5].record(stream)
     torch.cuda.synchronize()
     batch_ms = ev[4].elapsed_time(ev[5]) / reps
+    tf.close()
     ok = int(offsets[nq].item()) == total
     # spot check against the metacells themselves
     cells = mc.cpu().numpy().view(rec)
@@ -172,6 +178,7 @@ def iso_bench(args, P, D, torch, dev, stream):
         "forest_build_ms": round(forest_ms, 2),
         "batch_ms": round(batch_ms, 3),
         "batch_qps": round(nq / (batch_ms * 1e-3), 1),
+        "batch_autotuned": tuned,
         "reported_ids": total,
         "plain_qps": round(plain_qps, 1),
         "ips_qps": round(1e3 / ips_ms, 1),
