@@ -346,9 +346,11 @@ int pst_gpu_set_query_grouping(pst_tree* tree, int queries_per_walk);
  * returning pst_gpu_batch_write / pst_gpu_batch_search (and the batch
  * graphs made from them), single searches and pst_gpu_search_into walk
  * only the query's boundary paths and stream the IDs; results are the same
- * sets as without it (order within a query's results differs). Trees of fewer than 2^31 points; not for a
- * forest or a tree without IDs (PST_ERR_INVALID). Device memory:
- * pst_gpu_level_index_bytes. */
+ * sets as without it (order within a query's results differs). Trees of
+ * fewer than 2^31 points; not for a forest or a tree without IDs
+ * (PST_ERR_INVALID). Device memory: pst_gpu_level_index_bytes, plus
+ * per-handle batch scratch of about 1.5 KB per query of the largest batch
+ * written (128 ranges of 12 bytes). */
 int pst_gpu_set_level_index(pst_tree* tree, int on);
 /* Bytes of the level index for n points (bit_length(n) levels of n key/ID
  * pairs, each level padded to a multiple of 64); 0 for n = 0 or a tree
