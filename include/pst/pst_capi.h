@@ -294,9 +294,11 @@ int pst_gpu_batch_write(pst_tree* tree, const void* queries_d, size_t num_querie
 /* Count, scan and write in one asynchronous call: counts_d[q] (nq
  * entries) and offsets_d (nq + 1 entries, offsets_d[nq] = total) are
  * written on the device; results go to out_d, of which only the first
- * out_capacity records are written (compare offsets_d[nq] with it). No host
- * synchronisation and no allocation after the first call for a batch size,
- * so the call can be captured into a HIP graph. */
+ * out_capacity records are written (compare offsets_d[nq] with it).
+ * out_capacity = 0 is a sizing call: counts and offsets only, out_d is not
+ * touched (it may be NULL). No host synchronisation and no allocation after
+ * the first call for a batch size, so the call can be captured into a HIP
+ * graph. */
 int pst_gpu_batch_search(pst_tree* tree, const void* queries_d, size_t num_queries, int ret,
                          unsigned long long* counts_d, unsigned long long* offsets_d, void* out_d,
                          size_t out_capacity, pst_stream_t stream);
@@ -316,7 +318,13 @@ int pst_gpu_autotune(pst_tree* tree, const void* queries_d, size_t num_queries, 
  * reference). Runs the batch once eagerly (all allocation happens then),
  * then captures it on `stream` (not the null stream). At most 8192 queries:
  * such batches are grouped by one kernel, with no library call. Replays read
- * the current contents of queries_d and rewrite counts_d, offsets_d, out_d. */
+ * the current contents of queries_d and rewrite counts_d, offsets_d, out_d.
+ * The graph holds the addresses of the tree's search buffers and level
+ * index. While any graph of a tree exists, the calls on that tree that would
+ * free or reallocate them fail with PST_ERR_INVALID and change nothing: a
+ * batch larger than the buffers hold, pst_gpu_autotune,
+ * pst_gpu_set_level_index(tree, 0) and pst_gpu_destroy. Destroy the graphs
+ * first. */
 typedef struct pst_graph pst_graph;
 int pst_gpu_batch_graph_create(pst_tree* tree, const void* queries_d, size_t num_queries, int ret,
                                unsigned long long* counts_d, unsigned long long* offsets_d, void* out_d,
@@ -381,6 +389,9 @@ int pst_server_step(pst_server* server);
 int pst_server_poll(pst_server* server, unsigned long long* tickets, size_t capacity, size_t* num);
 int pst_server_result(pst_server* server, unsigned long long ticket, void** res_d, size_t* num_res_elems);
 int pst_server_release(pst_server* server, unsigned long long ticket);
+/* Create the server after building the trees' level indexes: creation
+ * sizes every tree's search buffers for max_batch queries (so serving does
+ * not allocate), and an index built later sizes its own on first use. */
 /* Tickets submitted and not yet released. */
 size_t pst_server_outstanding(const pst_server* server);
 const char* pst_server_last_error(void);

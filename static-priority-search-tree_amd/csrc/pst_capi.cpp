@@ -32,6 +32,7 @@ struct pst_tree {
   pst::LevelScratch lsc;
   hipStream_t stream = nullptr;
   unsigned long long* debug = nullptr;  // per-round search counters (debug builds)
+  int graphs = 0;  // live batch graphs: they hold the scratch and index addresses
 };
 
 namespace {
@@ -291,6 +292,8 @@ int pst_gpu_create(const void* pt_arr_d, size_t n, int dtype, int id_dtype, int 
 
 int pst_gpu_destroy(pst_tree* t) {
   if (!t) return PST_OK;
+  if (t->graphs > 0)
+    return fail(PST_ERR_INVALID, "Error: the tree still has batch graphs; destroy them before the tree");
   hipError_t e = hipSuccess;
   if (t->nodes) e = hipFree(t->nodes);
   if (t->submin) {
@@ -600,6 +603,8 @@ int pst_gpu_batch_search(pst_tree* t, const void* queries_d, size_t nq, int ret,
     a.counts = counts_d;
     if (st.ok()) st = pst::level_search(a, t->lv_key, t->lv_id, t->lsc, s, 1);
     if (st.ok()) st = pst::exclusive_scan(counts_d, offsets_d, nq, s);
+    if (!st.ok()) return fail(st);
+    if (out_capacity == 0) return PST_OK;  // a sizing call: counts and offsets only
     a.mode = 1;
     a.ret_point = ret;
     a.counts = t->sc.cursor;
@@ -615,7 +620,7 @@ int pst_gpu_batch_search(pst_tree* t, const void* queries_d, size_t nq, int ret,
   if (rc != PST_OK) return rc;
   Status st = pst::exclusive_scan(counts_d, offsets_d, nq, s);
   if (!st.ok()) return fail(st);
-  if (t->n == 0) return PST_OK;
+  if (t->n == 0 || out_capacity == 0) return PST_OK;  // nothing to write (a sizing call: counts and offsets only)
   st = pst::scratch_reserve(t->sc, nq, 0, s);
   if (!st.ok()) return fail(st);
   pst::SearchArgs a = base_args(t);
@@ -645,6 +650,7 @@ int pst_gpu_set_search_tuning(pst_tree* t, int rounds, int budget_small, int bud
 
 int pst_gpu_autotune(pst_tree* t, const void* queries_d, size_t nq, int ret, pst_stream_t stream, int* chosen6) {
   if (!t || !queries_d || nq == 0) return fail(PST_ERR_INVALID, "Error: autotuning needs a tree and queries");
+  if (t->graphs > 0) return fail(PST_ERR_INVALID, pst::kPinnedScratchMsg);
   if (ret != PST_RET_POINT && t->id_dtype == PST_NO_ID)
     return fail(PST_ERR_INVALID, "Error: IDs requested from a tree built without IDs");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -732,6 +738,7 @@ int pst_gpu_autotune(pst_tree* t, const void* queries_d, size_t nq, int ret, pst
 struct pst_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  pst_tree* tree = nullptr;
 };
 
 int pst_gpu_batch_graph_create(pst_tree* t, const void* queries_d, size_t nq, int ret, unsigned long long* counts_d,
@@ -739,6 +746,7 @@ int pst_gpu_batch_graph_create(pst_tree* t, const void* queries_d, size_t nq, in
                                pst_graph** out) {
   if (!out) return fail(PST_ERR_INVALID, "Error: null output handle");
   *out = nullptr;
+  if (!t) return fail(PST_ERR_INVALID, "Error: null tree");
   if (!stream) return fail(PST_ERR_INVALID, "Error: graph capture needs a stream other than the null stream");
   if (nq > 8192) return fail(PST_ERR_INVALID, "Error: captured batches hold at most 8192 queries");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -764,6 +772,11 @@ int pst_gpu_batch_graph_create(pst_tree* t, const void* queries_d, size_t nq, in
     delete pg;
     return fail(pst::hip_status(e, "Error in instantiating the batch graph"));
   }
+  pg->tree = t;
+  // the graph holds the addresses of the handle's buffers: from now on a
+  // call that would reallocate them fails instead (until the last graph goes)
+  t->graphs++;
+  t->sc.pinned = t->lsc.pinned = 1;
   *out = pg;
   return PST_OK;
 }
@@ -778,6 +791,7 @@ int pst_gpu_batch_graph_destroy(pst_graph* g) {
   if (!g) return PST_OK;
   if (g->exec) hipGraphExecDestroy(g->exec);
   if (g->graph) hipGraphDestroy(g->graph);
+  if (g->tree && --g->tree->graphs == 0) g->tree->sc.pinned = g->tree->lsc.pinned = 0;
   delete g;
   return PST_OK;
 }
@@ -800,6 +814,8 @@ size_t pst_gpu_level_index_bytes(size_t n, int dtype, int id_dtype) {
 int pst_gpu_set_level_index(pst_tree* t, int on) {
   if (!t) return fail(PST_ERR_INVALID, "Error: null tree");
   if (!on) {
+    if (t->graphs > 0 && t->lv_key)
+      return fail(PST_ERR_INVALID, "Error: a batch graph of this tree uses its level index; destroy the graph first");
     hipSetDevice(t->device);
     if (t->lv_key) hipFree(t->lv_key);
     if (t->lv_id) hipFree(t->lv_id);

@@ -79,7 +79,12 @@ struct SearchScratch {
   size_t gstack_bytes = 0;
   void* tasks = nullptr;  // copy tasks of write passes (16-byte entries)
   size_t task_cap = 0;
+  // batch graphs of the handle hold these buffers' addresses: while any
+  // exists, a call that would reallocate them fails instead
+  int pinned = 0;
 };
+// Message of the error a pinned scratch gives for a call that needs more.
+extern const char* const kPinnedScratchMsg;
 // Work lists for nq queries whose walks start from `roots` root items.
 Status scratch_reserve(SearchScratch& sc, size_t nq, size_t roots, hipStream_t stream);
 void scratch_release(SearchScratch& sc);
@@ -112,7 +117,7 @@ struct SearchArgs {
   int groups_ready;     // the group table in the scratch already holds this batch
   const void* single_query;  // host Query<T> for one query (queries ignored)
   unsigned long long* debug;  // optional per-round counters (8 per round)
-  unsigned long long out_total;  // records in `out` (0: unlimited)
+  unsigned long long out_total = ~0ull;  // records `out` holds (~0: unlimited; 0: nothing is written)
 };
 Status run_search(const SearchArgs& a, SearchScratch& sc, hipStream_t stream);
 unsigned long long* search_error_flag(SearchScratch& sc);
@@ -135,6 +140,8 @@ struct LevelScratch {
   unsigned long long* woff = nullptr;  // their exclusive prefix (nq + 1)
   uint32_t* chunk_task = nullptr;      // copy chunk -> first range
   size_t cap = 0;                      // queries the buffers hold
+  int pinned = 0;                      // as SearchScratch::pinned
+  int external = 0;                    // the buffers belong to the caller (never freed or grown here)
 };
 Status level_scratch_reserve(LevelScratch& sc, size_t nq);
 void level_scratch_release(LevelScratch& sc);

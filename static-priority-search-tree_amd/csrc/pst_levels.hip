@@ -707,6 +707,8 @@ Status build_level_index(int dtype, int id_dtype, size_t n, const void* nodes_d,
 
 Status level_scratch_reserve(LevelScratch& sc, size_t nq) {
   if (sc.cap >= nq && sc.chunk_task) return Status{};
+  if (sc.external) return Status{1, "Error: the batch holds more queries than the search buffers"};
+  if (sc.pinned) return Status{1, kPinnedScratchMsg};
   level_scratch_release(sc);
   const size_t cap = std::max<size_t>(nq, 64);
   hipError_t e = hipMalloc(&sc.tsrc, cap * kTaskCap * 8);
@@ -724,6 +726,10 @@ Status level_scratch_reserve(LevelScratch& sc, size_t nq) {
 }
 
 void level_scratch_release(LevelScratch& sc) {
+  if (sc.external) {
+    sc = LevelScratch{};
+    return;
+  }
   if (sc.tsrc) hipFree(sc.tsrc);
   if (sc.tpre) hipFree(sc.tpre);
   if (sc.ntask) hipFree(sc.ntask);
@@ -766,7 +772,7 @@ Status level_search(const SearchArgs& a, const void* key_d, const void* id_d, Le
       P.offsets = a.offsets;
       P.caps = a.caps;
       P.cap_all = a.cap_all;
-      P.out_total = a.out_total ? a.out_total : ~0ull;
+      P.out_total = a.out_total;
       P.tsrc = sc.tsrc;
       P.tpre = sc.tpre;
       P.ntask = sc.ntask;

@@ -960,7 +960,7 @@ Status launch_rounds(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
   P.offsets = a.offsets;
   P.caps = a.caps;
   P.cap_all = a.cap_all;
-  P.out_total = a.out_total ? a.out_total : ~0ull;
+  P.out_total = a.out_total;
   P.out = a.out;
   P.err = sc.ctr + 2 * kMaxRounds + 2;
   P.dbg = a.debug;
@@ -996,6 +996,7 @@ Status launch_rounds(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
   P.total_waves = grid;
   const size_t gbytes = static_cast<size_t>(grid) * P.gcap * sizeof(Items::type);
   if (sc.gstack_bytes < gbytes) {
+    if (sc.pinned) return Status{1, kPinnedScratchMsg};
     if (sc.gstack) hipFree(sc.gstack);
     sc.gstack = nullptr;
     sc.gstack_bytes = 0;
@@ -1007,6 +1008,7 @@ Status launch_rounds(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
   if (W && P.copy_max > 0) {
     const size_t want = a.copy_tasks > 0 ? std::min(a.copy_tasks, kCopyTaskCap) : kCopyTaskCap;
     if (sc.task_cap < want) {
+      if (sc.pinned) return Status{1, kPinnedScratchMsg};
       if (sc.tasks) hipFree(sc.tasks);
       sc.tasks = nullptr;
       sc.task_cap = 0;
@@ -1088,9 +1090,16 @@ Status make_groups(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
 
 }  // namespace
 
+const char* const kPinnedScratchMsg =
+    "Error: a batch graph of this tree uses its search buffers, which this call would reallocate; "
+    "destroy the graph first";
+
 Status scratch_reserve(SearchScratch& sc, size_t nq, size_t roots, hipStream_t) {
   const size_t spill_cap = 1ull << 22;  // 4M items per round list
   const size_t need = std::max(std::max(nq, roots), spill_cap);
+  const bool grow = sc.list_cap < need || !sc.ctr || sc.cursor_cap < std::max<size_t>(nq, 1) ||
+                    (nq > 1 && sc.gcap < nq);
+  if (grow && sc.pinned) return Status{1, kPinnedScratchMsg};
   if (sc.list_cap < need) {
     if (sc.list[0]) hipFree(sc.list[0]);
     if (sc.list[1]) hipFree(sc.list[1]);

@@ -340,6 +340,11 @@ int pst_server_create(pst_tree* const* trees, int num_trees, size_t out_budget_b
                              : PST_ERR_HIP;
     if (rc == PST_OK) rc = pst_gpu_batch_write(tq.tree, tq.slots[0].q_dev, max_batch, PST_RET_POINT,
                                                tq.slots[0].off_dev, tq.slots[0].cnt_dev, s->pool_dev, tq.stream);
+    // ID writes of a tree with a level index use that index's own search
+    // buffers: size them too
+    if (rc == PST_OK && idt != PST_NO_ID)
+      rc = pst_gpu_batch_write(tq.tree, tq.slots[0].q_dev, max_batch, PST_RET_ID, tq.slots[0].off_dev,
+                               tq.slots[0].cnt_dev, s->pool_dev, tq.stream);
     if (rc == PST_OK && hipStreamSynchronize(tq.stream) != hipSuccess) rc = PST_ERR_HIP;
     if (rc != PST_OK) {
       const std::string m = std::string("Error in warming up the server: ") + pst_last_error();

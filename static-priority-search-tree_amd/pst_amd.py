@@ -343,7 +343,7 @@ class StaticPSTGPU:
         _check(lib().pst_gpu_batch_graph_create(self.h, queries_d.data_ptr(), nq, rp, counts_d.data_ptr(),
                                                 offsets_d.data_ptr(), out_d.data_ptr(), out_capacity,
                                                 _stream_ptr(stream), ctypes.byref(g)))
-        return BatchGraph(g)
+        return BatchGraph(g, self)
 
     def set_subtree_copy(self, copy_min: int, copy_max: int, max_tasks: int = 0):
         """Write pass: copy fully reported subtrees of copy_min..copy_max points
@@ -389,16 +389,27 @@ class StaticPSTGPU:
 
 
 class BatchGraph:
-    def __init__(self, g):
+    """A captured batch. It keeps its tree alive: the tree's search buffers
+    and level index cannot be freed or reallocated while a graph exists."""
+
+    def __init__(self, g, tree=None):
         self.g = g
+        self.tree = tree
 
     def launch(self, stream):
         _check(lib().pst_gpu_batch_graph_launch(self.g, _stream_ptr(stream)))
 
     def close(self):
         if getattr(self, "g", None):
-            lib().pst_gpu_batch_graph_destroy(self.g)
+            _check(lib().pst_gpu_batch_graph_destroy(self.g))
             self.g = None
+        self.tree = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class StaticPSTGPUArr(StaticPSTGPU):

@@ -60,8 +60,9 @@ def _read(P, ptr, n, rec):
     return P._d2h(ptr, n * rec.itemsize)[:n * rec.itemsize].view(rec) if n else np.zeros(0, rec)
 
 
+@pytest.mark.parametrize("index", [False, True])
 @pytest.mark.parametrize("budget", [1 << 30, 3 << 20])
-def test_server_results_equal_alone(torch, P, budget):
+def test_server_results_equal_alone(torch, P, budget, index):
     rng = np.random.default_rng(11)
     specs = [("f32", "u32", 1 << 15, False), ("i32", "u32", 50000, False), ("f64", "u64", 20000, True)]
     trees, pts, keep = [], [], []
@@ -69,6 +70,8 @@ def test_server_results_equal_alone(torch, P, budget):
         p = _pts(rng, t, i, n)
         d = P.to_device(p)
         tr = P.StaticPSTGPUArr(d, n, 64, t, i) if forest else P.StaticPSTGPU(d, n, t, i)
+        if index and not forest:
+            tr.set_level_index(True)  # before the server, which sizes the index's buffers
         trees.append(tr)
         pts.append(p)
         keep.append(d)
@@ -77,7 +80,7 @@ def test_server_results_equal_alone(torch, P, budget):
     want = {}
     done = {}
     nq = 400
-    print(f"server budget={budget} nq={nq}", flush=True)
+    print(f"server budget={budget} nq={nq} level index={index}", flush=True)
     for k in range(nq):
         ti = int(rng.integers(0, len(trees)))
         t, i = specs[ti][0], specs[ti][1]
@@ -144,3 +147,64 @@ def test_server_query_larger_than_budget_fails_alone(torch, P):
     srv.release(t_small)
     srv.close()
     tr.close()
+
+
+def test_server_level_index_trees_to_2e22(torch, P):
+    """The measured serving setup: deep trees of 2^20..2^22 points with the
+    level index on every tree, float and int32 trees, queries arriving
+    while earlier batches run, an output budget that makes admission wait.
+    Every result equals brute force."""
+    import pst_data as D
+    trees, pts, mixes = [], [], []
+    for k, lg in enumerate((20, 21, 22)):
+        n = 1 << lg
+        if k % 2 == 0:
+            p, t = D.interval_points(n, seed=300 + lg), "f32"
+            qs = D.float_query_mix(64, 64, seed=400 + lg)
+        else:
+            p, t = D.int_points(n, seed=300 + lg), "i32"
+            qs = D.int_query_mix(128, seed=400 + lg)
+        tr = P.StaticPSTGPU(P.to_device(p), n, t, "u32")
+        tr.set_level_index(True)
+        trees.append(tr)
+        pts.append(p)
+        mixes.append(qs)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(9)
+    # 96 MB of output: the larger results of the float trees make the
+    # admission wait for earlier tickets' release
+    srv = P.Server(trees, 96 << 20, max_batch=128)
+    want, done = {}, {}
+    nq = 240
+    print(f"server level index 2^20..2^22 nq={nq}", flush=True)
+    for k in range(nq):
+        ti = int(rng.integers(0, 3))
+        q = mixes[ti][int(rng.integers(0, len(mixes[ti])))]
+        want[srv.submit(ti, q)] = (ti, q)
+        if k % 5 == 0:
+            srv.step()
+            for d in srv.poll():
+                r = srv.result(d)
+                done[d] = _read(P, *r, np.dtype(np.uint32)).copy()
+                srv.release(d)
+    guard = 0
+    while len(done) < nq:
+        srv.step()
+        for d in srv.poll():
+            r = srv.result(d)
+            done[d] = _read(P, *r, np.dtype(np.uint32)).copy()
+            srv.release(d)
+        guard += 1
+        assert guard < 5_000_000, "server made no progress"
+    srv.close()
+    for tk, (ti, q) in want.items():
+        p = pts[ti]
+        kind = int(q["kind"])
+        m = p["dim2"] >= q["min_dim2"]
+        if kind != 2:
+            m &= p["dim1"] <= q["max_dim1"]
+        if kind != 1:
+            m &= p["dim1"] >= q["min_dim1"]
+        assert np.array_equal(np.sort(done[tk]), np.flatnonzero(m).astype(np.uint32)), (tk, ti, kind)
+    for tr in trees:
+        tr.close()
