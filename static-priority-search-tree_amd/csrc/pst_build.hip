@@ -53,10 +53,12 @@ struct IdStore<NoId> {
 // descending index, as the reference's comparisonTiebreaker does for
 // compareDim2(...) > 0.
 // ---------------------------------------------------------------------------
-template <typename T, typename I, typename K, bool DIM2, bool DESC, bool WITH_ID>
+// ORDER_CHECK: also sets flags[1] when some point's ID is above the next
+// point's (IDs not in index order), and flags[0] for a NaN ID.
+template <typename T, typename I, typename K, bool DIM2, bool DESC, bool WITH_ID, bool ORDER_CHECK = false>
 __global__ void k_keys_single(const Point<T, I>* __restrict__ pts, uint32_t n, K* __restrict__ keys,
                               uint32_t* __restrict__ vals, const uint32_t* __restrict__ r2,
-                              int* __restrict__ nan_flag) {
+                              int* __restrict__ flags) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint32_t i = DESC ? n - 1 - j : j;
@@ -70,29 +72,33 @@ __global__ void k_keys_single(const Point<T, I>* __restrict__ pts, uint32_t n, K
   } else {
     k = static_cast<K>(ord_key(d));
   }
-  if (bad) *nan_flag = 1;
+  if constexpr (ORDER_CHECK) {
+    bad |= is_nan(p.id);
+    if (i + 1 < n && ord_key(p.id) > ord_key(pts[i + 1].id)) flags[1] = 1;
+  }
+  if (bad) flags[0] = 1;
   keys[j] = DESC ? ~k : k;
   vals[j] = r2 ? r2[i] : i;
 }
 
 // Two-pass keys (when dim or id is 8 bytes): pass A orders by id.
-template <typename T, typename I, bool DESC>
-__global__ void k_keys_id(const Point<T, I>* __restrict__ pts, uint32_t n, uint64_t* __restrict__ keys,
+template <typename T, typename I, typename K, bool DESC>
+__global__ void k_keys_id(const Point<T, I>* __restrict__ pts, uint32_t n, K* __restrict__ keys,
                           uint32_t* __restrict__ vals, int* __restrict__ nan_flag) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint32_t i = DESC ? n - 1 - j : j;
   const I id = pts[i].id;
   if (is_nan(id)) *nan_flag = 1;
-  const uint64_t k = static_cast<uint64_t>(ord_key(id));
-  keys[j] = DESC ? ~k : k;
+  const K k = static_cast<K>(ord_key(id));
+  keys[j] = DESC ? static_cast<K>(~k) : k;
   vals[j] = i;
 }
 
 // Pass B: orders by dim, visiting points in pass-A order.
-template <typename T, typename I, bool DIM2, bool DESC>
+template <typename T, typename I, typename K, bool DIM2, bool DESC>
 __global__ void k_keys_dim_perm(const Point<T, I>* __restrict__ pts, uint32_t n,
-                                const uint32_t* __restrict__ perm, uint64_t* __restrict__ keys,
+                                const uint32_t* __restrict__ perm, K* __restrict__ keys,
                                 uint32_t* __restrict__ vals, const uint32_t* __restrict__ r2,
                                 int* __restrict__ nan_flag) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -100,8 +106,8 @@ __global__ void k_keys_dim_perm(const Point<T, I>* __restrict__ pts, uint32_t n,
   const uint32_t i = perm[j];
   const T d = DIM2 ? pts[i].dim2_val : pts[i].dim1_val;
   if (is_nan(d)) *nan_flag = 1;
-  const uint64_t k = static_cast<uint64_t>(ord_key(d));
-  keys[j] = DESC ? ~k : k;
+  const K k = static_cast<K>(ord_key(d));
+  keys[j] = DESC ? static_cast<K>(~k) : k;
   vals[j] = r2 ? r2[i] : i;
 }
 
@@ -556,9 +562,69 @@ struct DevBuf {
 
 inline unsigned grid_for(size_t n, unsigned block) { return static_cast<unsigned>((n + block - 1) / block); }
 
+// Construction buffers of n points, carved from one allocation: two key
+// arrays, four 32-bit arrays (sort values, the key2 order, the level arrays
+// and the key2 ranks), flags, radix sort storage, level tables.
+struct BuildLayout {
+  size_t key0, key1, v0, v1, v2, r2, flags, tmp, starts, segmin, total;
+  size_t tmp_bytes;
+  uint32_t t0;
+};
+
+inline size_t al256(size_t b) { return (b + 255) & ~size_t{255}; }
+
+// Levels walked by global passes: every segment of level t0 fits in LDS.
+inline uint32_t finish_level(uint64_t M) {
+  uint32_t t0 = 0;
+  while (true) {
+    const uint64_t base = M >> t0;
+    const uint64_t r = M & ((1ull << t0) - 1ull);
+    const uint64_t mx = base - 1 + (r ? 1 : 0);
+    if (mx <= kFinishTile) break;
+    t0++;
+  }
+  return t0;
+}
+
+template <typename K>
+BuildLayout build_layout(size_t n) {
+  BuildLayout L{};
+  size_t tb = 0;
+  if constexpr (sizeof(K) == 4) radix_sort_u32(nullptr, tb, nullptr, nullptr, nullptr, nullptr, n, nullptr);
+  else radix_sort_u64(nullptr, tb, nullptr, nullptr, nullptr, nullptr, n, nullptr);
+  L.tmp_bytes = tb;
+  L.t0 = finish_level(static_cast<uint64_t>(n) + 1);
+  const size_t nstart = (1ull << (L.t0 + 1)) - 1;
+  const size_t nmin = L.t0 ? (1ull << L.t0) - 1 : 1;
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = o;
+    o += al256(bytes ? bytes : 1);
+    return at;
+  };
+  L.key0 = take(n * sizeof(K));
+  L.key1 = take(n * sizeof(K));
+  L.v0 = take(n * 4);
+  L.v1 = take(n * 4);
+  L.v2 = take(n * 4);
+  L.r2 = take(n * 4);
+  L.flags = take(16);
+  L.tmp = take(tb);
+  L.starts = take(nstart * 4);
+  L.segmin = take(nmin * 8);
+  L.total = o;
+  return L;
+}
+
+// Sort keys wide enough for a dim or an ID of the point type.
+template <typename T, typename I>
+using BuildKey = typename std::conditional<(sizeof(T) == 8 || (has_id_v<I> && sizeof(I) == 8)), uint64_t,
+                                           uint32_t>::type;
+
 template <typename T, typename I>
 Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, BuildTimes* times) {
   using IS = typename IdStore<I>::type;
+  using K = BuildKey<T, I>;
   Status st;
   if (n64 == 0) return st;
   if (n64 >= (1ull << 32) - 1) {
@@ -572,8 +638,6 @@ Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, 
   const auto* ptsS = static_cast<const Point<T, IS>*>(pts_v);
   auto* nodes = static_cast<Node<T, IS>*>(nodes_v);
   constexpr bool kHasId = has_id_v<I>;
-  constexpr bool kSingle = !kHasId ? true : (sizeof(T) == 4 && sizeof(I) == 4);
-  using K = typename std::conditional<(!kHasId && sizeof(T) == 4), uint32_t, uint64_t>::type;
 
   hipEvent_t ev[5] = {};
   if (times)
@@ -582,118 +646,100 @@ Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, 
     if (times) hipEventRecord(ev[k], s);
   };
 
-  DevBuf kbuf[2], vbuf[2], perm2b, r2b, flag, tmpb;
   hipError_t e;
 #define PST_TRY(expr, what)                         \
   do {                                              \
     e = (expr);                                     \
     if (e != hipSuccess) return hip_status(e, what); \
   } while (0)
-  PST_TRY(kbuf[0].alloc(n64 * sizeof(K)), "Error in allocating construction key array");
-  PST_TRY(kbuf[1].alloc(n64 * sizeof(K)), "Error in allocating construction key array");
-  PST_TRY(vbuf[0].alloc(n64 * 4), "Error in allocating construction index array");
-  PST_TRY(vbuf[1].alloc(n64 * 4), "Error in allocating construction index array");
-  PST_TRY(perm2b.alloc(n64 * 4), "Error in allocating dimension-2 order array");
-  PST_TRY(r2b.alloc(n64 * 4), "Error in allocating dimension-2 rank array");
-  PST_TRY(flag.alloc(sizeof(int)), "Error in allocating construction flag");
-  PST_TRY(hipMemsetAsync(flag.p, 0, sizeof(int), s), "Error in clearing construction flag");
-
-  size_t tmp_bytes = 0;
-  if constexpr (sizeof(K) == 4)
-    PST_TRY(radix_sort_u32(nullptr, tmp_bytes, nullptr, nullptr, nullptr, nullptr, n, s),
-            "Error in sizing radix sort storage");
-  else
-    PST_TRY(radix_sort_u64(nullptr, tmp_bytes, nullptr, nullptr, nullptr, nullptr, n, s),
-            "Error in sizing radix sort storage");
-  PST_TRY(tmpb.alloc(tmp_bytes), "Error in allocating radix sort storage");
-  auto sortK = [&](int from) -> hipError_t {
-    size_t tb = tmp_bytes;
-    if constexpr (sizeof(K) == 4)
-      return radix_sort_u32(tmpb.p, tb, kbuf[from].as<uint32_t>(), kbuf[1 - from].as<uint32_t>(),
-                            vbuf[from].as<uint32_t>(), vbuf[1 - from].as<uint32_t>(), n, s);
-    else
-      return radix_sort_u64(tmpb.p, tb, kbuf[from].as<uint64_t>(), kbuf[1 - from].as<uint64_t>(),
-                            vbuf[from].as<uint32_t>(), vbuf[1 - from].as<uint32_t>(), n, s);
+  const BuildLayout BL = build_layout<K>(n64);
+  DevBuf arena;
+  PST_TRY(arena.alloc(BL.total), "Error in allocating construction arrays");
+  unsigned char* base = arena.as<unsigned char>();
+  K* key0 = reinterpret_cast<K*>(base + BL.key0);
+  K* key1 = reinterpret_cast<K*>(base + BL.key1);
+  uint32_t* v0 = reinterpret_cast<uint32_t*>(base + BL.v0);
+  uint32_t* v1 = reinterpret_cast<uint32_t*>(base + BL.v1);
+  uint32_t* v2 = reinterpret_cast<uint32_t*>(base + BL.v2);
+  uint32_t* r2 = reinterpret_cast<uint32_t*>(base + BL.r2);
+  int* flags = reinterpret_cast<int*>(base + BL.flags);
+  void* tmp = base + BL.tmp;
+  PST_TRY(hipMemsetAsync(flags, 0, 2 * sizeof(int), s), "Error in clearing construction flags");
+  // stable radix sort of (key, value) pairs: kin/vin to kout/vout
+  auto sortK = [&](const K* kin, K* kout, const uint32_t* vin, uint32_t* vout) -> hipError_t {
+    size_t tb = BL.tmp_bytes;
+    if constexpr (sizeof(K) == 4) return radix_sort_u32(tmp, tb, kin, kout, vin, vout, n, s);
+    else return radix_sort_u64(tmp, tb, kin, kout, vin, vout, n, s);
   };
   const unsigned B = 256;
   const unsigned G = grid_for(n, B);
-  int* nan_flag = flag.as<int>();
+  int* nan_flag = flags;
   mark(0);
-  // ---- key2 descending -> perm2, r2
-  if constexpr (kSingle) {
-    k_keys_single<T, I, K, true, true, kHasId>
-        <<<G, B, 0, s>>>(pts, n, kbuf[0].as<K>(), vbuf[0].as<uint32_t>(), nullptr, nan_flag);
-    PST_TRY(sortK(0), "Error in sorting by dimension 2");
-    PST_TRY(hipMemcpyAsync(perm2b.p, vbuf[1].p, n64 * 4, hipMemcpyDeviceToDevice, s),
-            "Error in copying dimension-2 order");
+  // ---- key2 descending -> perm2 (v1), r2. One sort by dim2 when the IDs
+  // are in index order (the sort's index tie-break is then the (id, index)
+  // tie-break), else a sort by ID and then a stable one by dim2.
+  k_keys_single<T, I, K, true, true, false, kHasId><<<G, B, 0, s>>>(pts, n, key0, v0, nullptr, flags);
+  bool ordered = true;
+  if constexpr (kHasId) {
+    int hflags[2] = {0, 0};
+    PST_TRY(hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost, s), "Error in reading construction flags");
+    PST_TRY(hipStreamSynchronize(s), "Error in synchronising construction");
+    ordered = hflags[1] == 0;
+  }
+  if (ordered) {
+    PST_TRY(sortK(key0, key1, v0, v1), "Error in sorting by dimension 2");
   } else {
     if constexpr (kHasId) {
-      k_keys_id<T, I, true><<<G, B, 0, s>>>(pts, n, kbuf[0].as<uint64_t>(), vbuf[0].as<uint32_t>(), nan_flag);
-      PST_TRY(sortK(0), "Error in sorting by ID");
-      k_keys_dim_perm<T, I, true, true><<<G, B, 0, s>>>(pts, n, vbuf[1].as<uint32_t>(), kbuf[0].as<uint64_t>(),
-                                                       vbuf[0].as<uint32_t>(), nullptr, nan_flag);
-      PST_TRY(sortK(0), "Error in sorting by dimension 2");
-      PST_TRY(hipMemcpyAsync(perm2b.p, vbuf[1].p, n64 * 4, hipMemcpyDeviceToDevice, s),
-              "Error in copying dimension-2 order");
+      k_keys_id<T, I, K, true><<<G, B, 0, s>>>(pts, n, key0, v0, nan_flag);
+      PST_TRY(sortK(key0, key1, v0, v2), "Error in sorting by ID");
+      k_keys_dim_perm<T, I, K, true, true><<<G, B, 0, s>>>(pts, n, v2, key0, v0, nullptr, nan_flag);
+      PST_TRY(sortK(key0, key1, v0, v1), "Error in sorting by dimension 2");
     }
   }
-  k_scatter_rank<<<G, B, 0, s>>>(perm2b.as<uint32_t>(), n, r2b.as<uint32_t>());
-  // ---- key1 ascending, carrying r2 -> A
-  if constexpr (kSingle) {
-    k_keys_single<T, I, K, false, false, kHasId><<<G, B, 0, s>>>(pts, n, kbuf[0].as<K>(), vbuf[0].as<uint32_t>(),
-                                                               r2b.as<uint32_t>(), nan_flag);
-    PST_TRY(sortK(0), "Error in sorting by dimension 1");
+  const uint32_t* perm2 = v1;
+  k_scatter_rank<<<G, B, 0, s>>>(perm2, n, r2);
+  // ---- key1 ascending, carrying r2 -> A (v2)
+  if (ordered) {
+    k_keys_single<T, I, K, false, false, false><<<G, B, 0, s>>>(pts, n, key0, v0, r2, flags);
+    PST_TRY(sortK(key0, key1, v0, v2), "Error in sorting by dimension 1");
   } else {
     if constexpr (kHasId) {
-      k_keys_id<T, I, false><<<G, B, 0, s>>>(pts, n, kbuf[0].as<uint64_t>(), vbuf[0].as<uint32_t>(), nan_flag);
-      PST_TRY(sortK(0), "Error in sorting by ID");
-      k_keys_dim_perm<T, I, false, false><<<G, B, 0, s>>>(pts, n, vbuf[1].as<uint32_t>(), kbuf[0].as<uint64_t>(),
-                                                         vbuf[0].as<uint32_t>(), r2b.as<uint32_t>(), nan_flag);
-      PST_TRY(sortK(0), "Error in sorting by dimension 1");
+      k_keys_id<T, I, K, false><<<G, B, 0, s>>>(pts, n, key0, v0, nan_flag);
+      PST_TRY(sortK(key0, key1, v0, v2), "Error in sorting by ID");
+      k_keys_dim_perm<T, I, K, false, false><<<G, B, 0, s>>>(pts, n, v2, key0, v0, r2, nan_flag);
+      PST_TRY(sortK(key0, key1, v0, v2), "Error in sorting by dimension 1");
     }
   }
   mark(1);
-  // A now in vbuf[1]; vbuf[0] is free for the next level.
-  uint32_t* Acur = vbuf[1].as<uint32_t>();
-  uint32_t* Anxt = vbuf[0].as<uint32_t>();
+  uint32_t* Acur = v2;
+  uint32_t* Anxt = v0;
 
   // ---- level schedule
-  uint32_t t0 = 0;
-  while (true) {
-    const uint64_t base = M >> t0;
-    const uint64_t r = M & ((1ull << t0) - 1ull);
-    const uint64_t mx = base - 1 + (r ? 1 : 0);
-    if (mx <= kFinishTile) break;
-    t0++;
-  }
+  const uint32_t t0 = BL.t0;
   // segment starts of levels 0..t0, slot-indexed (level t at offset 2^t - 1)
   const size_t nstart = (1ull << (t0 + 1)) - 1;
   std::vector<uint32_t> hstarts(nstart);
   for (uint32_t t = 0; t <= t0; t++)
     for (uint32_t j = 0; j < (1u << t); j++)
       hstarts[(1u << t) - 1 + j] = static_cast<uint32_t>(std::max<int64_t>(0, level_start(M, t, j)));
-  DevBuf startsb, segminb;
-  PST_TRY(startsb.alloc(nstart * 4), "Error in allocating level tables");
-  PST_TRY(hipMemcpyAsync(startsb.p, hstarts.data(), nstart * 4, hipMemcpyHostToDevice, s),
+  uint32_t* starts_d = reinterpret_cast<uint32_t*>(base + BL.starts);
+  PST_TRY(hipMemcpyAsync(starts_d, hstarts.data(), nstart * 4, hipMemcpyHostToDevice, s),
           "Error in copying level tables");
   const size_t nmin = t0 ? (1ull << t0) - 1 : 1;
-  PST_TRY(segminb.alloc(nmin * 8), "Error in allocating level minima");
-  k_fill_u64<<<grid_for(nmin, B), B, 0, s>>>(segminb.as<unsigned long long>(), nmin, ~0ull);
-  const uint32_t* starts = startsb.as<uint32_t>();
-  unsigned long long* segmin = segminb.as<unsigned long long>();
+  unsigned long long* segmin = reinterpret_cast<unsigned long long*>(base + BL.segmin);
+  k_fill_u64<<<grid_for(nmin, B), B, 0, s>>>(segmin, nmin, ~0ull);
+  const uint32_t* starts = starts_d;
   for (uint32_t t = 0; t < t0; t++) {
     const uint32_t n_level = n - ((1u << t) - 1);
     const unsigned tiles = grid_for(n_level, kLevelTile);
     k_level_argmin<<<tiles, kLevelBlock, 0, s>>>(Acur, n_level, 1u << t, starts + (1u << t) - 1,
                                                  segmin + (1u << t) - 1);
     k_level_emit<T, IS><<<tiles, kLevelBlock, 0, s>>>(Acur, Anxt, n_level, t, M, starts + (1u << t) - 1,
-                                                      segmin + (1u << t) - 1, perm2b.as<uint32_t>(), ptsS,
-                                                      nodes);
+                                                      segmin + (1u << t) - 1, perm2, ptsS, nodes);
     std::swap(Acur, Anxt);
   }
   mark(2);
-  k_finish<T, IS><<<1u << t0, kFinishBlock, 0, s>>>(Acur, t0, M, starts + (1u << t0) - 1,
-                                                    perm2b.as<uint32_t>(), ptsS, nodes);
+  k_finish<T, IS><<<1u << t0, kFinishBlock, 0, s>>>(Acur, t0, M, starts + (1u << t0) - 1, perm2, ptsS, nodes);
   mark(3);
   PST_TRY(hipGetLastError(), "Error in launching construction kernels");
   int hflag = 0;
