@@ -6,11 +6,18 @@
 // The reference launches a grid-stride loop that, per tile, block-scans
 // the hit flags through cooperative groups and takes one atomicAdd per
 // tile (interval-parallel-search.h:70-117 with
-// helpers/calc-alloc-report-ind-offset.h:46-133). Here: a streaming filter
-// over the AoS array with 16 points per thread per tile (coalesced 12-byte
-// records for f32/u32), wave64 ballot/mbcnt counts, one LDS scan across the
-// block's four waves and one global atomic per 4096-point tile. The
-// output counter is a per-call device word, not a global symbol.
+// helpers/calc-alloc-report-ind-offset.h:46-133). Here every wavefront is
+// an independent streaming filter with no block barriers:
+//  - a wave takes chunks of 64 x 16 consecutive points (grid-stride over
+//    chunks) and issues all 16 record loads of a chunk before testing any,
+//    each load one lane-contiguous run of 64 records;
+//  - per record position, a ballot gives the wave's hits; hits go to a
+//    per-wave stage in LDS at their ballot/mbcnt prefix;
+//  - a full stage is flushed with one atomic on the per-call counter
+//    followed by coalesced stores; a chunk with many hits (dense queries)
+//    reserves its own range and is written directly.
+// So the counter sees about one atomic per 512 results (sparse) or per
+// chunk (dense), not one per tile.
 #include <algorithm>
 
 #include "pst_internal.hpp"
@@ -19,65 +26,93 @@ namespace pst {
 
 namespace {
 
-constexpr int kIpsBlock = 256;
-constexpr int kIpsElems = 16;
-constexpr uint32_t kIpsTile = kIpsBlock * kIpsElems;
+constexpr int kIpsBlock = 256;              // four independent waves per workgroup
+constexpr int kIpsElems = 16;               // records per lane per chunk
+constexpr uint32_t kIpsChunk = kWave * kIpsElems;  // 1024 records per wave chunk
+constexpr uint32_t kIpsStage = 512;         // staged results per wave
+constexpr uint32_t kIpsDirect = 256;        // a chunk with at least this many hits is written directly
 
 template <typename T, typename IS, bool RET_POINT>
 __global__ __launch_bounds__(kIpsBlock) void k_ips(const Point<T, IS>* __restrict__ pts, uint64_t n, T q,
                                                    void* __restrict__ out, unsigned long long cap,
                                                    unsigned long long* __restrict__ counter) {
   using Rec = typename std::conditional<RET_POINT, Point<T, IS>, IS>::type;
-  __shared__ uint32_t wcount[kIpsBlock / kWave];
-  __shared__ unsigned long long tile_base;
-  const uint32_t tid = threadIdx.x;
-  const uint32_t wave = tid / kWave;
+  __shared__ Rec stage_all[kIpsBlock / kWave][kIpsStage];
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x / kWave;
+  Rec* stage = stage_all[wave];
   Rec* o = static_cast<Rec*>(out);
-  for (uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kIpsTile; tile < n;
-       tile += static_cast<uint64_t>(gridDim.x) * kIpsTile) {
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (kIpsBlock / kWave);
+  const uint64_t nchunks = (n + kIpsChunk - 1) / kIpsChunk;
+  uint32_t staged = 0;
+
+  auto rec_of = [](const Point<T, IS>& p) -> Rec {
+    if constexpr (RET_POINT) return p;
+    else return p.id;
+  };
+  auto flush = [&]() {
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(counter, static_cast<unsigned long long>(staged));
+    base = __shfl(base, 0, kWave);
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t i = lane; i < staged; i += kWave)
+      if (base + i < cap) o[base + i] = stage[i];
+    __builtin_amdgcn_wave_barrier();
+    staged = 0;
+  };
+
+  for (uint64_t c = static_cast<uint64_t>(blockIdx.x) * (kIpsBlock / kWave) + wave; c < nchunks; c += waves) {
+    const uint64_t first = c * kIpsChunk;
     Point<T, IS> p[kIpsElems];
-    uint32_t hit = 0;
+    uint64_t hb[kIpsElems];
+    if (first + kIpsChunk <= n) {
+      // full chunk: every load issued before any test
 #pragma unroll
-    for (int e = 0; e < kIpsElems; e++) {
-      const uint64_t i = tile + static_cast<uint64_t>(e) * kIpsBlock + tid;
-      if (i < n) {
-        p[e] = pts[i];
-        if (p[e].dim1_val <= q && q <= p[e].dim2_val) hit |= 1u << e;
-      }
-    }
-    const uint32_t mine = __popc(hit);
-    // exclusive scan of per-thread counts: within the wave by prefix sums
-    // over lanes, across the block's waves through LDS
-    uint32_t incl = mine;
+      for (int e = 0; e < kIpsElems; e++) p[e] = pts[first + static_cast<uint64_t>(e) * kWave + lane];
 #pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      const uint32_t v = __shfl_up(incl, off, kWave);
-      if (static_cast<int>(lane_id()) >= off) incl += v;
-    }
-    if (lane_id() == kWave - 1) wcount[wave] = incl;
-    __syncthreads();
-    uint32_t wbase = 0, total = 0;
+      for (int e = 0; e < kIpsElems; e++) hb[e] = __ballot(p[e].dim1_val <= q && q <= p[e].dim2_val);
+    } else {
 #pragma unroll
-    for (int w = 0; w < kIpsBlock / kWave; w++) {
-      const uint32_t c = wcount[w];
-      wbase += w < static_cast<int>(wave) ? c : 0;
-      total += c;
-    }
-    if (tid == 0) tile_base = total ? atomicAdd(counter, static_cast<unsigned long long>(total)) : 0;
-    __syncthreads();
-    unsigned long long pos = tile_base + wbase + (incl - mine);
-#pragma unroll
-    for (int e = 0; e < kIpsElems; e++) {
-      if (hit & (1u << e)) {
-        if (pos < cap) {
-          if constexpr (RET_POINT) o[pos] = p[e];
-          else o[pos] = p[e].id;
+      for (int e = 0; e < kIpsElems; e++) {
+        const uint64_t i = first + static_cast<uint64_t>(e) * kWave + lane;
+        bool h = false;
+        if (i < n) {
+          p[e] = pts[i];
+          h = p[e].dim1_val <= q && q <= p[e].dim2_val;
         }
-        pos++;
+        hb[e] = __ballot(h);
       }
     }
-    __syncthreads();  // wcount / tile_base reuse
+    uint32_t hits = 0;
+#pragma unroll
+    for (int e = 0; e < kIpsElems; e++) hits += static_cast<uint32_t>(__builtin_popcountll(hb[e]));
+    if (hits == 0) continue;
+    if (hits >= kIpsDirect) {
+      // dense chunk: its own range, written straight from the registers
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(counter, static_cast<unsigned long long>(hits));
+      base = __shfl(base, 0, kWave);
+      uint32_t run = 0;
+#pragma unroll
+      for (int e = 0; e < kIpsElems; e++) {
+        if ((hb[e] >> lane) & 1ull) {
+          const unsigned long long at = base + run + lanes_below(hb[e]);
+          if (at < cap) o[at] = rec_of(p[e]);
+        }
+        run += static_cast<uint32_t>(__builtin_popcountll(hb[e]));
+      }
+      continue;
+    }
+    if (staged + hits > kIpsStage) flush();
+    uint32_t run = staged;
+#pragma unroll
+    for (int e = 0; e < kIpsElems; e++) {
+      if ((hb[e] >> lane) & 1ull) stage[run + lanes_below(hb[e])] = rec_of(p[e]);
+      run += static_cast<uint32_t>(__builtin_popcountll(hb[e]));
+    }
+    staged = run;
   }
+  if (staged) flush();
 }
 
 template <typename F>
@@ -116,11 +151,13 @@ Status ips_search(int dtype, int id_dtype, const void* pts_d, size_t n, int ret_
       __builtin_memcpy(&q, search_val, sizeof(T));
       unsigned grid = num_blocks;
       if (grid == 0) {
+        // four workgroups (16 waves) per CU, fewer when there are fewer chunks
         int dev = 0, cus = 0;
         hipGetDevice(&dev);
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        const uint64_t tiles = (n + kIpsTile - 1) / kIpsTile;
-        grid = static_cast<unsigned>(std::min<uint64_t>(tiles, static_cast<uint64_t>(cus) * 8));
+        const uint64_t chunks = (n + kIpsChunk - 1) / kIpsChunk;
+        const uint64_t blocks = (chunks + kIpsBlock / kWave - 1) / (kIpsBlock / kWave);
+        grid = static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(blocks, static_cast<uint64_t>(cus) * 4)));
       }
       const auto* p = static_cast<const Point<T, IS>*>(pts_d);
       if (ret_point) {

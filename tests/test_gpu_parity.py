@@ -227,6 +227,37 @@ def test_interval_parallel_search_equals_brute_force(torch, P, t, i):
             assert got == want
 
 
+@pytest.mark.parametrize("n", [1, 1000, 1024, 70000, 1 << 20])
+def test_interval_parallel_search_dense_sparse_and_capacity(torch, P, n):
+    """Hit densities that take every path of the filter: chunks where every
+    record hits (written directly), sparse chunks (staged), both mixed in
+    one wave's chunks, bounds equal to stored values; a capacity below the
+    hit count writes only that many results (no duplicates, nothing past
+    it) and still counts every hit."""
+    rng = np.random.default_rng(17)
+    p = np.zeros(n, O.point_dtype("f32", "u32"))
+    a = rng.random(n).astype(np.float32)
+    w = np.where((np.arange(n) // 3000) % 2 == 0, 1.0, 0.001).astype(np.float32)  # dense and sparse runs
+    p["dim1"], p["dim2"], p["id"] = a, a + w, np.arange(n, dtype=np.uint32)
+    d = P.to_device(p)
+    for q in (np.float32(0.5), p["dim1"][n // 2], p["dim2"][n // 3], np.float32(2.0)):
+        exp = np.sort(p["id"][O.brute_force_ips(p, q)])
+        got = np.sort(P.intervalParallelSearch(d, n, q, "f32", "u32"))
+        assert np.array_equal(got, exp), (n, q)
+    q = np.float32(0.5)
+    exp = np.sort(p["id"][O.brute_force_ips(p, q)])
+    cap = len(exp) // 2
+    out = torch.full((cap + 64,), -1, dtype=torch.int32, device="cuda")
+    ctr = torch.zeros(1, dtype=torch.int64, device="cuda")
+    P.ips_into(d, n, q, out, cap, ctr)
+    torch.cuda.synchronize()
+    assert int(ctr.item()) == len(exp)
+    o = out.cpu().numpy().view(np.uint32)
+    head = np.sort(o[:cap])
+    assert len(np.unique(head)) == cap and np.all(np.isin(head, exp))
+    assert np.all(out[cap:].cpu().numpy() == -1)
+
+
 @pytest.mark.parametrize("group,copy", [(8, None), (1, None), (8, (0, 0)), (8, (16, 8191)), (1, (64, 65535)),
                                         (8, (16, 65535, 5)), (8, (256, 2047, 300))])
 def test_batch_search_interval_workload(torch, P, group, copy):
