@@ -21,7 +21,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpst_hip.so")
 
 SOURCES = ["pst_capi.cpp", "pst_server.cpp", "pst_build.hip", "pst_search.hip", "pst_sort.hip", "pst_ips.hip",
-           "pst_metacell.hip", "pst_persist.hip", "pst_isosurface.hip", "pst_levels.hip"]
+           "pst_metacell.hip", "pst_persist.hip", "pst_isosurface.hip", "pst_levels.hip", "pst_forest.hip"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 ARCH = os.environ.get("PST_OFFLOAD_ARCH", "gfx950")

@@ -5,6 +5,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -192,6 +193,28 @@ int blocking_search(pst_tree* t, int kind, int ret, size_t* num, void** res, con
   }
   *res = out;
   return PST_OK;
+}
+
+// Forest batches small enough for the stationary two-pass search
+// (pst_forest.hip) take it instead of per-tree walks.
+bool forest_dense(const pst_tree* t, size_t nq) {
+  return t->slots_per_tree && t->slots_per_tree <= pst::kForestDenseMaxSlots && nq <= pst::kForestDenseMax;
+}
+
+// Grows the handle's per-(query, segment) buffer of forest batches.
+Status forest_seg_reserve(pst_tree* t, size_t nq) {
+  int dev = 0, cus = 0;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const size_t need = std::max<size_t>(pst::forest_dense_scratch_bytes(t->n, t->slots_per_tree, nq, cus), 16);
+  if (t->sc.fseg_bytes >= need) return Status{};
+  if (t->sc.pinned) return Status{1, pst::kPinnedScratchMsg};
+  if (t->sc.fseg) hipFree(t->sc.fseg);
+  t->sc.fseg = nullptr;
+  t->sc.fseg_bytes = 0;
+  Status st = pst::hip_status(hipMalloc(&t->sc.fseg, need), "Error in allocating forest search buffers");
+  if (st.ok()) t->sc.fseg_bytes = need;
+  return st;
 }
 
 }  // namespace
@@ -555,7 +578,13 @@ int pst_gpu_batch_count(pst_tree* t, const void* queries_d, size_t nq, unsigned 
   a.mode = 0;
   a.ret_point = 0;
   a.counts = counts_d;
-  Status st = t->lv_key ? pst::level_search(a, t->lv_key, t->lv_id, t->lsc, s) : pst::run_search(a, t->sc, s);
+  Status st;
+  if (forest_dense(t, nq)) {
+    st = forest_seg_reserve(t, nq);
+    if (st.ok()) st = pst::forest_dense_search(a, t->sc.fseg, s, 0);
+  } else {
+    st = t->lv_key ? pst::level_search(a, t->lv_key, t->lv_id, t->lsc, s) : pst::run_search(a, t->sc, s);
+  }
   return st.ok() ? PST_OK : fail(st);
 }
 
@@ -579,8 +608,13 @@ int pst_gpu_batch_write(pst_tree* t, const void* queries_d, size_t nq, int ret, 
   a.caps = caps_d;
   a.cap_all = ~0ull;
   a.out = out_d;
-  st = (t->lv_key && ret != PST_RET_POINT) ? pst::level_search(a, t->lv_key, t->lv_id, t->lsc, s)
-                                           : pst::run_search(a, t->sc, s);
+  if (forest_dense(t, nq)) {
+    st = forest_seg_reserve(t, nq);
+    if (st.ok()) st = pst::forest_dense_search(a, t->sc.fseg, s, 0);
+  } else {
+    st = (t->lv_key && ret != PST_RET_POINT) ? pst::level_search(a, t->lv_key, t->lv_id, t->lsc, s)
+                                             : pst::run_search(a, t->sc, s);
+  }
   return st.ok() ? PST_OK : fail(st);
 }
 
@@ -592,6 +626,30 @@ int pst_gpu_batch_search(pst_tree* t, const void* queries_d, size_t nq, int ret,
   if (ret != PST_RET_POINT && t->id_dtype == PST_NO_ID)
     return fail(PST_ERR_INVALID, "Error: IDs requested from a tree built without IDs");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (forest_dense(t, nq) && t->n > 0) {
+    // stationary forest passes: the count keeps its segment offsets and the
+    // write step places every hit from them
+    Status st = forest_seg_reserve(t, nq);
+    pst::SearchArgs a = base_args(t);
+    a.queries = queries_d;
+    a.nq = nq;
+    a.mode = 0;
+    a.counts = counts_d;
+    if (st.ok()) st = pst::forest_dense_search(a, t->sc.fseg, s, 1);
+    if (st.ok()) st = pst::exclusive_scan(counts_d, offsets_d, nq, s);
+    if (!st.ok()) return fail(st);
+    if (out_capacity == 0) return PST_OK;
+    a.mode = 1;
+    a.ret_point = ret;
+    a.counts = nullptr;  // the kept segment offsets already give every position
+    a.offsets = offsets_d;
+    a.caps = counts_d;
+    a.cap_all = ~0ull;
+    a.out = out_d;
+    a.out_total = out_capacity;
+    st = pst::forest_dense_search(a, t->sc.fseg, s, 2);
+    return st.ok() ? PST_OK : fail(st);
+  }
   if (t->lv_key && ret != PST_RET_POINT && t->n > 0) {
     // through the level index: the count pass keeps its ranges and the
     // write step only clips them

@@ -79,6 +79,8 @@ struct SearchScratch {
   size_t gstack_bytes = 0;
   void* tasks = nullptr;  // copy tasks of write passes (16-byte entries)
   size_t task_cap = 0;
+  void* fseg = nullptr;  // forest batches: per (query, segment) hit counts, then offsets
+  size_t fseg_bytes = 0;
   // batch graphs of the handle hold these buffers' addresses: while any
   // exists, a call that would reallocate them fails instead
   int pinned = 0;
@@ -123,6 +125,19 @@ Status run_search(const SearchArgs& a, SearchScratch& sc, hipStream_t stream);
 unsigned long long* search_error_flag(SearchScratch& sc);
 // offsets[q] = sum of in[0..q), offsets[n] = total (one workgroup).
 Status exclusive_scan(const unsigned long long* in, unsigned long long* offsets, size_t n, hipStream_t s);
+
+// ---- forest batches with the trees held stationary (pst_forest.hip) ----
+// Batches of at most this many queries on a forest of trees of at most 511
+// points run as two passes over the forest instead of one walk per tree.
+constexpr size_t kForestDenseMax = 512;
+constexpr size_t kForestDenseMaxSlots = 511;
+size_t forest_dense_scratch_bytes(size_t n, size_t slots_per_tree, size_t nq, int cus);
+// mode 0: counts[q]; plan 1 also keeps the per-segment offsets in segbuf.
+// mode 1: writes query q's results at out + offsets[q] (at most caps[q] or
+// cap_all, and within out_total); plan 2 uses the offsets kept by a plan 1
+// count of the same batch, otherwise the pass counts first (and stores each
+// query's total in counts). Asynchronous.
+Status forest_dense_search(const SearchArgs& a, void* segbuf, hipStream_t stream, int plan);
 
 // ---- level index of a deep tree (pst_levels.hip) ----
 // Device bytes of the index: bit_length(n) levels of n (key, ID) pairs,

@@ -1152,6 +1152,7 @@ void scratch_release(SearchScratch& sc) {
   if (sc.gtmp) hipFree(sc.gtmp);
   if (sc.gstack) hipFree(sc.gstack);
   if (sc.tasks) hipFree(sc.tasks);
+  if (sc.fseg) hipFree(sc.fseg);
   sc = SearchScratch{};
 }
 

@@ -3,6 +3,8 @@ ABI): each tree equals the oracle's tree of its input chunk (and the
 reference's own forest fixture), and searches report exactly the points
 that satisfy the predicate, including the right and three-sided searches
 that the reference leaves as stubs."""
+from collections import Counter
+
 import numpy as np
 import pytest
 
@@ -159,3 +161,104 @@ def test_forest_calc_global_mem_needed(P):
     m = P.StaticPSTGPUArr.calcGlobalMemNeeded(n, tpb, "f32", "u32")
     trees = (n + 126) // 127
     assert m >= trees * 128 * 16 + n * 12
+
+
+STATIONARY_CASES = [("f32", "u32", 64, 5000), ("f32", "u32", 1, 300), ("f32", "u32", 31, 3001),
+                    ("i32", "u32", 100, 20000), ("f64", None, 255, 9000), ("i64", "i64", 128, 4099),
+                    ("f32", "u64", 256, 7777), ("f64", "f32", 64, 2000), ("f32", "u32", 64, 127),
+                    ("f32", "u32", 64, 1), ("f32", "u32", 1024, 6000)]
+
+
+def _query_batch(rng, t, nq, lo, hi):
+    import pst_amd as P
+    qs = np.zeros(nq, P.query_dtype(t))
+    a = rng.uniform(lo, hi, nq)
+    b = a + rng.exponential((hi - lo) / 4, nq)
+    c = rng.uniform(lo, hi, nq)
+    if t in ("i32", "i64"):
+        a, b, c = a.astype(np.int64), b.astype(np.int64), c.astype(np.int64)
+    qs["min_dim1"], qs["max_dim1"], qs["min_dim2"] = a, b, c
+    qs["kind"] = rng.integers(0, 4, nq)  # kind 3 searches as a right search
+    if t in ("f32", "f64") and nq > 8:
+        qs["min_dim1"][0] = np.nan
+        qs["max_dim1"][1] = np.nan
+        qs["min_dim2"][2] = np.nan
+        qs["min_dim2"][3] = -np.inf
+        qs["max_dim1"][3] = np.inf
+        qs["kind"][3] = 1
+    return qs
+
+
+def _expect_batch(p, qs):
+    out = []
+    for q in qs:
+        kind = int(q["kind"])
+        out.append(O.brute_force(p, kind if kind < 3 else 2, q["min_dim1"], q["max_dim1"], q["min_dim2"]))
+    return out
+
+
+@pytest.mark.parametrize("t,i,tpb,n", STATIONARY_CASES)
+def test_forest_stationary_batches(torch, P, t, i, tpb, n):
+    """Forest batches of up to 512 queries (two passes over the forest with
+    the trees held stationary; 1024 threads per block and 513 queries take
+    the walks): counts, one-call writes, truncating capacities, separate
+    count and write calls with per-query caps, IDs and whole points; every
+    query's results equal brute force, including NaN and infinite bounds."""
+    rng = np.random.default_rng(tpb * 7 + n)
+    p = _rand(rng, t, i, n)
+    f, keep = _forest(P, p, t, i, tpb)
+    lo, hi = (-1, 8) if t in ("f32", "f64") else (-60, 80)
+    rec = np.dtype(O.NP[i]) if i is not None else P.point_dtype(t, i)
+    for nq in (7, 512, 513):
+        print(f"stationary forest {t}/{i} tpb={tpb} n={n} nq={nq}", flush=True)
+        qs = _query_batch(rng, t, nq, lo, hi)
+        exp = _expect_batch(p, qs)
+        want = np.array([len(e) for e in exp], np.int64)
+        total = int(want.sum())
+        dq = P.to_device(qs)
+        rp = P.RET_POINT if i is None else P.RET_ID
+        rb = rec.itemsize
+        counts = torch.zeros(nq, dtype=torch.int64, device="cuda")
+        offs = torch.zeros(nq + 1, dtype=torch.int64, device="cuda")
+        f.batch_count(dq, nq, counts)
+        torch.cuda.synchronize()
+        assert np.array_equal(counts.cpu().numpy(), want)
+        out = torch.full((total * rb + 64,), 0x5A, dtype=torch.uint8, device="cuda")
+        f.batch_search(dq, nq, counts, offs, out, total, ret=rp)
+        torch.cuda.synchronize()
+        o = offs.cpu().numpy()
+        raw = out.cpu().numpy()
+        res = raw[:total * rb].view(rec)
+        for k in range(nq):
+            seg = res[o[k]:o[k] + want[k]]
+            if rp == P.RET_POINT:
+                assert G.canon(seg, True, i is not None) == G.canon(p[exp[k]], True, i is not None), k
+            else:
+                assert np.array_equal(np.sort(seg), np.sort(p["id"][exp[k]])), k
+        assert np.all(raw[total * rb:] == 0x5A)
+        # a truncating capacity writes exactly the full call's first records
+        cap = total // 3
+        out2 = torch.full_like(out, 0x5A)
+        f.batch_search(dq, nq, counts, offs, out2, cap, ret=rp)
+        torch.cuda.synchronize()
+        raw2 = out2.cpu().numpy()
+        if nq <= 512:
+            assert np.array_equal(raw2[:cap * rb], raw[:cap * rb])
+        assert np.all(raw2[cap * rb:] == 0x5A)
+        # separate count and write calls, each query capped at half its results
+        caps = torch.from_numpy(want // 2).to("cuda")
+        out3 = torch.full_like(out, 0x5A)
+        f.batch_write(dq, nq, offs, out3, ret=rp, caps_d=caps)
+        torch.cuda.synchronize()
+        res3 = out3.cpu().numpy()[:total * rb].view(rec)
+        for k in range(0, nq, 5):
+            seg = res3[o[k]:o[k] + want[k] // 2]
+            if rp == P.RET_ID:
+                assert np.all(np.isin(seg, p["id"][exp[k]])) and len(np.unique(seg)) == len(seg), k
+            else:
+                full = Counter(G.canon(p[exp[k]], True, i is not None))
+                got = Counter(G.canon(seg, True, i is not None))
+                assert not (got - full), k
+            gap = out3.cpu().numpy()[(o[k] + want[k] // 2) * rb:(o[k] + want[k]) * rb]
+            assert np.all(gap == 0x5A), k
+    f.close()
