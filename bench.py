@@ -6,12 +6,19 @@ Workload (N = 1): a 2^26-point tree of float32 intervals with uint32 IDs
 one batch of 2048 queries: 1024 two-sided stabbing queries (the
 reference's twoSidedLeftSearch(q, q): dim1 <= q <= dim2, q ~ U[0,1)) and
 1024 three-sided queries (threeSidedSearch(a, a + w, c), a, c ~ U[0,1),
-w = exp(U[ln 1e-4, ln 0.5])), run through the batched path (count pass,
-device prefix sum, write pass; IDs returned). Reported beside it, outside
-the timed steps: the plain one-launch-per-query path (preallocated
-output), interval parallel search on the same data, two-sided right
-queries, tree construction, and the CPU baseline (the oracle's stack
-walks, single thread, on the same tree).
+w = exp(U[ln 1e-4, ln 0.5])), run through the batched tree walk (count
+pass, device prefix sum, write pass; IDs returned). `value` is that
+batch's queries/s. Reported beside it, outside the timed steps: the same
+batch through the level index, with whole points returned, with the
+generator's half-left / half-right two-sided mix, and one walk per query;
+the plain one-launch-per-query path (allocating per call as the reference
+does, and into a preallocated output); interval parallel search on the
+same data; tree construction and the memory a tree holds; the CPU
+baseline (the oracle's stack walks on the same tree over one thread and
+all usable cores, a construction sample, and the reference's own
+StaticPSTCPUIter on 2^20 points with 256 stabbing queries beside the GPU
+on the same configuration); the isosurface, int32 batch and serving
+workloads.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
@@ -193,22 +200,22 @@ def iso_bench(args, P, D, torch, dev, stream):
 
 def serve_bench(args, P, D, torch, dev):
     """Serving form: 8 resident trees of 2^20 ... 2^27 points, alternating
-    float32 intervals and int32 geometric-width points; queries arrive as a
-    seeded Poisson stream at args.serve_qps, pick a tree with probability
-    proportional to its size, and draw a query from that tree's mix (float
-    trees: stabbing (q, q) and three-sided as in the main workload; int32
-    trees: the two-/three-sided mix of the int32 batch workload). The
+    float32 intervals and int32 geometric-width points, the level index on
+    every tree (built before the server). Queries arrive as a seeded Poisson
+    stream, pick a tree with probability proportional to its size, and draw
+    a query from that tree's mix (float trees: two-sided (q, q), half left
+    and half right, and three-sided; int32 trees: the int32 batch mix). The
     library's server batches them on the fly and admits them by a 32 GB
-    output budget. Latency = arrival to results ready in device memory."""
+    output budget. Swept over offered loads; latency = scheduled arrival to
+    results ready in device memory (open loop: a late submission counts)."""
     import numpy as np
-    trees, keep, mixes, sizes = [], [], [], []
+    trees, mixes, sizes, kinds = [], [], [], []
     for k, lg in enumerate(range(20, 28)):
         n = 1 << lg
         if k % 2 == 0:
             pts = D.interval_points(n, seed=100 + lg)
             t = "f32"
             qs = D.float_query_mix(2048, 2048, seed=200 + lg)
-            qs["kind"][:2048] = 1
         else:
             pts = D.int_points(n, seed=100 + lg)
             t = "i32"
@@ -226,62 +233,75 @@ def serve_bench(args, P, D, torch, dev):
             tr.set_level_index(True)
             index_bytes += P.StaticPSTGPU.level_index_bytes(n, tr.t, "u32")
         torch.cuda.synchronize()
-    srv = P.Server(trees, args.serve_budget_gb << 30, max_batch=args.serve_max_batch)
-    rng = np.random.default_rng(args.serve_seed)
-    nq = int(args.serve_qps * args.serve_seconds)
-    gaps = rng.exponential(1.0 / args.serve_qps, nq)
-    arrive = np.cumsum(gaps)
     prob = np.array(sizes, np.float64) / sum(sizes)
-    tree_of = rng.choice(len(trees), nq, p=prob)
-    pick = rng.integers(0, 4096, nq)
-    arr_at, lat, ids = {}, [], 0
-    i = 0
-    t0 = time.perf_counter()
-    while len(lat) < nq:
-        now = time.perf_counter() - t0
-        while i < nq and arrive[i] <= now:
-            tk = srv.submit(int(tree_of[i]), mixes[tree_of[i]][pick[i]])
-            arr_at[tk] = arrive[i]
-            i += 1
-        srv.step()
-        for tk in srv.poll():
-            tnow = time.perf_counter() - t0
-            _, cnt = srv.result(tk)
-            ids += cnt
-            lat.append(tnow - arr_at.pop(tk))
-            srv.release(tk)
-        if time.perf_counter() - t0 > args.serve_seconds * 4 + 20:
-            break
-    elapsed = time.perf_counter() - t0
-    srv.close()
-    lat_ms = np.array(lat) * 1e3
+    points = []
+    for load in args.serve_qps:
+        srv = P.Server(trees, args.serve_budget_gb << 30, max_batch=args.serve_max_batch)
+        rng = np.random.default_rng(args.serve_seed)
+        nq = int(load * args.serve_seconds)
+        arrive = np.cumsum(rng.exponential(1.0 / load, nq))
+        tree_of = rng.choice(len(trees), nq, p=prob)
+        pick = rng.integers(0, 4096, nq)
+        arr_of = np.zeros(nq + 1)  # by ticket (tickets of a new server count from 1)
+        lat = np.zeros(nq)
+        ids, done, i = 0, 0, 0
+        t0 = time.perf_counter()
+        while done < nq:
+            now = time.perf_counter() - t0
+            j = int(np.searchsorted(arrive, now, side="right"))
+            if j > i:
+                idx = np.arange(i, j)
+                for tr_i in np.unique(tree_of[i:j]):
+                    sel = idx[tree_of[i:j] == tr_i]
+                    tk = srv.submit_many(int(tr_i), mixes[tr_i][pick[sel]])
+                    arr_of[tk.astype(np.int64)] = arrive[sel]
+                i = j
+            srv.step()
+            tk, cnt = srv.poll_counts()
+            if len(tk):
+                tnow = time.perf_counter() - t0
+                lat[done:done + len(tk)] = tnow - arr_of[tk.astype(np.int64)]
+                ids += int(cnt.sum())
+                done += len(tk)
+                srv.release_many(tk)
+            if time.perf_counter() - t0 > args.serve_seconds * 4 + 20:
+                break
+        elapsed = time.perf_counter() - t0
+        srv.close()
+        lat_ms = lat[:done] * 1e3
+        pt = {"offered_qps": load, "submitted": nq, "completed": done,
+              "served_qps": round(done / elapsed, 1),
+              "p50_ms": round(float(np.percentile(lat_ms, 50)), 3) if done else None,
+              "p99_ms": round(float(np.percentile(lat_ms, 99)), 3) if done else None,
+              "reported_ids_per_s": round(ids / elapsed, 1)}
+        points.append(pt)
+        log(f"[bench] serve {load:.0f} q/s offered: {done}/{nq} in {elapsed:.2f} s, "
+            f"served {pt['served_qps']:.0f} q/s, p50 {pt['p50_ms']} ms, p99 {pt['p99_ms']} ms")
     # plain path on a sample of the same mix
-    samp = np.arange(0, nq, max(1, nq // 200))[:200]
+    rng = np.random.default_rng(args.serve_seed)
+    tree_of = rng.choice(len(trees), 200, p=prob)
+    pick = rng.integers(0, 4096, 200)
     cap = 1 << 27
     pout = torch.empty(cap, dtype=torch.int32, device=dev)
     torch.cuda.synchronize()
     tp = time.perf_counter()
-    for j in samp:
-        q = mixes[tree_of[j]][pick[j]]
-        trees[tree_of[j]].search_into(int(q["kind"]), q["min_dim1"], q["max_dim1"], q["min_dim2"], pout, cap)
-    plain_qps = len(samp) / (time.perf_counter() - tp)
+    for ti, k in zip(tree_of, pick):
+        q = mixes[ti][k]
+        trees[ti].search_into(int(q["kind"]), q["min_dim1"], q["max_dim1"], q["min_dim2"], pout, cap)
+    plain_qps = len(tree_of) / (time.perf_counter() - tp)
     for tr in trees:
         tr.close()
-    done = len(lat)
-    log(f"[bench] serve: {done}/{nq} queries at {args.serve_qps} q/s offered, "
-        f"{done / elapsed:.0f} q/s served, p50 {np.percentile(lat_ms, 50):.2f} ms, "
-        f"p99 {np.percentile(lat_ms, 99):.2f} ms, plain {plain_qps:.0f} q/s")
+    # capacity: the highest load served at >= 95% of the offered rate
+    kept = [p for p in points if p["completed"] == p["submitted"] and p["served_qps"] >= 0.95 * p["offered_qps"]]
     return {
         "workload": "8 resident trees 2^20..2^27 (alternating f32 intervals / int32 geometric widths), "
-                    f"Poisson arrivals at {args.serve_qps} q/s for {args.serve_seconds} s, tree chosen with "
-                    "probability proportional to size, 32 GB output budget, IDs returned"
+                    f"Poisson arrivals for {args.serve_seconds} s per offered load, tree chosen with probability "
+                    f"proportional to size, {args.serve_budget_gb} GB output budget, at most "
+                    f"{args.serve_max_batch} queries per batch, IDs returned"
                     + (", level index on every tree" if args.level_index else ""),
         "level_index_bytes": index_bytes,
-        "offered_qps": args.serve_qps, "completed": done, "submitted": nq,
-        "served_qps": round(done / elapsed, 1),
-        "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
-        "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
-        "reported_ids": int(ids),
+        "loads": points,
+        "sustained_qps": max((p["served_qps"] for p in kept), default=None),
         "plain_qps_same_mix": round(plain_qps, 1),
     }
 
@@ -385,11 +405,131 @@ def int_batch_bench(args, P, D, torch, dev):
         "graph_qps": round(nq / (graph_ms * 1e-3), 1),
         "eager_ms_per_batch": round(eager_ms, 3),
         "reported_ids_per_s": round(total / (graph_ms * 1e-3), 1),
-        "alg_GBps": round(total * BYTES_PER_ID / (graph_ms * 1e-3) / 1e9, 1),
+        "alg_GBps_21B": round(total * BYTES_PER_ID / (graph_ms * 1e-3) / 1e9, 1),
+        "frac_21B": round(total * BYTES_PER_ID / (graph_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+        "model_21B_applies": bool(total * BYTES_PER_ID / (graph_ms * 1e-3) / 1e9 <= HBM_PEAK_GBPS),
         "autotuned": tuned,
         "level_index": index,
         "results_checked": bool(ok),
     }
+
+
+def _free_bytes(torch):
+    torch.cuda.synchronize()
+    return torch.cuda.mem_get_info()[0]
+
+
+def cpu_info():
+    """Host cores this process may use and the CPU model (lscpu / /proc/cpuinfo)."""
+    import subprocess
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    if not model and os.path.exists("/proc/cpuinfo"):
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    nproc = os.cpu_count() or usable
+    return usable, nproc, model
+
+
+def plain_legs(P, torch, tree, qs, sel, host_counts, dev, point_sel=None):
+    """The plain one-launch-per-query path on the queries `sel`:
+    (a) the reference-faithful drop-in call, which allocates n * sizeof(RetType)
+        for each result (pst/static-pst-gpu.tu:512, 563, 615) and is freed
+        after it; (b) a preallocated output (search_into). IDs returned;
+    (a) with whole points (the reference's default RetType) on point_sel."""
+    import ctypes
+    import numpy as np
+    L = P.lib()
+    num, ptr = ctypes.c_size_t(), ctypes.c_void_p()
+
+    def call_a(k, ret):
+        kind = int(qs["kind"][k])
+        a, b, c = (P._scalar(v, "f32") for v in (qs["min_dim1"][k], qs["max_dim1"][k], qs["min_dim2"][k]))
+        if kind == 0:
+            rc = L.pst_gpu_three_sided_search(tree.h, ret, ctypes.byref(num), ctypes.byref(ptr), a.ctypes.data,
+                                              b.ctypes.data, c.ctypes.data)
+        elif kind == 1:
+            rc = L.pst_gpu_two_sided_left_search(tree.h, ret, ctypes.byref(num), ctypes.byref(ptr), b.ctypes.data,
+                                                 c.ctypes.data)
+        else:
+            rc = L.pst_gpu_two_sided_right_search(tree.h, ret, ctypes.byref(num), ctypes.byref(ptr), a.ctypes.data,
+                                                  c.ctypes.data)
+        P._check(rc)
+        got = num.value
+        P._check(L.pst_free(ptr))
+        return got
+
+    ok = True
+    cap = int(host_counts[sel].max()) + 1
+    pout = torch.empty(cap, dtype=torch.int32, device=dev)
+    for k in sel[:2]:
+        call_a(k, P.RET_ID)
+        tree.search_into(int(qs["kind"][k]), qs["min_dim1"][k], qs["max_dim1"][k], qs["min_dim2"][k], pout, cap)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in sel:
+        ok &= call_a(k, P.RET_ID) == int(host_counts[k])
+    a_qps = len(sel) / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    for k in sel:
+        ok &= tree.search_into(int(qs["kind"][k]), qs["min_dim1"][k], qs["max_dim1"][k], qs["min_dim2"][k],
+                               pout, cap) == int(host_counts[k])
+    b_qps = len(sel) / (time.perf_counter() - t0)
+    res = {"a_alloc_per_call_qps": round(a_qps, 1), "b_preallocated_qps": round(b_qps, 1), "queries": len(sel)}
+    if point_sel is not None:
+        t0 = time.perf_counter()
+        for k in point_sel:
+            ok &= call_a(k, P.RET_POINT) == int(host_counts[k])
+        res["a_alloc_per_call_points_qps"] = round(len(point_sel) / (time.perf_counter() - t0), 1)
+        res["point_queries"] = len(point_sel)
+    del pout
+    return res, ok
+
+
+def batch_leg(P, torch, tree, d_q, nq, dev, stream, reps, ret=0, counts=None, total=None):
+    """Count + scan + write batches of one query set: q/s over `reps` timed
+    batches after one warm batch, and the count / write pass split (events
+    on the launch stream). Returns (figures, counts, offsets, out)."""
+    if counts is None:
+        counts = torch.zeros(nq, dtype=torch.int64, device=dev)
+        tree.batch_count(d_q, nq, counts, stream=stream)
+        torch.cuda.synchronize()
+        total = int(counts.sum().item())
+    rec = 12 if ret == P.RET_POINT else 4
+    offsets = torch.zeros(nq + 1, dtype=torch.int64, device=dev)
+    out = torch.empty(total * rec // 4 + 1, dtype=torch.int32, device=dev)
+    tree.batch_search(d_q, nq, counts, offsets, out, total, ret=ret, stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        tree.batch_search(d_q, nq, counts, offsets, out, total, ret=ret, stream=stream)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record(stream)
+    tree.batch_count(d_q, nq, counts, stream=stream)
+    ev[1].record(stream)
+    tree.batch_write(d_q, nq, offsets, out, ret=ret, caps_d=counts, stream=stream)
+    ev[2].record(stream)
+    torch.cuda.synchronize()
+    fig = {"ms_per_batch": round(ms, 3), "qps": round(nq / (ms * 1e-3), 1),
+           "reported_ids": total, "reported_ids_per_s": round(total / (ms * 1e-3), 1),
+           "count_pass_ms": round(ev[0].elapsed_time(ev[1]), 3),
+           "write_pass_ms": round(ev[1].elapsed_time(ev[2]), 3)}
+    return fig, counts, offsets, out
 
 
 def main():
@@ -400,10 +540,14 @@ def main():
     ap.add_argument("--log2n", type=int, default=26)
     ap.add_argument("--nq2", type=int, default=1024)
     ap.add_argument("--nq3", type=int, default=1024)
-    ap.add_argument("--plain", type=int, default=256, help="queries timed on the plain path")
+    ap.add_argument("--plain", type=int, default=128, help="queries timed on each plain-path variant")
     ap.add_argument("--ips", type=int, default=64, help="queries timed for interval parallel search")
-    ap.add_argument("--cpu-sample", type=int, default=512, help="queries in the CPU baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=256, help="queries of the one-thread CPU baseline sample")
+    ap.add_argument("--cpu-sample-par", type=int, default=1024, help="queries of the all-cores CPU sample")
+    ap.add_argument("--cpu-build-log2n", type=int, default=22, help="CPU construction sample size")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mixed", type=int, default=1, help="also time the half-left / half-right two-sided mix")
+    ap.add_argument("--points", type=int, default=1, help="also time the batch with whole points returned")
     ap.add_argument("--iso-side", type=int, default=1024, help="isosurface volume side (0: skip)")
     ap.add_argument("--volume-file", default="", help="raw float32 volume for the isosurface leg "
                     "(default: the synthetic volume)")
@@ -415,20 +559,19 @@ def main():
     ap.add_argument("--int-log2n", type=int, default=27, help="int32 batch workload size (0: skip)")
     ap.add_argument("--int-queries", type=int, default=8192)
     ap.add_argument("--int-steps", type=int, default=5)
-    ap.add_argument("--serve-qps", type=float, default=20000.0, help="offered load of the serving run (0: skip)")
-    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-qps", type=float, nargs="*", default=[20000.0, 50000.0, 100000.0, 200000.0],
+                    help="offered loads of the serving sweep (none: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-budget-gb", type=int, default=32)
     ap.add_argument("--serve-max-batch", type=int, default=1024)
     ap.add_argument("--serve-seed", type=int, default=4)
-    ap.add_argument("--sort-queries", type=int, default=0,
-                    help="hand the batch to the library pre-ordered by min_dim1/max_dim1 "
-                         "(the library groups queries itself)")
     ap.add_argument("--ungrouped", type=int, default=1,
                     help="also time the batch with one walk per query (grouping off)")
     ap.add_argument("--level-index", type=int, default=1,
-                    help="1: the headline batch runs through the level index (the walk is timed beside it)")
-    ap.add_argument("--profile-mode", action="store_true",
-                    help="build + one batched step + a 1 GiB copy (for counter scaling), no JSON")
+                    help="1: also build the level index and time the batches and plain calls through it")
+    ap.add_argument("--profile-mode", default="",
+                    choices=["", "walk", "index", "build", "ips"],
+                    help="build + one pass of the named path + a 1 GiB copy (for counter runs), no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -451,14 +594,14 @@ def main():
     d_pts = P.to_device(pts, dev)
     torch.cuda.synchronize()
 
-    # ---- construction
-    # the first build of a process also loads the construction kernels and
-    # maps fresh memory; later builds show the steady construction time
+    # ---- construction (and the device memory a tree holds)
     log("[bench] building tree")
+    free0 = _free_bytes(torch)
     t0 = time.perf_counter()
     tree = P.StaticPSTGPU(d_pts, n, "f32", "u32")
     torch.cuda.synchronize()
     cold_build_ms = (time.perf_counter() - t0) * 1e3
+    tree_bytes = free0 - _free_bytes(torch)
     warm = []
     for _ in range(3):
         t0 = time.perf_counter()
@@ -471,124 +614,177 @@ def main():
     build_bytes = n * 12 + S * 17
     log(f"[bench] build {build_ms:.1f} ms (median of 3; first build of the process {cold_build_ms:.1f} ms)")
 
-    # ---- queries
+    if args.profile_mode == "build":
+        log("[bench] profile mode (build) done")
+        tree.close()
+        return
+
+    # ---- the headline batch: two-sided stabbing queries (q, q) as the
+    # reference's twoSidedLeftSearch, and three-sided queries
     qs = D.float_query_mix(args.nq3, args.nq2, seed=0)
-    qs["kind"][:args.nq2] = 1  # two-sided left stabbing (q, q)
-    if args.sort_queries:
-        # order each kind by the bound that fixes where in dim1 its search runs
-        for lo, hi in ((0, args.nq2), (args.nq2, args.nq2 + args.nq3)):
-            key = qs["max_dim1"][lo:hi] if lo == 0 else qs["min_dim1"][lo:hi]
-            qs[lo:hi] = qs[lo:hi][np.argsort(key, kind="stable")]
+    qs["kind"][:args.nq2] = 1
     nq = len(qs)
     d_q = P.to_device(qs, dev)
     counts = torch.zeros(nq, dtype=torch.int64, device=dev)
-    offsets = torch.zeros(nq + 1, dtype=torch.int64, device=dev)
-    # size the output from one count pass
     tree.batch_count(d_q, nq, counts, stream=stream)
     torch.cuda.synchronize()
     total = int(counts.sum().item())
-    log(f"[bench] batch of {nq}: {total} reported IDs")
-    out = torch.empty(total + 1, dtype=torch.int32, device=dev)
-
-    # sanity (outside timing): batch counts equal brute-force counts for a few queries
     host_counts = counts.cpu().numpy()
+    log(f"[bench] batch of {nq}: {total} reported IDs")
     ok = True
     d1, d2 = pts["dim1"], pts["dim2"]
-    for k in (0, 1, args.nq2, args.nq2 + 1):
-        a, b, c, kind = qs["min_dim1"][k], qs["max_dim1"][k], qs["min_dim2"][k], int(qs["kind"][k])
-        if kind == 1:
-            want = int(np.count_nonzero((d1 <= b) & (d2 >= c)))
-        else:
-            want = int(np.count_nonzero((d1 >= a) & (d1 <= b) & (d2 >= c)))
-        ok &= want == int(host_counts[k])
 
-    if args.profile_mode:
-        if args.level_index:
+    def brute_count(q):
+        a, b, c, kind = q["min_dim1"], q["max_dim1"], q["min_dim2"], int(q["kind"])
+        m = d2 >= c
+        if kind != 2:
+            m &= d1 <= b
+        if kind != 1:
+            m &= d1 >= a
+        return int(np.count_nonzero(m))
+
+    for k in (0, 1, args.nq2, args.nq2 + 1):
+        ok &= brute_count(qs[k]) == int(host_counts[k])
+
+    if args.profile_mode in ("walk", "index", "ips"):
+        if args.profile_mode == "index":
             tree.set_level_index(True)
-        tree.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
+        if args.profile_mode == "ips":
+            ips_out = torch.empty(int(host_counts[:args.nq2].max()) + 1, dtype=torch.int32, device=dev)
+            ips_ctr = torch.zeros(1, dtype=torch.int64, device=dev)
+            for k in range(8):
+                P.ips_into(d_pts, n, qs["max_dim1"][k], ips_out, ips_out.numel(), ips_ctr, stream=stream)
+        else:
+            offsets = torch.zeros(nq + 1, dtype=torch.int64, device=dev)
+            out = torch.empty(total + 1, dtype=torch.int32, device=dev)
+            tree.batch_search(d_q, nq, counts, offsets, out, total, stream=stream)
         torch.cuda.synchronize()
         # plain copy of a known size: the scale for HBM byte counters
         x = torch.ones(1 << 28, dtype=torch.int32, device=dev)
         y = torch.empty_like(x)
         y.copy_(x)
         torch.cuda.synchronize()
-        log(f"[bench] profile mode done: {total} IDs, copy of {x.numel() * 4} bytes")
+        log(f"[bench] profile mode ({args.profile_mode}) done: {total} IDs, copy of {x.numel() * 4} bytes")
         tree.close()
         return
 
-    def timed_steps():
-        for _ in range(args.warmup):
-            tree.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            tree.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0
-
-    def pass_times():
-        # per-pass timing with events on the launch stream (roofline figure)
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-        tc = tw = 0.0
-        reps = max(1, args.steps)
-        for _ in range(reps):
-            ev[0].record(stream)
-            tree.batch_count(d_q, nq, counts, stream=stream)
-            ev[1].record(stream)
-            tree.batch_write(d_q, nq, offsets, out, caps_d=counts, stream=stream)
-            ev[2].record(stream)
-            torch.cuda.synchronize()
-            tc += ev[0].elapsed_time(ev[1])
-            tw += ev[1].elapsed_time(ev[2])
-        return tc / reps, tw / reps
-
-    def sample_results(ks):
-        c, o = counts.cpu().numpy(), offsets.cpu().numpy()
-        return {k: np.sort(out[o[k]:o[k] + c[k]].cpu().numpy()) for k in ks}
-
     # ---- timed steps on the tree walk: batched count + scan + write
-    elapsed = timed_steps()
+    offsets = torch.zeros(nq + 1, dtype=torch.int64, device=dev)
+    out = torch.empty(total + 1, dtype=torch.int32, device=dev)
+    for _ in range(args.warmup):
+        tree.batch_search(d_q, nq, counts, offsets, out, total, stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tree.batch_search(d_q, nq, counts, offsets, out, total, stream=stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
     ok &= int(offsets[nq].item()) == total
-    walk_ms = elapsed * 1e3 / args.steps
-    w_count, w_write = pass_times()
+    ms_per_step = elapsed * 1e3 / args.steps
+    qps = nq * args.steps / elapsed
+    ids_per_s = total * args.steps / elapsed
+    # per-pass times with events on the launch stream, averaged over the steps
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    t_count = t_write = 0.0
+    for _ in range(args.steps):
+        ev[0].record(stream)
+        tree.batch_count(d_q, nq, counts, stream=stream)
+        ev[1].record(stream)
+        tree.batch_write(d_q, nq, offsets, out, caps_d=counts, stream=stream)
+        ev[2].record(stream)
+        torch.cuda.synchronize()
+        t_count += ev[0].elapsed_time(ev[1]) / args.steps
+        t_write += ev[1].elapsed_time(ev[2]) / args.steps
     sample_ks = sorted({0, 1, args.nq2 // 2, args.nq2 - 1, args.nq2, args.nq2 + args.nq3 // 2, nq - 1})
-    walk_ref = sample_results(sample_ks)
-    log(f"[bench] walk: {walk_ms:.2f} ms per batch ({nq / (walk_ms * 1e-3):.0f} q/s), count pass {w_count:.2f} ms, "
-        f"write pass {w_write:.2f} ms")
-    # HBM bytes per launch of the walk's write pass on this workload, from
-    # the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated)
-    walk_traffic = None
+    oh = offsets.cpu().numpy()
+    walk_ref = {k: np.sort(out[oh[k]:oh[k] + host_counts[k]].cpu().numpy()) for k in sample_ks}
+    for k in sample_ks[:3]:
+        q = qs[k]
+        m = (d2 >= q["min_dim2"]) & (d1 <= q["max_dim1"])
+        if int(q["kind"]) == 0:
+            m &= d1 >= q["min_dim1"]
+        ok &= np.array_equal(walk_ref[k].view(np.uint32), np.flatnonzero(m).astype(np.uint32))
+    log(f"[bench] walk: {ms_per_step:.2f} ms per batch ({qps:.0f} q/s), count pass {t_count:.2f} ms, "
+        f"write pass {t_write:.2f} ms")
+    # roofline of the write pass on the per-reported-ID model (21 B/ID)
+    achieved = total * BYTES_PER_ID / (t_write * 1e-3) / 1e9
+    traffic = traffic_src = None
     tpath = os.path.join(ROOT, "profiles", "write_pass_k_search_round_k_copy_tasks_f32u32_2e26_batch2048.traffic.json")
     if args.log2n == 26 and args.nq2 == 1024 and args.nq3 == 1024 and os.path.exists(tpath):
         with open(tpath) as f:
-            walk_traffic = round(json.load(f)["traffic_bytes_per_launch"])
-    walk = {"ms_per_step": round(walk_ms, 3), "qps": round(nq / (walk_ms * 1e-3), 2),
-            "count_pass_ms": round(w_count, 3), "write_pass_ms": round(w_write, 3),
-            "write_pass_alg_GBps": round(total * BYTES_PER_ID / (w_write * 1e-3) / 1e9, 1),
-            "write_pass_traffic_bytes_per_launch": walk_traffic,
-            "write_launches": WRITE_LAUNCHES}
+            tj = json.load(f)
+        traffic, traffic_src = round(tj["traffic_bytes_per_launch"]), tj["source"]
+    traffic_frac = None if traffic is None else round(traffic / (t_write * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
 
-    # ---- the same batch with one walk per query (grouping off), for comparison
+    # ---- the same batch with one walk per query (grouping off)
     ungrouped_qps = None
     if args.ungrouped:
         tree.set_grouping(1)
-        tree.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
+        tree.batch_search(d_q, nq, counts, offsets, out, total, stream=stream)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        tree.batch_search(d_q, nq, counts, offsets, out, total + 1, stream=stream)
+        tree.batch_search(d_q, nq, counts, offsets, out, total, stream=stream)
         torch.cuda.synchronize()
         ungrouped_qps = nq / (time.perf_counter() - t0)
         ok &= int(offsets[nq].item()) == total
         tree.set_grouping(8)
         log(f"[bench] ungrouped batch {ungrouped_qps:.0f} q/s")
+    del out
+    torch.cuda.empty_cache()
 
-    # ---- level index: build it, then the same batch through it
+    # ---- whole points returned (the reference's default RetType), walk
+    points_fig = None
+    if args.points:
+        fig, _, po, pout = batch_leg(P, torch, tree, d_q, nq, dev, stream, 2, ret=P.RET_POINT, counts=counts,
+                                     total=total)
+        prec = P.point_dtype("f32", "u32")
+        poh = po.cpu().numpy()
+        for k in sample_ks[:3]:
+            rows = pout.view(torch.uint8)[poh[k] * 12:(poh[k] + host_counts[k]) * 12].cpu().numpy().view(prec)
+            ok &= np.array_equal(np.sort(rows["id"]), walk_ref[k].view(np.uint32))
+            ok &= bool(np.all(pts[rows["id"]] == rows))
+        fig["alg_GBps_write_pass"] = round(total * (12 + 12 + 1 + 12) / (fig["write_pass_ms"] * 1e-3) / 1e9, 1)
+        points_fig = fig
+        del pout, po
+        torch.cuda.empty_cache()
+        log(f"[bench] points returned: {fig['qps']:.0f} q/s")
+
+    # ---- plain path through the walk
+    np_plain = min(args.plain, nq)
+    sel = np.concatenate([np.arange(np_plain // 2), args.nq2 + np.arange(np_plain - np_plain // 2)])
+    psel = sel[::4]
+    plain_walk, okp = plain_legs(P, torch, tree, qs, sel, host_counts, dev, point_sel=psel)
+    ok &= okp
+    log(f"[bench] plain (walk): {plain_walk}")
+
+    # ---- the generator's two-sided mix (half left, half right (q, q))
+    mixed = None
+    if args.mixed:
+        qm = D.float_query_mix(args.nq3, args.nq2, seed=0)
+        d_qm = P.to_device(qm, dev)
+        fig, mc, mo, mout = batch_leg(P, torch, tree, d_qm, nq, dev, stream, 2)
+        mch = mc.cpu().numpy()
+        rk = int(np.flatnonzero(qm["kind"] == 2)[0])
+        ok &= brute_count(qm[rk]) == int(mch[rk])
+        moh = mo.cpu().numpy()
+        mixed_ref = np.sort(mout[moh[rk]:moh[rk] + mch[rk]].cpu().numpy())
+        fig["alg_GBps_write_pass_21B"] = round(fig["reported_ids"] * BYTES_PER_ID / (fig["write_pass_ms"] * 1e-3) / 1e9, 1)
+        mixed = {"queries": f"{int(np.sum(qm['kind'] == 1))} twoSidedLeftSearch(q, q) + "
+                            f"{int(np.sum(qm['kind'] == 2))} twoSidedRightSearch(q, q) + {args.nq3} threeSidedSearch",
+                 "walk": fig}
+        del mout, mo
+        torch.cuda.empty_cache()
+        log(f"[bench] two-sided mix (walk): {fig['qps']:.0f} q/s, {fig['reported_ids']} IDs")
+
+    # ---- level index: build it, then the batches and plain calls through it
     index = None
     if args.level_index:
+        free1 = _free_bytes(torch)
         t0 = time.perf_counter()
         tree.set_level_index(True)
         torch.cuda.synchronize()
         first_index_ms = (time.perf_counter() - t0) * 1e3
+        index_resident = free1 - _free_bytes(torch)
         iw = []
         for _ in range(2):
             tree.set_level_index(False)
@@ -596,82 +792,36 @@ def main():
             tree.set_level_index(True)
             torch.cuda.synchronize()
             iw.append((time.perf_counter() - t0) * 1e3)
-        out.fill_(-1)
-        elapsed = timed_steps()
-        ok &= int(offsets[nq].item()) == total
-        got = sample_results(sample_ks)
-        same = all(np.array_equal(got[k], walk_ref[k]) for k in sample_ks)
+        fig, _, io, iout = batch_leg(P, torch, tree, d_q, nq, dev, stream, args.steps, counts=counts, total=total)
+        ioh = io.cpu().numpy()
+        same = all(np.array_equal(np.sort(iout[ioh[k]:ioh[k] + host_counts[k]].cpu().numpy()), walk_ref[k])
+                   for k in sample_ks)
         ok &= same
-        t_count, t_write = pass_times()
-        index = {"build_ms": round(min(iw), 2), "first_build_ms": round(first_index_ms, 2),
+        del iout, io
+        torch.cuda.empty_cache()
+        fig["copy_frac_8B"] = round(total * COPY_BYTES_PER_ID / (fig["write_pass_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
+        saved = ms_per_step - fig["ms_per_batch"]
+        index = {"batch": fig, "build_ms": round(min(iw), 2), "first_build_ms": round(first_index_ms, 2),
                  "bytes": P.StaticPSTGPU.level_index_bytes(n, "f32", "u32"),
-                 "same_results_as_walk": bool(same)}
-        log(f"[bench] level index: build {min(iw):.1f} ms (first {first_index_ms:.1f} ms), "
-            f"{index['bytes'] / 2**30:.1f} GiB, results as the walk's: {same}")
-    else:
-        t_count, t_write = w_count, w_write
-    ms_per_step = elapsed * 1e3 / args.steps
-    qps = nq * args.steps / elapsed
-    ids_per_s = total * args.steps / elapsed
-    if args.level_index:
-        alg_bytes = total * COPY_BYTES_PER_ID
-        launches = 1
-        kernel = ("write pass: k_lv_plan<float, unsigned int, true> + offset scan + k_lv_mark + "
-                  "k_lv_copy<unsigned int> (the copy streams every ID; the pass time is its upper bound)")
-        tpath = os.path.join(ROOT, "profiles", "write_pass_k_lv_copy_f32u32_2e26_batch2048.traffic.json")
-    else:
-        alg_bytes = total * BYTES_PER_ID
-        launches = WRITE_LAUNCHES
-        kernel = ("write pass: k_search_round<float, unsigned int, false, true, 8> x 4 "
-                  "+ k_copy_tasks<float, unsigned int, false, 8> x 1")
-    achieved = alg_bytes / (t_write * 1e-3) / 1e9
-    traffic, traffic_src = None, None
-    if args.log2n == 26 and args.nq2 == 1024 and args.nq3 == 1024 and os.path.exists(tpath):
-        with open(tpath) as f:
-            tj = json.load(f)
-        traffic, traffic_src = round(tj["traffic_bytes_per_launch"]), tj["source"]
-    log(f"[bench] count pass {t_count:.2f} ms, write pass {t_write:.2f} ms, "
-        f"write-pass alg {achieved:.0f} GB/s")
-
-    # ---- plain path: one blocking search per query, preallocated output
-    np_plain = min(args.plain, nq)
-    sel = np.concatenate([np.arange(np_plain // 2), args.nq2 + np.arange(np_plain - np_plain // 2)])
-    cap = int(host_counts[sel].max()) + 1
-    pout = torch.empty(cap, dtype=torch.int32, device=dev)
-    for k in sel[:4]:
-        tree.search_into(int(qs["kind"][k]), qs["min_dim1"][k], qs["max_dim1"][k], qs["min_dim2"][k], pout, cap)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    plain_ids = 0
-    for k in sel:
-        got = tree.search_into(int(qs["kind"][k]), qs["min_dim1"][k], qs["max_dim1"][k],
-                               qs["min_dim2"][k], pout, cap)
-        plain_ids += got
-        ok &= got == int(host_counts[k])
-    plain_s = time.perf_counter() - t0
-    plain_qps = len(sel) / plain_s
-    log(f"[bench] plain path {plain_qps:.0f} q/s")
-
-    # ---- two-sided right (q, q): outputs average n/2
-    right_q = D.stabbing_queries(8, seed=5)
-    rq = np.zeros(len(right_q), P.query_dtype("f32"))
-    rq["min_dim1"] = right_q
-    rq["min_dim2"] = right_q
-    rq["kind"] = 2
-    d_rq = P.to_device(rq, dev)
-    rc = torch.zeros(len(rq), dtype=torch.int64, device=dev)
-    ro = torch.zeros(len(rq) + 1, dtype=torch.int64, device=dev)
-    tree.batch_count(d_rq, len(rq), rc, stream=stream)
-    torch.cuda.synchronize()
-    rtotal = int(rc.sum().item())
-    rout = torch.empty(rtotal + 1, dtype=torch.int32, device=dev)
-    tree.batch_search(d_rq, len(rq), rc, ro, rout, rtotal + 1, stream=stream)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    tree.batch_search(d_rq, len(rq), rc, ro, rout, rtotal + 1, stream=stream)
-    torch.cuda.synchronize()
-    right_s = time.perf_counter() - t0
-    del rout
+                 "resident_bytes_measured": int(index_resident),
+                 "break_even_batches": round(min(iw) / saved, 2) if saved > 0 else None,
+                 "same_results_as_walk": bool(same),
+                 "alg_bytes_per_id": COPY_BYTES_PER_ID}
+        if mixed is not None:
+            mfig, _, mo, mout = batch_leg(P, torch, tree, d_qm, nq, dev, stream, 2, counts=mc,
+                                          total=mixed["walk"]["reported_ids"])
+            moh = mo.cpu().numpy()
+            ok &= np.array_equal(np.sort(mout[moh[rk]:moh[rk] + mch[rk]].cpu().numpy()), mixed_ref)
+            mfig["copy_frac_8B"] = round(mfig["reported_ids"] * COPY_BYTES_PER_ID / (mfig["write_pass_ms"] * 1e-3)
+                                         / 1e9 / HBM_PEAK_GBPS, 4)
+            mixed["level_index"] = mfig
+            del mout, mo
+            torch.cuda.empty_cache()
+        plain_index, okp = plain_legs(P, torch, tree, qs, sel, host_counts, dev)
+        ok &= okp
+        index["plain"] = plain_index
+        log(f"[bench] level index: build {min(iw):.1f} ms, batch {fig['qps']:.0f} q/s, plain {plain_index}")
+        tree.set_level_index(False)
 
     # ---- interval parallel search on the same data (stabbing queries)
     nips = min(args.ips, args.nq2)
@@ -690,39 +840,109 @@ def main():
     ips_gbps = (n * 12 + host_counts[:nips].mean() * 4) / (ips_ms * 1e-3) / 1e9
     log(f"[bench] IPS {1e3 / ips_ms:.0f} q/s ({ips_gbps:.0f} GB/s)")
 
-    # ---- CPU baseline: the oracle's walks on the same tree (exported)
+    # ---- GPU side of the CPU-runnable configuration: 2^20 points, seed 0,
+    # 256 stabbing queries (q, q)
+    c0n = 1 << 20
+    c0 = D.interval_points(c0n, seed=0)
+    c0q = D.stabbing_queries(256, seed=0)
+    c0d = P.to_device(c0, dev)
+    c0b = []
+    for _ in range(4):
+        t0 = time.perf_counter()
+        t_c0 = P.StaticPSTGPU(c0d, c0n, "f32", "u32")
+        torch.cuda.synchronize()
+        c0b.append((time.perf_counter() - t0) * 1e3)
+        if len(c0b) < 4:
+            t_c0.close()
+    c0r = np.zeros(256, P.query_dtype("f32"))
+    c0r["min_dim1"] = c0r["max_dim1"] = c0r["min_dim2"] = c0q
+    c0r["kind"] = 1
+    c0fig, c0c, _, _ = batch_leg(P, torch, t_c0, P.to_device(c0r, dev), 256, dev, stream, 5)
+    c0counts = c0c.cpu().numpy()
+    c0pout = torch.empty(int(c0counts.max()) + 1, dtype=torch.int32, device=dev)
+    t0 = time.perf_counter()
+    for k in range(256):
+        ok &= t_c0.search_into(1, c0q[k], c0q[k], c0q[k], c0pout, c0pout.numel()) == int(c0counts[k])
+    c0_plain = 256 / (time.perf_counter() - t0)
+    t_c0.close()
+    gpu_cfg0 = {"build_ms": round(float(np.median(c0b[1:])), 3), "batch": c0fig, "plain_qps": round(c0_plain, 1),
+                "reported_ids": int(c0counts.sum())}
+    log(f"[bench] 2^20 / 256 stabbing queries on the GPU: build {gpu_cfg0['build_ms']} ms, batch "
+        f"{c0fig['ms_per_batch']} ms, plain {c0_plain:.0f} q/s")
+
+    # ---- CPU baseline
     cpu = None
     if not args.no_cpu:
+        import ctypes
+        import tempfile
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O  # CPU checker, used here only as the timed CPU baseline
+        usable, nproc, model = cpu_info()
+        threads = max(1, min(usable, 16))
+        # (i) the 2^26 tree: the oracle's stack walks (a restatement of the
+        # reference's StaticPSTCPUIter searches) on the exported tree, which
+        # is bit-identical to the reference's
         ex_bytes = P.lib().pst_gpu_export_bytes(tree.h)
         tr = O.empty_tree("f32", "u32", n)
         assert tr.blob.nbytes >= ex_bytes
-        import ctypes
         off = (ctypes.c_size_t * 6)()
         P._check(P.lib().pst_gpu_export_tree(tree.h, tr.blob.ctypes.data, ex_bytes, off))
-        ns = min(args.cpu_sample, nq)
-        csel = np.concatenate([np.arange(ns // 2), args.nq2 + np.arange(ns - ns // 2)])
-        kinds = np.where(qs["kind"][csel] == 1, 1, 0)
-        tot, secs = O.time_queries(tr, kinds, qs["min_dim1"][csel], qs["max_dim1"][csel],
-                                   qs["min_dim2"][csel], threads=1)
-        ok &= tot == int(host_counts[csel].sum())
-        cpu = {"value": round(ns / secs, 3), "unit": "queries/s", "cores": 1, "kind": "port",
-               "sample": f"{ns} of the batch's queries ({ns // 2} two-sided stabbing, "
-                         f"{ns - ns // 2} three-sided) on the same 2^{args.log2n}-point tree, "
-                         f"oracle stack walk (g++ -O2), one thread, {secs:.1f} s, "
-                         f"{tot} IDs"}
-        log(f"[bench] CPU {ns / secs:.2f} q/s over {secs:.1f} s")
+        kinds = np.where(qs["kind"] == 1, 1, 0)
+
+        def cpu_sample(ns, th):
+            csel = np.concatenate([np.arange(ns // 2), args.nq2 + np.arange(ns - ns // 2)])
+            tot, secs = O.time_queries(tr, kinds[csel], qs["min_dim1"][csel], qs["max_dim1"][csel],
+                                       qs["min_dim2"][csel], threads=th)
+            return ns, tot, secs, int(host_counts[csel].sum())
+
+        ns1, tot1, secs1, want1 = cpu_sample(min(args.cpu_sample, nq), 1)
+        nsP, totP, secsP, wantP = cpu_sample(min(args.cpu_sample_par, nq), threads)
+        ok &= tot1 == want1 and totP == wantP
         del tr
+        # (ii) construction: the oracle's (index init + 2 sorts + populate) on
+        # a 2^k sample of the same generator, extrapolated as n log n
+        bn = 1 << args.cpu_build_log2n
+        _, bsecs = O.time_build(D.interval_points(bn, seed=0), "f32", "u32")
+        est = bsecs * (n * args.log2n) / (bn * args.cpu_build_log2n)
+        # (iii) the reference's own StaticPSTCPUIter on the CPU configuration
+        # (2^20 points, seed 0, 256 stabbing queries): construction and queries
+        ref0 = None
+        if O.ref_bench_available():
+            r = O.time_reference(c0, np.ones(256, np.int32), c0q, c0q, c0q, [1, threads], tempfile.gettempdir())
+            ok &= all(x["ids"] == int(c0counts.sum()) for x in r["runs"])
+            ref0 = {"build_s": round(r["build_s"], 4),
+                    "qps_1_thread": round(256 / r["runs"][0]["secs"], 1),
+                    f"qps_{threads}_threads": round(256 / r["runs"][1]["secs"], 1),
+                    "reported_ids": r["runs"][0]["ids"],
+                    "gpu_batch_speedup_vs_all_cores": round(c0fig["qps"] / (256 / r["runs"][1]["secs"]), 1),
+                    "gpu_build_speedup": round(r["build_s"] * 1e3 / gpu_cfg0["build_ms"], 1)}
+        cpu = {"value": round(nsP / secsP, 3), "unit": "queries/s", "cores": threads, "kind": "port",
+               "sample": f"{nsP} of the batch's queries ({nsP // 2} two-sided stabbing, {nsP - nsP // 2} "
+                         f"three-sided) on the same 2^{args.log2n}-point tree, the oracle's stack walks "
+                         f"(restating StaticPSTCPUIter's searches; g++ -O2) over {threads} std::threads "
+                         f"sharing the tree, {secsP:.1f} s, {totP} IDs; results compared with the batch's "
+                         "counts; no host-to-device copy of results",
+               "one_thread_qps": round(ns1 / secs1, 3),
+               "one_thread_sample": f"{ns1} queries, {secs1:.1f} s, {tot1} IDs",
+               "host_cpu": model, "nproc": nproc, "usable_cores": usable,
+               "build": {"sample_points": bn, "sample_s": round(bsecs, 3),
+                         f"estimate_2e{args.log2n}_s": round(est, 2),
+                         "method": "oracle construction (index init + 2 sorts + populate, one thread, "
+                                   "g++ -O2) timed on the sample, scaled by n log n"},
+               "cfg0_reference": ref0}
+        log(f"[bench] CPU {nsP / secsP:.1f} q/s on {threads} threads ({ns1 / secs1:.1f} on one), "
+            f"{model}, build sample {bsecs:.2f} s; reference 2^20: {ref0}")
 
     tree.close()
-    del d_pts, out
+    del d_pts
     torch.cuda.empty_cache()
     iso = iso_bench(args, P, D, torch, dev, stream) if (args.iso_side > 0 or args.volume_file) else None
     torch.cuda.empty_cache()
     intb = int_batch_bench(args, P, D, torch, dev) if args.int_log2n > 0 else None
     torch.cuda.empty_cache()
-    serve = serve_bench(args, P, D, torch, dev) if args.serve_qps > 0 else None
+    serve = serve_bench(args, P, D, torch, dev) if args.serve_qps else None
+    calc = P.StaticPSTGPU.calcGlobalMemNeeded(n, "f32", "u32")
+    frac = achieved / HBM_PEAK_GBPS
     res = {
         "metric": "queries/s + reported IDs/s (GB/s vs HBM peak), 2^26-pt PST; p50/p99 latency",
         "value": round(qps, 2),
@@ -737,36 +957,48 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded SplitMix64 generator, pst_data.py)",
         "config": {
-            "workload": f"batched search on a 2^{args.log2n}-point f32/u32 StaticPSTGPU: "
+            "workload": f"batched search on a 2^{args.log2n}-point f32/u32 StaticPSTGPU, tree walk: "
                         f"{args.nq2} twoSidedLeftSearch(q, q) + {args.nq3} threeSidedSearch per step, "
                         "count + scan + write, IDs returned",
             "points": n, "queries_per_step": nq, "reported_ids_per_step": total,
         },
+        "path": "tree walk",
         "reported_ids_per_s": round(ids_per_s, 1),
-        "alg_GBps_step": round(ids_per_s * BYTES_PER_ID / 1e9, 1),
-        "path": "level index" if args.level_index else "tree walk",
         "count_pass_ms": round(t_count, 3),
         "write_pass_ms": round(t_write, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                     "kernel": kernel, "launches_per_pass": launches,
-                     "avg_launch_ms": round(t_write / launches, 3),
-                     "alg_bytes_per_launch": round(alg_bytes / launches),
-                     "alg_bytes_per_id": COPY_BYTES_PER_ID if args.level_index else BYTES_PER_ID,
+                     "unit": "GB/s", "frac": round(frac, 4), "traffic": traffic,
+                     "traffic_frac": traffic_frac,
+                     "alg_bytes_per_id": BYTES_PER_ID,
+                     "kernel": "write pass: k_search_round<float, unsigned int, false, true, 8> x 4 "
+                               "+ k_copy_tasks<float, unsigned int, false, 8> x 1",
+                     "launches_per_pass": WRITE_LAUNCHES,
+                     "avg_launch_ms": round(t_write / WRITE_LAUNCHES, 3),
+                     "alg_bytes_per_launch": round(total * BYTES_PER_ID / WRITE_LAUNCHES),
+                     "model_applies": bool(frac <= 1.0),
+                     "note": "achieved = reported IDs x 21 B / write-pass time (events on the launch stream); "
+                             "21 B prices a reported node's dim1, dim2, med, bitcode and ID reads and its write. "
+                             "Fully reported subtrees are copied from the per-slot ID array instead (8 B per ID), "
+                             "so a frac above 1 means the 21 B model does not apply; traffic_frac is the counter "
+                             "bytes over the same time",
                      "traffic_source": traffic_src},
         "level_index": index,
-        "walk": walk,
-        "cpu_baseline": cpu,
-        "plain_per_query_qps": round(plain_qps, 2),
-        "batched_over_plain": round(qps / plain_qps, 2),
+        "two_sided_mix": mixed,
+        "points_returned": points_fig,
+        "plain": {"walk": plain_walk, "level_index": index["plain"] if index else None},
+        "batched_over_plain_b": round(qps / plain_walk["b_preallocated_qps"], 2),
         "ungrouped_batch_qps": None if ungrouped_qps is None else round(ungrouped_qps, 2),
-        "right_qq_qps": round(len(rq) / right_s, 2),
-        "right_qq_ids_per_query": rtotal // len(rq),
+        "cpu_baseline": cpu,
+        "gpu_cfg0": gpu_cfg0,
         "ips_qps": round(1e3 / ips_ms, 2),
         "ips_GBps": round(float(ips_gbps), 1),
+        "ips_frac": round(float(ips_gbps) / HBM_PEAK_GBPS, 4),
         "build_ms": round(build_ms, 2),
         "first_build_ms": round(cold_build_ms, 2),
         "build_GBps_io_floor": round(build_bytes / (build_ms * 1e-3) / 1e9, 1),
+        "build_frac_io_floor": round(build_bytes / (build_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+        "memory": {"calc_global_mem_needed": calc, "tree_resident_measured": int(tree_bytes),
+                   "level_index_bytes": P.StaticPSTGPU.level_index_bytes(n, "f32", "u32")},
         "results_checked": bool(ok and (iso is None or iso["results_checked"])
                                 and (intb is None or intb["results_checked"])),
         "isosurface": iso,

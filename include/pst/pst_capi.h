@@ -389,6 +389,16 @@ int pst_server_step(pst_server* server);
 int pst_server_poll(pst_server* server, unsigned long long* tickets, size_t capacity, size_t* num);
 int pst_server_result(pst_server* server, unsigned long long ticket, void** res_d, size_t* num_res_elems);
 int pst_server_release(pst_server* server, unsigned long long ticket);
+/* Batched forms of submit, poll and release for drivers that handle many
+ * queries per loop: num records of the tree's query type submitted in
+ * order (tickets[i] for records[i]; stops at the first invalid record),
+ * completed tickets with their result counts (~0 for a ticket whose output
+ * exceeds the server's budget), and releases of num tickets. */
+int pst_server_submit_many(pst_server* server, int tree, const void* queries, size_t num, int ret,
+                           unsigned long long* tickets);
+int pst_server_poll_counts(pst_server* server, unsigned long long* tickets, unsigned long long* counts,
+                           size_t capacity, size_t* num);
+int pst_server_release_many(pst_server* server, const unsigned long long* tickets, size_t num);
 /* Create the server after building the trees' level indexes: creation
  * sizes every tree's search buffers for max_batch queries (so serving does
  * not allocate), and an index built later sizes its own on first use. */

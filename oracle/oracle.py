@@ -31,6 +31,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
 REF_BIN = os.path.join(HERE, "_ref", "pst_ref")
+REF_BENCH = os.path.join(HERE, "_ref", "ref_bench")
 
 # dtype codes shared with include/pst/pst_capi.h
 DT = {"f32": 0, "f64": 1, "i32": 2, "i64": 3, "u32": 4, "u64": 5, None: -1}
@@ -75,6 +76,9 @@ def lib():
         L.oracle_time_build.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                         ctypes.c_size_t, ctypes.c_void_p,
                                         ctypes.POINTER(ctypes.c_double)]
+        L.oracle_count_queries.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
+                                           ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -151,6 +155,20 @@ def time_queries(tr: Tree, kinds, min1s, max1s, min2s, threads: int = 1):
                               kinds.ctypes.data, a.ctypes.data, b.ctypes.data, c.ctypes.data,
                               threads, ctypes.byref(tot), ctypes.byref(secs))
     return tot.value, secs.value
+
+
+def count_queries(tr: Tree, kinds, min1s, max1s, min2s, threads: int = 1) -> np.ndarray:
+    """Per-query result counts of the oracle's walks (kinds 0 three-sided,
+    1 left, 2 right), over `threads` threads."""
+    T = NP[tr.t]
+    kinds = np.ascontiguousarray(kinds, dtype=np.int32)
+    a = np.ascontiguousarray(min1s, dtype=T)
+    b = np.ascontiguousarray(max1s, dtype=T)
+    c = np.ascontiguousarray(min2s, dtype=T)
+    out = np.zeros(len(kinds), np.uint64)
+    lib().oracle_count_queries(DT[tr.t], DT[tr.i], tr.blob.ctypes.data, tr.n, len(kinds), kinds.ctypes.data,
+                               a.ctypes.data, b.ctypes.data, c.ctypes.data, threads, out.ctypes.data)
+    return out.astype(np.int64)
 
 
 def time_build(pts: np.ndarray, t: str, i: Optional[str]):
@@ -307,6 +325,34 @@ def run_reference(pts: np.ndarray, t: str, i: Optional[str], queries, workdir: s
         results.append(rows)
         k += 1 + cnt
     return arrs, results
+
+
+def ref_bench_available() -> bool:
+    return os.path.exists(REF_BENCH)
+
+
+def time_reference(pts: np.ndarray, kinds, min1s, max1s, min2s, threads, workdir: str) -> dict:
+    """Times the reference's own StaticPSTCPUIter (oracle/_ref/ref_bench,
+    f32 points with u32 IDs): construction of `pts`, then the query list
+    once per entry of `threads` (IDs returned). Returns the driver's JSON:
+    {"n", "nq", "build_s", "runs": [{"threads", "secs", "ids"}]}."""
+    import json
+    n = len(pts)
+    path = os.path.join(workdir, "ref_bench_in.bin")
+    nq = len(kinds)
+    rec = np.zeros(nq, np.dtype([("kind", np.int32), ("rp", np.int32), ("a", np.float32), ("b", np.float32),
+                                 ("c", np.float32)]))
+    rec["kind"], rec["a"], rec["b"], rec["c"] = kinds, min1s, max1s, min2s
+    with open(path, "wb") as f:
+        f.write(b"PSTR" + struct.pack("<iiQQ", DT["f32"], DT["u32"], n, nq))
+        f.write(np.ascontiguousarray(pts).tobytes())
+        f.write(rec.tobytes())
+    try:
+        out = subprocess.run([REF_BENCH, path, *[str(t) for t in threads]], check=True, capture_output=True,
+                             text=True).stdout
+    finally:
+        os.remove(path)
+    return json.loads(out.strip().splitlines()[-1])
 
 
 def _parse_tree(tree_lines, arrs, T, IT):

@@ -193,6 +193,39 @@ int oracle_time_queries(int dt, int idt, void* blob, size_t n, size_t nq, const 
   });
 }
 
+// Per-query result counts of the oracle's stack walks for a query list
+// (kinds[q], bounds of type T), over `threads` std::threads sharing the
+// tree: counts[q] = number of nodes the walk reports.
+int oracle_count_queries(int dt, int idt, void* blob, size_t n, size_t nq, const int32_t* kinds,
+                         const void* min1s, const void* max1s, const void* min2s, int threads,
+                         uint64_t* counts) {
+  return dispatch_t(dt, [&](auto t) {
+    using T = decltype(t);
+    return dispatch_id(idt, [&](auto i) {
+      using I = IdOf<decltype(i)>;
+      Layout L = layout_for<T, I>(n);
+      TreeView<T, I> tv{static_cast<unsigned char*>(blob), L};
+      Searcher<T, I> s{tv, n};
+      const T* a = static_cast<const T*>(min1s);
+      const T* b = static_cast<const T*>(max1s);
+      const T* c = static_cast<const T*>(min2s);
+      const int P = threads < 1 ? 1 : threads;
+      auto worker = [&](int w) {
+        for (size_t q = w; q < nq; q += P) {
+          uint64_t k = 0;
+          Code code = kinds[q] == 0 ? THREE_SEARCH : (kinds[q] == 1 ? LEFT_SEARCH : RIGHT_SEARCH);
+          s.run(code, a[q], b[q], c[q], [&](long long) { k++; });
+          counts[q] = k;
+        }
+      };
+      std::vector<std::thread> th;
+      for (int w = 0; w < P; w++) th.emplace_back(worker, w);
+      for (auto& x : th) x.join();
+      return 0;
+    });
+  });
+}
+
 // Times the oracle's construction of n points (index init + 2 sorts +
 // populate), the span the reference's CONSTR_TIMED blocks cover.
 int oracle_time_build(int dt, int idt, const void* pts, size_t n, void* blob, double* secs) {

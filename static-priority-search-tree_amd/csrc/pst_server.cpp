@@ -395,6 +395,18 @@ int pst_server_submit(pst_server* s, int tree, const void* query, int ret, unsig
   return PST_OK;
 }
 
+int pst_server_submit_many(pst_server* s, int tree, const void* queries, size_t num, int ret,
+                           unsigned long long* tickets) {
+  if (!s || (num && (!queries || !tickets)) || tree < 0 || tree >= static_cast<int>(s->trees.size()))
+    return sfail(PST_ERR_INVALID, "Error: invalid submission");
+  const size_t rec = s->trees[tree].qrec;
+  for (size_t i = 0; i < num; i++) {
+    int rc = pst_server_submit(s, tree, static_cast<const unsigned char*>(queries) + i * rec, ret, &tickets[i]);
+    if (rc != PST_OK) return rc;
+  }
+  return PST_OK;
+}
+
 int pst_server_step(pst_server* s) {
   if (!s) return sfail(PST_ERR_INVALID, "Error: null server");
   for (auto& tq : s->trees) {
@@ -411,6 +423,20 @@ int pst_server_poll(pst_server* s, unsigned long long* tickets, size_t capacity,
   if (!s || !num) return sfail(PST_ERR_INVALID, "Error: null argument");
   const size_t k = std::min(capacity, s->done.size());
   for (size_t i = 0; i < k; i++) tickets[i] = s->done[i];
+  s->done.erase(s->done.begin(), s->done.begin() + k);
+  *num = k;
+  return PST_OK;
+}
+
+int pst_server_poll_counts(pst_server* s, unsigned long long* tickets, unsigned long long* counts, size_t capacity,
+                           size_t* num) {
+  if (!s || !num || (capacity && (!tickets || !counts))) return sfail(PST_ERR_INVALID, "Error: null argument");
+  const size_t k = std::min(capacity, s->done.size());
+  for (size_t i = 0; i < k; i++) {
+    const Ticket& tk = s->tickets[s->done[i]];
+    tickets[i] = s->done[i];
+    counts[i] = tk.failed ? ~0ull : tk.count;
+  }
   s->done.erase(s->done.begin(), s->done.begin() + k);
   *num = k;
   return PST_OK;
@@ -442,6 +468,15 @@ int pst_server_release(pst_server* s, unsigned long long ticket) {
   if (r != s->range_refs.end() && --r->second == 0) {
     s->pool->release(range);
     s->range_refs.erase(r);
+  }
+  return PST_OK;
+}
+
+int pst_server_release_many(pst_server* s, const unsigned long long* tickets, size_t num) {
+  if (!s || (num && !tickets)) return sfail(PST_ERR_INVALID, "Error: null argument");
+  for (size_t i = 0; i < num; i++) {
+    int rc = pst_server_release(s, tickets[i]);
+    if (rc != PST_OK) return rc;
   }
   return PST_OK;
 }

@@ -39,6 +39,7 @@ EXPORTED = [
     "pst_synth_gaussian_volume", "pst_gpu_dtype", "pst_gpu_id_dtype",
     "pst_server_create", "pst_server_destroy", "pst_server_submit", "pst_server_step", "pst_server_poll",
     "pst_server_result", "pst_server_release", "pst_server_outstanding", "pst_server_last_error",
+    "pst_server_submit_many", "pst_server_poll_counts", "pst_server_release_many",
     "pst_gpu_autotune", "pst_gpu_set_subtree_copy", "pst_gpu_batch_graph_create", "pst_gpu_batch_graph_launch", "pst_gpu_batch_graph_destroy",
     "pst_interval_parallel_search",
     "pst_interval_parallel_search_into",
@@ -118,6 +119,9 @@ def lib():
         L.pst_server_poll.argtypes = [vp, vp, sz, c.POINTER(sz)]
         L.pst_server_result.argtypes = [vp, c.c_ulonglong, c.POINTER(vp), c.POINTER(sz)]
         L.pst_server_release.argtypes = [vp, c.c_ulonglong]
+        L.pst_server_submit_many.argtypes = [vp, i32, vp, sz, i32, vp]
+        L.pst_server_poll_counts.argtypes = [vp, vp, vp, sz, c.POINTER(sz)]
+        L.pst_server_release_many.argtypes = [vp, vp, sz]
         L.pst_server_outstanding.restype = sz
         L.pst_server_outstanding.argtypes = [vp]
         L.pst_server_last_error.restype = c.c_char_p
@@ -604,6 +608,25 @@ class Server:
 
     def release(self, ticket):
         self._chk(lib().pst_server_release(self.h, ticket))
+
+    def submit_many(self, tree, queries, ret=RET_ID):
+        """queries: contiguous records of query_dtype(tree's dtype); returns their tickets (uint64)."""
+        q = np.ascontiguousarray(queries)
+        t = np.zeros(len(q), np.uint64)
+        self._chk(lib().pst_server_submit_many(self.h, tree, q.ctypes.data, len(q), ret, t.ctypes.data))
+        return t
+
+    def poll_counts(self, capacity=1 << 16):
+        """Completed tickets and their result counts (~0 for a failed ticket)."""
+        t = np.zeros(capacity, np.uint64)
+        c = np.zeros(capacity, np.uint64)
+        n = ctypes.c_size_t()
+        self._chk(lib().pst_server_poll_counts(self.h, t.ctypes.data, c.ctypes.data, capacity, ctypes.byref(n)))
+        return t[:n.value], c[:n.value]
+
+    def release_many(self, tickets):
+        t = np.ascontiguousarray(tickets, dtype=np.uint64)
+        self._chk(lib().pst_server_release_many(self.h, t.ctypes.data, len(t)))
 
     def outstanding(self):
         return lib().pst_server_outstanding(self.h)

@@ -39,8 +39,7 @@ def _tree(P, n, seed, index):
 def _expected_counts(pts, qs):
     out = []
     for q in qs:
-        out.append(int(np.count_nonzero(O.brute_force(pts, int(q["kind"]), q["min_dim1"], q["max_dim1"],
-                                                      q["min_dim2"]))))
+        out.append(len(O.brute_force(pts, int(q["kind"]), q["min_dim1"], q["max_dim1"], q["min_dim2"])))
     return np.array(out, np.int64)
 
 
