@@ -142,7 +142,11 @@ int pst_gpu_search_into(pst_tree* tree, int kind, int ret, const void* min_dim1_
 
 /* Replaces StaticPSTGPU::calcGlobalMemNeeded(num_elems)
  * (pst/static-pst-gpu.h:132, pst/static-pst-gpu.tu:646-671): bytes of
- * device memory construction and one blocking search need. */
+ * device memory a tree needs: its resident arrays (node records, subtree
+ * minima, IDs) plus the larger of construction's buffers (freed when it
+ * ends) and one blocking search (the handle's search buffers, kept, and a
+ * result array of num_elems points). A level index, built on request, adds
+ * pst_gpu_level_index_bytes on top. */
 size_t pst_gpu_calc_global_mem_needed(size_t num_elems, int dtype, int id_dtype);
 
 /* The tree in the reference's memory layout, copied to host memory:
@@ -268,6 +272,32 @@ int pst_isosurface_write(const void* vertex_arr_d, int dtype, const size_t pt_gr
 int pst_isosurface(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3], const size_t metacell_dims[3],
                    const void* metacell_ids_d, int id_dtype, size_t num_ids, const void* isovalue,
                    float** triangles_d, size_t* num_triangles);
+
+/* Marching cubes over the active metacells: the consumer the reference
+ * anticipates for the IDs of an isovalue search (its sample
+ * .marching-cubes/marchingCubes_kernel.cu:155-434 classifies voxels, scans
+ * their triangle counts and generates the triangles of a 256-case table).
+ * Same arguments, ID rules, output layout (9 floats per triangle, grid
+ * coordinates) and blocking/asynchronous forms as pst_isosurface_* above,
+ * with one patch per voxel instead of six tetrahedra: corner k of a voxel
+ * (x = k & 1, y = k >> 1 & 1, z = k >> 2) is below when its value is <
+ * isovalue; on each cube face every run of below corners is cut off by its
+ * own segment (two diagonal below corners stay apart), so neighbouring
+ * voxels cut shared faces alike and the mesh has no cracks; the segments
+ * close into loops that are fanned into triangles wound with their normal
+ * toward values >= isovalue; surface vertices interpolate along grid edges
+ * from the endpoint of lower linear index. Triangles of a metacell come in
+ * voxel order (x fastest). At most 5 triangles per voxel. */
+int pst_marching_cubes_count(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3],
+                             const size_t metacell_dims[3], const void* metacell_ids_d, int id_dtype, size_t num_ids,
+                             const void* isovalue, unsigned long long* counts_d, pst_stream_t stream);
+int pst_marching_cubes_write(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3],
+                             const size_t metacell_dims[3], const void* metacell_ids_d, int id_dtype, size_t num_ids,
+                             const void* isovalue, const unsigned long long* offsets_d, float* triangles_d,
+                             size_t capacity, pst_stream_t stream);
+int pst_marching_cubes(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3],
+                       const size_t metacell_dims[3], const void* metacell_ids_d, int id_dtype, size_t num_ids,
+                       const void* isovalue, float** triangles_d, size_t* num_triangles);
 
 /* Synthetic f32 volume (no counterpart in the reference; a seeded input of
  * the isosurface workload): value = sum_g amp_g * exp(-|p - c_g|^2 /

@@ -489,15 +489,18 @@ int pst_gpu_search_into(pst_tree* t, int kind, int ret, const void* min1, const 
 
 size_t pst_gpu_calc_global_mem_needed(size_t n, int dtype, int id_dtype) {
   if (!valid_t(dtype) || !valid_id(id_dtype)) return 0;
+  if (n == 0) return 0;
   const size_t S = slots_for(n);
+  // resident: node records, subtree minima and the per-slot ID array
   const size_t tree = S * (pst::node_size(dtype, id_dtype) + pst::dtype_size(dtype) + id_size(id_dtype));
-  // construction: two key arrays, two 32-bit value arrays, order and rank
-  // arrays, and radix-sort storage (about one more key array)
-  const size_t key = (pst::dtype_size(dtype) == 4 && id_size(id_dtype) == 0) ? 4 : 8;
-  const size_t construct = n * (3 * key + 4 * 4);
-  // search: a result array of n points, two 4M-entry work lists and a
-  // 4M-entry copy task list
-  const size_t search = n * pst::point_size(dtype, id_dtype) + (3ull << 22) * 16;
+  // construction's buffers, freed when it ends
+  const size_t construct = pst::build_scratch_bytes(dtype, id_dtype, n);
+  // one blocking search: the handle's search buffers (kept) and a result
+  // array of n points (the reference's allocation, pst/static-pst-gpu.tu:563)
+  int dev = 0, cus = 0;
+  hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+  const size_t search = pst::search_scratch_bytes(n, dtype, cus) + n * pst::point_size(dtype, id_dtype);
   return tree + (construct > search ? construct : search);
 }
 
@@ -959,9 +962,15 @@ int iso_args(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3], 
 }
 }  // namespace
 
-int pst_isosurface_count(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3],
-                         const size_t metacell_dims[3], const void* metacell_ids_d, int id_dtype, size_t num_ids,
-                         const void* isovalue, unsigned long long* counts_d, pst_stream_t stream) {
+namespace {
+using IsoCountFn = pst::Status (*)(int, int, const void*, const uint32_t*, const uint32_t*, const void*, size_t,
+                                   const void*, unsigned long long*, unsigned long long*, hipStream_t);
+using IsoWriteFn = pst::Status (*)(int, int, const void*, const uint32_t*, const uint32_t*, const void*, size_t,
+                                   const void*, const unsigned long long*, size_t, float*, hipStream_t);
+
+int surface_count(IsoCountFn fn, const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3],
+                  const size_t metacell_dims[3], const void* metacell_ids_d, int id_dtype, size_t num_ids,
+                  const void* isovalue, unsigned long long* counts_d, pst_stream_t stream) {
   uint32_t p32[3], m32[3];
   int rc = iso_args(vertex_arr_d, dtype, pt_grid_dims, metacell_dims, metacell_ids_d, id_dtype, num_ids, isovalue,
                     p32, m32);
@@ -971,28 +980,28 @@ int pst_isosurface_count(const void* vertex_arr_d, int dtype, const size_t pt_gr
   hipGetDevice(&dev);
   unsigned long long* err = device_counter(dev);
   if (!err) return fail(PST_ERR_HIP, "Error in allocating a device counter");
-  Status st = pst::iso_count(dtype, id_dtype, vertex_arr_d, p32, m32, metacell_ids_d, num_ids, isovalue, counts_d,
-                             err, reinterpret_cast<hipStream_t>(stream));
+  Status st = fn(dtype, id_dtype, vertex_arr_d, p32, m32, metacell_ids_d, num_ids, isovalue, counts_d, err,
+                 reinterpret_cast<hipStream_t>(stream));
   return st.ok() ? PST_OK : fail(st);
 }
 
-int pst_isosurface_write(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3],
-                         const size_t metacell_dims[3], const void* metacell_ids_d, int id_dtype, size_t num_ids,
-                         const void* isovalue, const unsigned long long* offsets_d, float* triangles_d,
-                         size_t capacity, pst_stream_t stream) {
+int surface_write(IsoWriteFn fn, const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3],
+                  const size_t metacell_dims[3], const void* metacell_ids_d, int id_dtype, size_t num_ids,
+                  const void* isovalue, const unsigned long long* offsets_d, float* triangles_d, size_t capacity,
+                  pst_stream_t stream) {
   uint32_t p32[3], m32[3];
   int rc = iso_args(vertex_arr_d, dtype, pt_grid_dims, metacell_dims, metacell_ids_d, id_dtype, num_ids, isovalue,
                     p32, m32);
   if (rc != PST_OK) return rc;
   if (num_ids && (!offsets_d || (capacity && !triangles_d))) return fail(PST_ERR_INVALID, "Error: null argument");
-  Status st = pst::iso_write(dtype, id_dtype, vertex_arr_d, p32, m32, metacell_ids_d, num_ids, isovalue, offsets_d,
-                             capacity, triangles_d, reinterpret_cast<hipStream_t>(stream));
+  Status st = fn(dtype, id_dtype, vertex_arr_d, p32, m32, metacell_ids_d, num_ids, isovalue, offsets_d, capacity,
+                 triangles_d, reinterpret_cast<hipStream_t>(stream));
   return st.ok() ? PST_OK : fail(st);
 }
 
-int pst_isosurface(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3], const size_t metacell_dims[3],
-                   const void* metacell_ids_d, int id_dtype, size_t num_ids, const void* isovalue,
-                   float** triangles_d, size_t* num_triangles) {
+int surface_blocking(IsoCountFn cfn, IsoWriteFn wfn, const void* vertex_arr_d, int dtype,
+                     const size_t pt_grid_dims[3], const size_t metacell_dims[3], const void* metacell_ids_d,
+                     int id_dtype, size_t num_ids, const void* isovalue, float** triangles_d, size_t* num_triangles) {
   if (!triangles_d || !num_triangles) return fail(PST_ERR_INVALID, "Error: null argument");
   *triangles_d = nullptr;
   *num_triangles = 0;
@@ -1009,9 +1018,7 @@ int pst_isosurface(const void* vertex_arr_d, int dtype, const size_t pt_grid_dim
   if (e != hipSuccess) return fail(PST_ERR_HIP, std::string("Error in allocating isosurface counts: ") + hipGetErrorString(e));
   unsigned long long* offsets = counts + num_ids;
   Status st = pst::hip_status(hipMemsetAsync(err, 0, 8, nullptr), "Error in clearing the error flag");
-  if (st.ok())
-    st = pst::iso_count(dtype, id_dtype, vertex_arr_d, p32, m32, metacell_ids_d, num_ids, isovalue, counts, err,
-                        nullptr);
+  if (st.ok()) st = cfn(dtype, id_dtype, vertex_arr_d, p32, m32, metacell_ids_d, num_ids, isovalue, counts, err, nullptr);
   if (st.ok()) st = pst::exclusive_scan(counts, offsets, num_ids, nullptr);
   unsigned long long total = 0, bad = 0;
   if (st.ok())
@@ -1023,8 +1030,7 @@ int pst_isosurface(const void* vertex_arr_d, int dtype, const size_t pt_grid_dim
   if (st.ok() && total)
     st = pst::hip_status(hipMalloc(&tri, total * 9 * sizeof(float)), "Error in allocating the triangle array");
   if (st.ok() && total)
-    st = pst::iso_write(dtype, id_dtype, vertex_arr_d, p32, m32, metacell_ids_d, num_ids, isovalue, offsets, total,
-                        tri, nullptr);
+    st = wfn(dtype, id_dtype, vertex_arr_d, p32, m32, metacell_ids_d, num_ids, isovalue, offsets, total, tri, nullptr);
   if (st.ok()) st = pst::hip_status(hipStreamSynchronize(nullptr), "Error in synchronising after isosurface extraction");
   hipFree(counts);
   if (!st.ok()) {
@@ -1034,6 +1040,51 @@ int pst_isosurface(const void* vertex_arr_d, int dtype, const size_t pt_grid_dim
   *triangles_d = tri;
   *num_triangles = total;
   return PST_OK;
+}
+}  // namespace
+
+int pst_isosurface_count(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3],
+                         const size_t metacell_dims[3], const void* metacell_ids_d, int id_dtype, size_t num_ids,
+                         const void* isovalue, unsigned long long* counts_d, pst_stream_t stream) {
+  return surface_count(pst::iso_count, vertex_arr_d, dtype, pt_grid_dims, metacell_dims, metacell_ids_d, id_dtype,
+                       num_ids, isovalue, counts_d, stream);
+}
+
+int pst_isosurface_write(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3],
+                         const size_t metacell_dims[3], const void* metacell_ids_d, int id_dtype, size_t num_ids,
+                         const void* isovalue, const unsigned long long* offsets_d, float* triangles_d,
+                         size_t capacity, pst_stream_t stream) {
+  return surface_write(pst::iso_write, vertex_arr_d, dtype, pt_grid_dims, metacell_dims, metacell_ids_d, id_dtype,
+                       num_ids, isovalue, offsets_d, triangles_d, capacity, stream);
+}
+
+int pst_isosurface(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3], const size_t metacell_dims[3],
+                   const void* metacell_ids_d, int id_dtype, size_t num_ids, const void* isovalue,
+                   float** triangles_d, size_t* num_triangles) {
+  return surface_blocking(pst::iso_count, pst::iso_write, vertex_arr_d, dtype, pt_grid_dims, metacell_dims,
+                          metacell_ids_d, id_dtype, num_ids, isovalue, triangles_d, num_triangles);
+}
+
+int pst_marching_cubes_count(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3],
+                             const size_t metacell_dims[3], const void* metacell_ids_d, int id_dtype, size_t num_ids,
+                             const void* isovalue, unsigned long long* counts_d, pst_stream_t stream) {
+  return surface_count(pst::mc_count, vertex_arr_d, dtype, pt_grid_dims, metacell_dims, metacell_ids_d, id_dtype,
+                       num_ids, isovalue, counts_d, stream);
+}
+
+int pst_marching_cubes_write(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3],
+                             const size_t metacell_dims[3], const void* metacell_ids_d, int id_dtype, size_t num_ids,
+                             const void* isovalue, const unsigned long long* offsets_d, float* triangles_d,
+                             size_t capacity, pst_stream_t stream) {
+  return surface_write(pst::mc_write, vertex_arr_d, dtype, pt_grid_dims, metacell_dims, metacell_ids_d, id_dtype,
+                       num_ids, isovalue, offsets_d, triangles_d, capacity, stream);
+}
+
+int pst_marching_cubes(const void* vertex_arr_d, int dtype, const size_t pt_grid_dims[3],
+                       const size_t metacell_dims[3], const void* metacell_ids_d, int id_dtype, size_t num_ids,
+                       const void* isovalue, float** triangles_d, size_t* num_triangles) {
+  return surface_blocking(pst::mc_count, pst::mc_write, vertex_arr_d, dtype, pt_grid_dims, metacell_dims,
+                          metacell_ids_d, id_dtype, num_ids, isovalue, triangles_d, num_triangles);
 }
 
 int pst_form_metacell_tags(const void* vertex_arr_d, int dtype, int id_dtype, const size_t pt_grid_dims[3],

@@ -40,6 +40,13 @@ struct BuildTimes {
 Status build_tree(int dtype, int id_dtype, const void* pts_d, size_t n, void* nodes_d, hipStream_t stream,
                   BuildTimes* times);
 
+// Device bytes one construction of n points allocates (freed when it ends).
+size_t build_scratch_bytes(int dtype, int id_dtype, size_t n);
+// Device bytes a handle keeps for one search of a deep tree of n points
+// (work lists, copy tasks, the overflow stacks of the default grid on a GPU
+// with `cus` compute units), allocated at its first search.
+size_t search_scratch_bytes(size_t n, int dtype, int cus);
+
 // Reference-layout blob (dim1, dim2, med, ids, bitcodes) written to a
 // device buffer, for export / parity checks.
 Status export_tree(int dtype, int id_dtype, size_t n, const void* nodes_d, void* blob_d, size_t off_dim1,
@@ -196,6 +203,15 @@ Status iso_count(int dtype, int id_dtype, const void* vertices_d, const uint32_t
 Status iso_write(int dtype, int id_dtype, const void* vertices_d, const uint32_t pt_dims[3],
                  const uint32_t mc_dims[3], const void* ids_d, size_t nids, const void* isovalue,
                  const unsigned long long* offsets_d, size_t capacity, float* tri_d, hipStream_t stream);
+
+// Marching cubes over the listed metacells: the same contract as iso_count
+// and iso_write, one surface patch per voxel from the 256-case table.
+Status mc_count(int dtype, int id_dtype, const void* vertices_d, const uint32_t pt_dims[3],
+                const uint32_t mc_dims[3], const void* ids_d, size_t nids, const void* isovalue,
+                unsigned long long* counts_d, unsigned long long* err_d, hipStream_t stream);
+Status mc_write(int dtype, int id_dtype, const void* vertices_d, const uint32_t pt_dims[3],
+                const uint32_t mc_dims[3], const void* ids_d, size_t nids, const void* isovalue,
+                const unsigned long long* offsets_d, size_t capacity, float* tri_d, hipStream_t stream);
 
 // ---- tree files and raw volumes (pst_persist.hip) ----
 struct TreeFileInfo {

@@ -235,6 +235,187 @@ __global__ __launch_bounds__(kIsoWave) void k_iso_write(const T* __restrict__ vo
   }
 }
 
+// ---------------------------------------------------------------------------
+// Marching cubes: one surface patch per voxel from a 256-case table over the
+// 8 corner signs (corner k below when f < q). The table is generated at
+// compile time from one rule, so no case list is transcribed:
+//  - on each face of the cube, walked counter-clockwise as seen from outside,
+//    every maximal run of below corners is cut off by one segment, from the
+//    edge where the walk leaves the run to the edge where it entered it; a
+//    face with two diagonal below corners thus gets two segments, each
+//    cutting off one corner (the below corners stay apart), and the choice
+//    depends on the face's corner signs alone, so the two voxels that share
+//    a face cut it the same way and the surface has no cracks;
+//  - every crossing edge starts one segment and ends one (the two faces of
+//    an edge walk it in opposite directions), so the segments form closed
+//    loops; each loop becomes a triangle fan, wound so that the normal
+//    points from below to above q.
+// Edges are numbered by axis: x edges 0-3 (corner pairs 0-1, 2-3, 4-5,
+// 6-7), y edges 4-7 (0-2, 1-3, 4-6, 5-7), z edges 8-11 (0-4, 1-5, 2-6, 3-7).
+// Surface vertices interpolate from the edge endpoint of lower linear
+// index, as for the tetrahedra, so shared edges give identical vertices.
+// ---------------------------------------------------------------------------
+constexpr int kMcMaxTri = 5;
+struct McCase {
+  uint8_t ntri;
+  uint8_t e[3 * kMcMaxTri];  // edge triples of the triangles
+};
+struct McTable {
+  McCase c[256];
+  int maxtri;
+};
+constexpr uint8_t kMcEdgeLo[12] = {0, 2, 4, 6, 0, 1, 4, 5, 0, 1, 2, 3};
+constexpr uint8_t kMcEdgeHi[12] = {1, 3, 5, 7, 2, 3, 6, 7, 4, 5, 6, 7};
+
+constexpr int mc_edge_of(int a, int b) {
+  for (int e = 0; e < 12; e++)
+    if ((kMcEdgeLo[e] == a && kMcEdgeHi[e] == b) || (kMcEdgeLo[e] == b && kMcEdgeHi[e] == a)) return e;
+  return -1;
+}
+
+constexpr McTable make_mc_table() {
+  McTable T{};
+  // faces: axis bit, side, and corner offsets u, v with u x v along the
+  // outward normal (x = 1, y = 2, z = 4)
+  const int ax[6] = {1, 1, 2, 2, 4, 4};
+  const int side[6] = {0, 1, 0, 1, 0, 1};
+  const int uu[6] = {4, 2, 1, 4, 2, 1};
+  const int vv[6] = {2, 4, 4, 1, 1, 2};
+  for (int cfg = 0; cfg < 256; cfg++) {
+    int next[12] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+    for (int f = 0; f < 6; f++) {
+      const int b = side[f] ? ax[f] : 0;
+      const int c[4] = {b, b | uu[f], b | uu[f] | vv[f], b | vv[f]};
+      bool in[4] = {};
+      for (int i = 0; i < 4; i++) in[i] = (cfg >> c[i]) & 1;
+      for (int i = 0; i < 4; i++) {
+        if (!in[i] || in[(i + 1) % 4]) continue;  // the walk leaves a run at corner i
+        int j = i;
+        while (in[(j + 3) % 4]) j = (j + 3) % 4;  // back to the run's first corner
+        next[mc_edge_of(c[i], c[(i + 1) % 4])] = mc_edge_of(c[(j + 3) % 4], c[j]);
+      }
+    }
+    bool seen[12] = {};
+    int nt = 0;
+    for (int e0 = 0; e0 < 12; e0++) {
+      if (next[e0] < 0 || seen[e0]) continue;
+      int cyc[12] = {};
+      int len = 0;
+      for (int e = e0; !seen[e]; e = next[e]) {
+        seen[e] = true;
+        cyc[len++] = e;
+      }
+      for (int k = 1; k + 1 < len; k++) {
+        if (nt < kMcMaxTri) {
+          T.c[cfg].e[3 * nt] = static_cast<uint8_t>(cyc[0]);
+          T.c[cfg].e[3 * nt + 1] = static_cast<uint8_t>(cyc[k + 1]);
+          T.c[cfg].e[3 * nt + 2] = static_cast<uint8_t>(cyc[k]);
+        }
+        nt++;
+      }
+    }
+    T.c[cfg].ntri = static_cast<uint8_t>(nt);
+    if (nt > T.maxtri) T.maxtri = nt;
+  }
+  return T;
+}
+
+constexpr McTable kMcHost = make_mc_table();
+static_assert(kMcHost.maxtri == kMcMaxTri, "marching cubes table capacity");
+__constant__ McTable kMc = make_mc_table();
+
+template <typename T>
+__device__ inline uint32_t mc_config(const T f[8], T q) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) m |= (f[k] < q ? 1u : 0u) << k;
+  return m;
+}
+
+template <typename T, typename I>
+__global__ __launch_bounds__(kIsoWave) void k_mc_count(const T* __restrict__ vol, IsoGrid g,
+                                                        const I* __restrict__ ids, uint64_t nids, T q,
+                                                        unsigned long long* __restrict__ counts,
+                                                        unsigned long long* __restrict__ err) {
+  const uint32_t lane = threadIdx.x;
+  const uint64_t ncells = static_cast<uint64_t>(g.gx) * g.gy * g.gz;
+  for (uint64_t i = blockIdx.x; i < nids; i += gridDim.x) {
+    const uint64_t id = static_cast<uint64_t>(ids[i]);
+    unsigned long long n = 0;
+    if (id >= ncells) {
+      if (lane == 0) atomicOr(err, 1ull);
+    } else {
+      uint32_t nvox = 1;
+      for (uint32_t j0 = 0; j0 < nvox; j0 += kIsoWave) {
+        Voxel v;
+        if (metacell_voxel(g, id, j0 + lane, v, nvox)) {
+          T f[8];
+          load_corners(vol, g.px, g.py, v, f);
+          n += kMc.c[mc_config(f, q)].ntri;
+        }
+      }
+    }
+    for (int o = kIsoWave / 2; o > 0; o >>= 1) n += __shfl_xor(n, o, kIsoWave);
+    if (lane == 0) counts[i] = n;
+  }
+}
+
+// One wave per metacell; per chunk of 64 voxels the lanes' triangles are
+// staged in LDS at their prefix positions (voxel order) and written out as
+// one contiguous run. Order within a metacell: voxel, then the voxel's
+// table order.
+template <typename T, typename I>
+__global__ __launch_bounds__(kIsoWave) void k_mc_write(const T* __restrict__ vol, IsoGrid g,
+                                                        const I* __restrict__ ids, uint64_t nids, T q,
+                                                        const unsigned long long* __restrict__ offsets,
+                                                        unsigned long long capacity, float* __restrict__ out) {
+  __shared__ float stage[kIsoWave * kMcMaxTri * 9];
+  const uint32_t lane = threadIdx.x;
+  const uint64_t ncells = static_cast<uint64_t>(g.gx) * g.gy * g.gz;
+  for (uint64_t i = blockIdx.x; i < nids; i += gridDim.x) {
+    const uint64_t id = static_cast<uint64_t>(ids[i]);
+    if (id >= ncells) continue;
+    unsigned long long base = offsets[i];
+    uint32_t nvox = 1;
+    for (uint32_t j0 = 0; j0 < nvox; j0 += kIsoWave) {
+      Voxel v;
+      T f[8];
+      const bool have = metacell_voxel(g, id, j0 + lane, v, nvox);
+      uint32_t cfg = 0, n = 0;
+      if (have) {
+        load_corners(vol, g.px, g.py, v, f);
+        cfg = mc_config(f, q);
+        n = kMc.c[cfg].ntri;
+      }
+      uint32_t incl = n;
+      for (int o = 1; o < kIsoWave; o <<= 1) {
+        const uint32_t u = __shfl_up(incl, o, kIsoWave);
+        if (static_cast<int>(lane) >= o) incl += u;
+      }
+      const uint32_t total = __shfl(incl, kIsoWave - 1, kIsoWave);
+      if (total == 0) continue;
+      float* my = stage + 9 * (incl - n);
+      for (uint32_t t = 0; t < n; t++) {
+        const uint8_t* e = kMc.c[cfg].e + 3 * t;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          const float3 p = edge_point(v, f, kMcEdgeLo[e[k]], kMcEdgeHi[e[k]], q);
+          my[9 * t + 3 * k] = p.x;
+          my[9 * t + 3 * k + 1] = p.y;
+          my[9 * t + 3 * k + 2] = p.z;
+        }
+      }
+      __syncthreads();
+      for (uint32_t k = lane; k < 9 * total; k += kIsoWave) {
+        const unsigned long long tri = base + k / 9;
+        if (tri < capacity) out[9 * base + k] = stage[k];
+      }
+      __syncthreads();
+      base += total;
+    }
+  }
+}
+
 template <typename F>
 Status with_t(int dt, F&& f) {
   switch (dt) {
@@ -275,6 +456,44 @@ unsigned iso_grid_blocks(size_t nids) {
 }
 
 }  // namespace
+
+Status mc_count(int dtype, int id_dtype, const void* vertices_d, const uint32_t pt_dims[3],
+                const uint32_t mc_dims[3], const void* ids_d, size_t nids, const void* isovalue,
+                unsigned long long* counts_d, unsigned long long* err_d, hipStream_t stream) {
+  IsoGrid g;
+  Status st = iso_grid(pt_dims, mc_dims, g);
+  if (!st.ok() || nids == 0) return st;
+  return with_t(dtype, [&](auto t) {
+    return with_idx(id_dtype, [&](auto i) {
+      using T = decltype(t);
+      using I = decltype(i);
+      T q;
+      std::memcpy(&q, isovalue, sizeof(T));
+      k_mc_count<T, I><<<iso_grid_blocks(nids), kIsoWave, 0, stream>>>(
+          static_cast<const T*>(vertices_d), g, static_cast<const I*>(ids_d), nids, q, counts_d, err_d);
+      return hip_status(hipGetLastError(), "Error in launching the marching cubes count");
+    });
+  });
+}
+
+Status mc_write(int dtype, int id_dtype, const void* vertices_d, const uint32_t pt_dims[3],
+                const uint32_t mc_dims[3], const void* ids_d, size_t nids, const void* isovalue,
+                const unsigned long long* offsets_d, size_t capacity, float* tri_d, hipStream_t stream) {
+  IsoGrid g;
+  Status st = iso_grid(pt_dims, mc_dims, g);
+  if (!st.ok() || nids == 0) return st;
+  return with_t(dtype, [&](auto t) {
+    return with_idx(id_dtype, [&](auto i) {
+      using T = decltype(t);
+      using I = decltype(i);
+      T q;
+      std::memcpy(&q, isovalue, sizeof(T));
+      k_mc_write<T, I><<<iso_grid_blocks(nids), kIsoWave, 0, stream>>>(
+          static_cast<const T*>(vertices_d), g, static_cast<const I*>(ids_d), nids, q, offsets_d, capacity, tri_d);
+      return hip_status(hipGetLastError(), "Error in launching the marching cubes write");
+    });
+  });
+}
 
 Status iso_count(int dtype, int id_dtype, const void* vertices_d, const uint32_t pt_dims[3],
                  const uint32_t mc_dims[3], const void* ids_d, size_t nids, const void* isovalue,

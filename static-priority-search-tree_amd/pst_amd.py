@@ -46,6 +46,7 @@ EXPORTED = [
     "pst_gpu_save", "pst_gpu_load", "pst_read_vertices", "pst_read_vertices_to_device",
     "pst_gpu_set_level_index", "pst_gpu_level_index_bytes",
     "pst_isosurface_count", "pst_isosurface_write", "pst_isosurface",
+    "pst_marching_cubes_count", "pst_marching_cubes_write", "pst_marching_cubes",
 ]
 ERR_IO = 6
 DT_NAME = {v: k for k, v in DT.items()}
@@ -129,6 +130,9 @@ def lib():
         L.pst_isosurface_count.argtypes = [vp, i32, dims3, dims3, vp, i32, sz, vp, vp, vp]
         L.pst_isosurface_write.argtypes = [vp, i32, dims3, dims3, vp, i32, sz, vp, vp, vp, sz, vp]
         L.pst_isosurface.argtypes = [vp, i32, dims3, dims3, vp, i32, sz, vp, c.POINTER(vp), c.POINTER(sz)]
+        L.pst_marching_cubes_count.argtypes = L.pst_isosurface_count.argtypes
+        L.pst_marching_cubes_write.argtypes = L.pst_isosurface_write.argtypes
+        L.pst_marching_cubes.argtypes = L.pst_isosurface.argtypes
         L.pst_calc_num_metacells.restype = sz
         L.pst_calc_num_metacells.argtypes = [dims3, dims3, dims3]
         L.pst_form_metacell_tags.argtypes = [vp, i32, i32, dims3, dims3, c.POINTER(vp), c.POINTER(sz), i32, i32]
@@ -462,16 +466,18 @@ def read_vertices_to_device(path: str, num_vertices: int, out_d, dtype: str = "f
 
 
 def isosurface(vertex_arr_d, pt_grid_dims, metacell_dims, ids_d, num_ids, isovalue, dtype="f32",
-               id_dtype="u32") -> np.ndarray:
-    """Triangles (n, 3, 3) of the isosurface in the listed metacells
-    (pst_isosurface: marching tetrahedra, grid coordinates)."""
+               id_dtype="u32", method="tetrahedra") -> np.ndarray:
+    """Triangles (n, 3, 3) of the isosurface in the listed metacells, grid
+    coordinates: method "tetrahedra" (pst_isosurface) or "cubes"
+    (pst_marching_cubes)."""
     vptr = vertex_arr_d.data_ptr() if hasattr(vertex_arr_d, "data_ptr") else vertex_arr_d
     iptr = ids_d.data_ptr() if hasattr(ids_d, "data_ptr") else ids_d
     dims3 = ctypes.c_size_t * 3
     q = _scalar(isovalue, dtype)
     p, n = ctypes.c_void_p(), ctypes.c_size_t()
-    _check(lib().pst_isosurface(vptr, DT[dtype], dims3(*pt_grid_dims), dims3(*metacell_dims), iptr, DT[id_dtype],
-                                num_ids, q.ctypes.data, ctypes.byref(p), ctypes.byref(n)))
+    fn = lib().pst_marching_cubes if method == "cubes" else lib().pst_isosurface
+    _check(fn(vptr, DT[dtype], dims3(*pt_grid_dims), dims3(*metacell_dims), iptr, DT[id_dtype],
+              num_ids, q.ctypes.data, ctypes.byref(p), ctypes.byref(n)))
     if not p.value:
         return np.zeros((0, 3, 3), np.float32)
     raw = _d2h(p.value, n.value * 36)
@@ -480,23 +486,26 @@ def isosurface(vertex_arr_d, pt_grid_dims, metacell_dims, ids_d, num_ids, isoval
 
 
 def isosurface_count(vertex_arr_d, pt_grid_dims, metacell_dims, ids_d, num_ids, isovalue, counts_d,
-                     dtype="f32", id_dtype="u32", stream=None):
-    """Triangles per listed metacell (pst_isosurface_count), asynchronous."""
+                     dtype="f32", id_dtype="u32", stream=None, method="tetrahedra"):
+    """Triangles per listed metacell (pst_isosurface_count, or
+    pst_marching_cubes_count for method "cubes"), asynchronous."""
     dims3 = ctypes.c_size_t * 3
     q = _scalar(isovalue, dtype)
-    _check(lib().pst_isosurface_count(vertex_arr_d.data_ptr(), DT[dtype], dims3(*pt_grid_dims), dims3(*metacell_dims),
-                                      ids_d.data_ptr(), DT[id_dtype], num_ids, q.ctypes.data, counts_d.data_ptr(),
-                                      _stream_ptr(stream)))
+    fn = lib().pst_marching_cubes_count if method == "cubes" else lib().pst_isosurface_count
+    _check(fn(vertex_arr_d.data_ptr(), DT[dtype], dims3(*pt_grid_dims), dims3(*metacell_dims),
+              ids_d.data_ptr(), DT[id_dtype], num_ids, q.ctypes.data, counts_d.data_ptr(), _stream_ptr(stream)))
 
 
 def isosurface_write(vertex_arr_d, pt_grid_dims, metacell_dims, ids_d, num_ids, isovalue, offsets_d, tri_d,
-                     capacity, dtype="f32", id_dtype="u32", stream=None):
-    """Triangles of the listed metacells at their offsets (pst_isosurface_write)."""
+                     capacity, dtype="f32", id_dtype="u32", stream=None, method="tetrahedra"):
+    """Triangles of the listed metacells at their offsets (pst_isosurface_write,
+    or pst_marching_cubes_write for method "cubes")."""
     dims3 = ctypes.c_size_t * 3
     q = _scalar(isovalue, dtype)
-    _check(lib().pst_isosurface_write(vertex_arr_d.data_ptr(), DT[dtype], dims3(*pt_grid_dims), dims3(*metacell_dims),
-                                      ids_d.data_ptr(), DT[id_dtype], num_ids, q.ctypes.data, offsets_d.data_ptr(),
-                                      tri_d.data_ptr(), capacity, _stream_ptr(stream)))
+    fn = lib().pst_marching_cubes_write if method == "cubes" else lib().pst_isosurface_write
+    _check(fn(vertex_arr_d.data_ptr(), DT[dtype], dims3(*pt_grid_dims), dims3(*metacell_dims),
+              ids_d.data_ptr(), DT[id_dtype], num_ids, q.ctypes.data, offsets_d.data_ptr(),
+              tri_d.data_ptr(), capacity, _stream_ptr(stream)))
 
 
 def intervalParallelSearch(pt_arr_d, num_elems, search_val, dtype="f32", id_dtype="u32",
