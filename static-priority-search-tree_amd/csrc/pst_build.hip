@@ -311,27 +311,135 @@ __device__ void lds_build(uint32_t* bufA, uint32_t* bufB, unsigned long long* sr
   }
 }
 
-struct GlobalPerm {
-  const uint32_t* perm2;
-  __device__ uint32_t operator()(uint32_t r) const { return perm2[r]; }
-};
+// The subtree build of the deep tree's last levels. Every point of the
+// subtree is looked up once up front (rank -> input index -> dim1), all
+// lookups in flight together; the levels then work on 16-bit local
+// positions: the node of a segment is the position of least rank, med is
+// read from LDS, and only the node's own record is read from the input.
+template <typename T, typename IS>
+__device__ void lds_build_local(uint16_t* bufA, uint16_t* bufB, const uint32_t* rank, const uint32_t* idx,
+                                const T* d1, unsigned long long* sres, unsigned long long* wred, uint32_t m,
+                                uint64_t root_slot, const Point<T, IS>* __restrict__ pts,
+                                Node<T, IS>* __restrict__ nodes) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nthr = blockDim.x;
+  const uint64_t Mp = static_cast<uint64_t>(m) + 1;
+  for (uint32_t u = 0; (Mp >> u) != 0; u++) {
+    const uint32_t nsub = 1u << u;
+    if (nsub <= nthr) {
+      const uint32_t G = nthr / nsub;
+      const uint32_t k = tid / G, g = tid % G;
+      const int64_t sz = level_size(Mp, u, k);
+      const uint32_t st = static_cast<uint32_t>(sz > 0 ? level_start(Mp, u, k) : 0);
+      unsigned long long best = ~0ull;
+      for (int64_t x = g; x < sz; x += G) {
+        const unsigned long long key =
+            (static_cast<unsigned long long>(rank[bufA[st + x]]) << 32) | static_cast<unsigned long long>(x);
+        best = key < best ? key : best;
+      }
+      const uint32_t wg = G < kWave ? G : kWave;
+      for (uint32_t off = wg >> 1; off >= 1; off >>= 1) {
+        const unsigned long long o = __shfl_xor(best, off, kWave);
+        best = o < best ? o : best;
+      }
+      if (G > kWave) {  // uniform across the block
+        if ((tid & (kWave - 1)) == 0) wred[tid / kWave] = best;
+        __syncthreads();
+        if (g == 0) {
+          const uint32_t w0 = tid / kWave, nw = G / kWave;
+          for (uint32_t w = 0; w < nw; w++) best = wred[w0 + w] < best ? wred[w0 + w] : best;
+        }
+      }
+      if (g == 0) sres[k] = best;
+      __syncthreads();
+      if (sz > 0) {
+        const unsigned long long mn = sres[k];
+        const uint32_t q = static_cast<uint32_t>(mn);
+        const uint32_t L = static_cast<uint32_t>(sz) >> 1;
+        if (g == 0) {
+          const Point<T, IS> p = pts[idx[bufA[st + q]]];
+          const T med = L >= 1 ? d1[bufA[st + (q < L ? L : L - 1)]] : p.dim1_val;
+          write_node(nodes, (root_slot + 1) * nsub - 1 + k, p, med);
+        }
+        const uint32_t dst0 = static_cast<uint32_t>(sz > 1 ? level_start(Mp, u + 1, 2 * k) : 0);
+        for (int64_t x = g; x < sz; x += G) {
+          if (static_cast<uint32_t>(x) == q) continue;
+          bufB[dst0 + x - (static_cast<uint32_t>(x) > q ? 1 : 0)] = bufA[st + x];
+        }
+      }
+    } else {
+      for (uint32_t k = tid; k < nsub; k += nthr) {
+        const int64_t sz = level_size(Mp, u, k);
+        if (sz <= 0) continue;
+        const uint32_t st = static_cast<uint32_t>(level_start(Mp, u, k));
+        unsigned long long best = ~0ull;
+        for (int64_t x = 0; x < sz; x++) {
+          const unsigned long long key =
+              (static_cast<unsigned long long>(rank[bufA[st + x]]) << 32) | static_cast<unsigned long long>(x);
+          best = key < best ? key : best;
+        }
+        const uint32_t q = static_cast<uint32_t>(best);
+        const uint32_t L = static_cast<uint32_t>(sz) >> 1;
+        const Point<T, IS> p = pts[idx[bufA[st + q]]];
+        const T med = L >= 1 ? d1[bufA[st + (q < L ? L : L - 1)]] : p.dim1_val;
+        write_node(nodes, (root_slot + 1) * nsub - 1 + k, p, med);
+        if (sz > 1) {
+          const uint32_t dst0 = static_cast<uint32_t>(level_start(Mp, u + 1, 2 * k));
+          for (int64_t x = 0; x < sz; x++) {
+            if (static_cast<uint32_t>(x) == q) continue;
+            bufB[dst0 + x - (static_cast<uint32_t>(x) > q ? 1 : 0)] = bufA[st + x];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    uint16_t* tmp = bufA;
+    bufA = bufB;
+    bufB = tmp;
+  }
+}
 
 template <typename T, typename IS>
 __global__ __launch_bounds__(kFinishBlock) void k_finish(
     const uint32_t* __restrict__ A, uint32_t t0, uint64_t M, const uint32_t* __restrict__ starts,
     const uint32_t* __restrict__ perm2, const Point<T, IS>* __restrict__ pts, Node<T, IS>* __restrict__ nodes) {
-  __shared__ uint32_t bufA[kFinishTile];
-  __shared__ uint32_t bufB[kFinishTile];
-  __shared__ unsigned long long sres[kFinishTile];
+  __shared__ uint16_t bufA[kFinishTile];
+  __shared__ uint16_t bufB[kFinishTile];
+  __shared__ uint32_t rank[kFinishTile];
+  __shared__ uint32_t idx[kFinishTile];
+  __shared__ T d1[kFinishTile];
+  __shared__ unsigned long long sres[kFinishBlock];  // one per segment of a level with <= kFinishBlock segments
   __shared__ unsigned long long wred[kFinishBlock / kWave];
   const uint32_t j = blockIdx.x;
   const int64_t m = level_size(M, t0, j);
   if (m <= 0) return;
   const uint32_t st = starts[j];
-  for (uint32_t x = threadIdx.x; x < m; x += blockDim.x) bufA[x] = A[st + x];
+  // rank -> input index -> dim1 for every point, all loads in flight together
+  constexpr int kPer = kFinishTile / kFinishBlock;
+  uint32_t r[kPer], ix[kPer];
+#pragma unroll
+  for (int e = 0; e < kPer; e++) {
+    const uint32_t x = threadIdx.x + e * kFinishBlock;
+    if (x < m) r[e] = A[st + x];
+  }
+#pragma unroll
+  for (int e = 0; e < kPer; e++) {
+    const uint32_t x = threadIdx.x + e * kFinishBlock;
+    if (x < m) ix[e] = perm2[r[e]];
+  }
+#pragma unroll
+  for (int e = 0; e < kPer; e++) {
+    const uint32_t x = threadIdx.x + e * kFinishBlock;
+    if (x < m) {
+      d1[x] = pts[ix[e]].dim1_val;
+      rank[x] = r[e];
+      idx[x] = ix[e];
+      bufA[x] = static_cast<uint16_t>(x);
+    }
+  }
   __syncthreads();
-  lds_build<T, IS>(bufA, bufB, sres, wred, static_cast<uint32_t>(m), (1ull << t0) - 1 + j, pts,
-                   GlobalPerm{perm2}, nodes);
+  lds_build_local<T, IS>(bufA, bufB, rank, idx, d1, sres, wred, static_cast<uint32_t>(m), (1ull << t0) - 1 + j,
+                         pts, nodes);
 }
 
 // ---------------------------------------------------------------------------
@@ -850,6 +958,17 @@ Status build_tree(int dtype, int id_dtype, const void* pts_d, size_t n, void* no
       return build_typed<decltype(t), decltype(i)>(pts_d, n, nodes_d, stream, times);
     });
   });
+}
+
+size_t build_scratch_bytes(int dtype, int id_dtype, size_t n) {
+  size_t bytes = 0;
+  Status st = with_t(dtype, [&](auto t) {
+    return with_id(id_dtype, [&](auto i) {
+      bytes = build_layout<BuildKey<decltype(t), decltype(i)>>(n).total;
+      return Status{};
+    });
+  });
+  return st.ok() ? bytes : 0;
 }
 
 Status export_tree(int dtype, int id_dtype, size_t n, const void* nodes_d, void* blob_d,

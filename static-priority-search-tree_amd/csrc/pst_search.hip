@@ -1090,6 +1090,20 @@ Status make_groups(const SearchArgs& a, SearchScratch& sc, hipStream_t s) {
 
 }  // namespace
 
+size_t search_scratch_bytes(size_t n, int dtype, int cus) {
+  SearchArgs a{};
+  a.n = n;
+  const uint32_t H = tree_shape(a).levels;
+  const uint32_t bound = 2 * kWave * (H + 2) + 4 * kWave;
+  const size_t gcap = ((bound + kMoveBlock - 1) / kMoveBlock) * kMoveBlock;
+  const size_t waves = static_cast<size_t>(cus > 0 ? cus : 1) * 32;  // the default grid of group walks
+  const size_t lists = 2 * (size_t{1} << 22) * sizeof(Entry);
+  const size_t tasks = kCopyTaskCap * sizeof(CopyTask);
+  const size_t ctr = (2 * kMaxRounds + 4) * 8 + 8;
+  (void)dtype;
+  return lists + tasks + waves * gcap * sizeof(Items::type) + ctr;
+}
+
 const char* const kPinnedScratchMsg =
     "Error: a batch graph of this tree uses its search buffers, which this call would reallocate; "
     "destroy the graph first";

@@ -118,3 +118,37 @@ def test_graph_pins_the_tree_buffers(torch, P):
     assert np.array_equal(cb.cpu().numpy(), _expected_counts(pts, big))
     tr.set_level_index(False)
     tr.close()
+
+
+def test_calc_global_mem_needed_covers_measured(torch, P):
+    """calcGlobalMemNeeded (pst/static-pst-gpu.tu:646-671) against what the
+    device reports: the tree, the handle's search buffers and a blocking
+    search's result array fit in it; the level index takes its documented
+    bytes on top."""
+    import ctypes
+    import pst_data as D
+    n = (1 << 22) + 5
+    pts = D.interval_points(n, seed=61)
+    d = P.to_device(pts)
+    free0 = torch.cuda.mem_get_info()[0]
+    tr = P.StaticPSTGPU(d, n, "f32", "u32")
+    num, ptr = ctypes.c_size_t(), ctypes.c_void_p()
+    q = np.array([0.5], np.float32)
+    P._check(P.lib().pst_gpu_two_sided_left_search(tr.h, P.RET_ID, ctypes.byref(num), ctypes.byref(ptr),
+                                                   q.ctypes.data, q.ctypes.data))
+    torch.cuda.synchronize()
+    used = free0 - torch.cuda.mem_get_info()[0]
+    P._check(P.lib().pst_free(ptr))
+    calc = P.StaticPSTGPU.calcGlobalMemNeeded(n, "f32", "u32")
+    S = (1 << n.bit_length()) - 1
+    print(f"resident after a blocking search {used}, calcGlobalMemNeeded {calc}, tree {S * 24}", flush=True)
+    assert S * 24 <= used <= calc
+    free1 = torch.cuda.mem_get_info()[0]
+    tr.set_level_index(True)
+    torch.cuda.synchronize()
+    grown = free1 - torch.cuda.mem_get_info()[0]
+    want = P.StaticPSTGPU.level_index_bytes(n, "f32", "u32")
+    # two allocations, rounded to the allocator's pages, plus at most the
+    # library's static device data if its code object loads here
+    assert want <= grown <= want + (48 << 20), (grown, want)
+    tr.close()
