@@ -166,6 +166,28 @@ def iso_bench(args, P, D, torch, dev, stream):
     iso_split = sorted(iso_runs[1:], key=sum)[1]  # median of the timed passes
     iso_ms = sum(iso_split)
     ok &= int(ioffs[nact].item()) == ntri
+    # the same active metacells through marching cubes
+    P.isosurface_count(vol_keep, dims, mcd, ids_d, nact, q_mid, icounts, stream=stream, method="cubes")
+    torch.cuda.synchronize()
+    mtri = int(icounts[:nact].sum().item())
+    del tri
+    mtri_d = torch.empty(max(mtri, 1) * 9, dtype=torch.float32, device=dev)
+    mc_runs = []
+    for _ in range(4):  # the first pass is untimed warm-up
+        ie = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ie[0].record(stream)
+        P.isosurface_count(vol_keep, dims, mcd, ids_d, nact, q_mid, icounts, stream=stream, method="cubes")
+        ie[1].record(stream)
+        ioffs[1:nact + 1] = torch.cumsum(icounts[:nact], 0)
+        ie[2].record(stream)
+        P.isosurface_write(vol_keep, dims, mcd, ids_d, nact, q_mid, ioffs, mtri_d, mtri, stream=stream,
+                           method="cubes")
+        ie[3].record(stream)
+        torch.cuda.synchronize()
+        mc_runs.append([ie[k].elapsed_time(ie[k + 1]) for k in range(3)])
+    mc_split = sorted(mc_runs[1:], key=sum)[1]
+    ok &= int(ioffs[nact].item()) == mtri
+    del mtri_d
     del vol_keep
     forest.close()
     mc_ms = ev[2].elapsed_time(ev[3])
@@ -194,6 +216,12 @@ def iso_bench(args, P, D, torch, dev, stream):
                        "scan_ms": round(iso_split[1], 3), "write_ms": round(iso_split[2], 3),
                        "triangle_GBps": round(ntri * 36 / (iso_split[2] * 1e-3) / 1e9, 1),
                        "method": "marching tetrahedra, count + scan + write (median of 3 after one warm pass)"},
+        "marching_cubes": {"isovalue": float(q_mid), "active_metacells": nact, "triangles": mtri,
+                           "ms": round(sum(mc_split), 3), "count_ms": round(mc_split[0], 3),
+                           "scan_ms": round(mc_split[1], 3), "write_ms": round(mc_split[2], 3),
+                           "triangle_GBps": round(mtri * 36 / (mc_split[2] * 1e-3) / 1e9, 1),
+                           "method": "marching cubes (256-case table), count + scan + write "
+                                     "(median of 3 after one warm pass)"},
         "results_checked": bool(ok),
     }
 
@@ -570,7 +598,7 @@ def main():
     ap.add_argument("--level-index", type=int, default=1,
                     help="1: also build the level index and time the batches and plain calls through it")
     ap.add_argument("--profile-mode", default="",
-                    choices=["", "walk", "index", "build", "ips"],
+                    choices=["", "walk", "index", "build", "ips", "forest"],
                     help="build + one pass of the named path + a 1 GiB copy (for counter runs), no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -588,6 +616,59 @@ def main():
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
+
+    def calib_copy():
+        # plain copy of a known size: the scale for HBM byte counters
+        x = torch.ones(1 << 28, dtype=torch.int32, device=dev)
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+
+    if args.profile_mode == "forest":
+        # the isosurface workload's forest batch (200 isovalues) and the
+        # marching cubes of the middle isovalue, once each
+        side = args.iso_side
+        dims = (side, side, side)
+        prm = torch.from_numpy(P.gaussian_volume_params(64, 1).ravel()).to(dev)
+        vol = torch.empty(side ** 3, dtype=torch.float32, device=dev)
+        P.synth_gaussian_volume(vol, dims, prm, 64, 0.01, 1, stream=stream)
+        nmc, _ = P.calcNumMetacells(dims, (4, 4, 4))
+        mc = torch.empty(nmc * 12, dtype=torch.uint8, device=dev)
+        P.form_metacells_into(vol, dims, (4, 4, 4), mc, stream=stream)
+        torch.cuda.synchronize()
+        vmin, vmax = float(vol.min().item()), float(vol.max().item())
+        forest = P.StaticPSTGPUArr(mc.data_ptr(), nmc, 64, "f32", "u32")
+        iso = np.linspace(vmin, vmax, args.iso_queries).astype(np.float32)
+        qs = np.zeros(len(iso), P.query_dtype("f32"))
+        qs["min_dim1"] = qs["max_dim1"] = qs["min_dim2"] = iso
+        qs["kind"] = 1
+        nq = len(qs)
+        d_q = P.to_device(qs, dev)
+        counts = torch.zeros(nq, dtype=torch.int64, device=dev)
+        offsets = torch.zeros(nq + 1, dtype=torch.int64, device=dev)
+        forest.batch_count(d_q, nq, counts, stream=stream)
+        torch.cuda.synchronize()
+        total = int(counts.sum().item())
+        out = torch.empty(total + 1, dtype=torch.int32, device=dev)
+        forest.batch_search(d_q, nq, counts, offsets, out, total, stream=stream)
+        torch.cuda.synchronize()
+        k_mid = nq // 2
+        o, c = int(offsets[k_mid].item()), int(counts[k_mid].item())
+        ids_d = out[o:o + c]
+        icounts = torch.zeros(c, dtype=torch.int64, device=dev)
+        P.isosurface_count(vol, dims, (4, 4, 4), ids_d, c, iso[k_mid], icounts, stream=stream, method="cubes")
+        torch.cuda.synchronize()
+        ntri = int(icounts.sum().item())
+        ioffs = torch.zeros(c + 1, dtype=torch.int64, device=dev)
+        ioffs[1:] = torch.cumsum(icounts, 0)
+        tri = torch.empty(max(ntri, 1) * 9, dtype=torch.float32, device=dev)
+        P.isosurface_write(vol, dims, (4, 4, 4), ids_d, c, iso[k_mid], ioffs, tri, ntri, stream=stream,
+                           method="cubes")
+        torch.cuda.synchronize()
+        calib_copy()
+        log(f"[bench] profile mode (forest) done: {total} IDs, {ntri} triangles")
+        forest.close()
+        return
     n = 1 << args.log2n
     log(f"[bench] generating {n} points")
     pts = D.interval_points(n, seed=0)
@@ -615,6 +696,7 @@ def main():
     log(f"[bench] build {build_ms:.1f} ms (median of 3; first build of the process {cold_build_ms:.1f} ms)")
 
     if args.profile_mode == "build":
+        calib_copy()
         log("[bench] profile mode (build) done")
         tree.close()
         return
@@ -659,12 +741,8 @@ def main():
             out = torch.empty(total + 1, dtype=torch.int32, device=dev)
             tree.batch_search(d_q, nq, counts, offsets, out, total, stream=stream)
         torch.cuda.synchronize()
-        # plain copy of a known size: the scale for HBM byte counters
-        x = torch.ones(1 << 28, dtype=torch.int32, device=dev)
-        y = torch.empty_like(x)
-        y.copy_(x)
-        torch.cuda.synchronize()
-        log(f"[bench] profile mode ({args.profile_mode}) done: {total} IDs, copy of {x.numel() * 4} bytes")
+        calib_copy()
+        log(f"[bench] profile mode ({args.profile_mode}) done: {total} IDs, copy of {1 << 30} bytes")
         tree.close()
         return
 
@@ -714,7 +792,9 @@ def main():
         with open(tpath) as f:
             tj = json.load(f)
         traffic, traffic_src = round(tj["traffic_bytes_per_launch"]), tj["source"]
-    traffic_frac = None if traffic is None else round(traffic / (t_write * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
+    # counter bytes of one launch over one launch's average duration
+    traffic_frac = None if traffic is None else round(traffic * WRITE_LAUNCHES / (t_write * 1e-3) / 1e9
+                                                      / HBM_PEAK_GBPS, 4)
 
     # ---- the same batch with one walk per query (grouping off)
     ungrouped_qps = None
@@ -807,6 +887,10 @@ def main():
                  "break_even_batches": round(min(iw) / saved, 2) if saved > 0 else None,
                  "same_results_as_walk": bool(same),
                  "alg_bytes_per_id": COPY_BYTES_PER_ID}
+        # the plain calls before the large mixed batch (its allocations slow later hipMallocs)
+        plain_index, okp = plain_legs(P, torch, tree, qs, sel, host_counts, dev)
+        ok &= okp
+        index["plain"] = plain_index
         if mixed is not None:
             mfig, _, mo, mout = batch_leg(P, torch, tree, d_qm, nq, dev, stream, 2, counts=mc,
                                           total=mixed["walk"]["reported_ids"])
@@ -817,9 +901,6 @@ def main():
             mixed["level_index"] = mfig
             del mout, mo
             torch.cuda.empty_cache()
-        plain_index, okp = plain_legs(P, torch, tree, qs, sel, host_counts, dev)
-        ok &= okp
-        index["plain"] = plain_index
         log(f"[bench] level index: build {min(iw):.1f} ms, batch {fig['qps']:.0f} q/s, plain {plain_index}")
         tree.set_level_index(False)
 

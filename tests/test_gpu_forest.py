@@ -253,8 +253,8 @@ def test_forest_stationary_batches(torch, P, t, i, tpb, n):
         res3 = out3.cpu().numpy()[:total * rb].view(rec)
         for k in range(0, nq, 5):
             seg = res3[o[k]:o[k] + want[k] // 2]
-            if rp == P.RET_ID:
-                assert np.all(np.isin(seg, p["id"][exp[k]])) and len(np.unique(seg)) == len(seg), k
+            if rp == P.RET_ID:  # a sub-multiset of the query's IDs (random IDs repeat)
+                assert not (Counter(seg.tolist()) - Counter(p["id"][exp[k]].tolist())), k
             else:
                 full = Counter(G.canon(p[exp[k]], True, i is not None))
                 got = Counter(G.canon(seg, True, i is not None))
