@@ -4,27 +4,30 @@
 // forest is read from HBM once per pass instead of once per query walk.
 //
 // The reference searches a forest with one thread block per tree and query
-// (pst/static-pst-gpu-arr-search-functions.tu:10-196). A tree of at most
-// 2^L - 1 <= 2047 points is loaded by one wave as 2^L / 64 records per lane
-// (one coalesced 16-byte load each for f32/u32), and per query:
-//  - the tree is skipped when the query's dim2 bound is above the root's
-//    dim2 (the tree's maximum, by heap order), or its dim1 bounds miss the
-//    tree's dim1 range (a wave min/max reduction per tree);
-//  - otherwise each lane tests its records against the query predicate,
-//    which is exactly the set a PST walk of that tree reports (a walk
-//    reports every node whose dim1 meets the query's bounds and whose dim2
-//    reaches min_dim2: heap order keeps such a node's ancestors unpruned,
-//    and the median splits steer the walk to it), and a ballot per record
-//    position gives the wave's hits.
-// Pass 1 counts the hits of every (query, segment); a scan per query turns
-// them into segment offsets and the query's total; pass 2 writes each
-// query's hits at its offset + segment offset + running count, compacted by
-// ballot/mbcnt, so output positions are fixed (no atomics) and results are
-// in tree order. Predicates follow the search kinds of the C ABI: THREE
-// min_dim1 <= dim1 <= max_dim1, LEFT dim1 <= max_dim1, any other kind
+// (pst/static-pst-gpu-arr-search-functions.tu:10-196). Here a wave stages
+// one tree of at most 2^L - 1 <= 511 points at a time: its slots that hold
+// a point are read once (coalesced) and compacted, in slot order, into LDS
+// as (dim1, dim2) pairs and IDs. A lane then owns a query (up to 512 per
+// batch, 64 per pass), and per pass:
+//  - the tree is skipped when none of the pass's queries may hit it: the
+//    query's dim2 bound is above the root's dim2 (the tree's maximum, by
+//    heap order), or its dim1 bounds miss the tree's dim1 range;
+//  - otherwise every lane tests every staged record (LDS broadcast reads)
+//    against its query's predicate, which is exactly the set a PST walk of
+//    that tree reports (a walk reports every node whose dim1 meets the
+//    query's bounds and whose dim2 reaches min_dim2: heap order keeps such
+//    a node's ancestors unpruned, and the median splits steer the walk to
+//    it).
+// Pass 1 counts the hits of every (query, segment) in registers; a scan per
+// query turns them into segment offsets and the query's total; pass 2 has
+// each lane write its query's hits at its offset + segment offset + running
+// count, so output positions are fixed (no atomics) and results are in
+// tree, then slot order. Predicates follow the search kinds of the C ABI:
+// THREE min_dim1 <= dim1 <= max_dim1, LEFT dim1 <= max_dim1, any other kind
 // dim1 >= min_dim1 (a RIGHT search), each with dim2 >= min_dim2, compared
 // in T.
 #include <algorithm>
+#include <limits>
 
 #include "pst_internal.hpp"
 
@@ -55,22 +58,39 @@ __device__ inline T wave_max(T v) {
   return v;
 }
 
+// A query as three closed bounds: lo <= dim1 <= hi, dim2 >= min2. A bound
+// the kind leaves open is the type's lowest or highest value (infinities
+// for floats), which every stored dim1 meets; a NaN bound stays NaN and
+// fails every comparison, as it does in the walk.
 template <typename T>
-__device__ inline bool hit(const Query<T>& q, T d1, T d2) {
-  if (!(q.min_dim2 <= d2)) return false;
-  if (q.kind == KIND_THREE) return q.min_dim1 <= d1 && d1 <= q.max_dim1;
-  if (q.kind == KIND_LEFT) return d1 <= q.max_dim1;
-  return q.min_dim1 <= d1;
+struct Box {
+  T lo, hi, min2, pad;
+};
+
+template <typename T>
+__device__ inline Box<T> box_of(const Query<T>& q) {
+  constexpr T lowest = std::numeric_limits<T>::has_infinity ? -std::numeric_limits<T>::infinity()
+                                                              : std::numeric_limits<T>::lowest();
+  constexpr T highest = std::numeric_limits<T>::has_infinity ? std::numeric_limits<T>::infinity()
+                                                               : std::numeric_limits<T>::max();
+  Box<T> b;
+  b.lo = q.kind == KIND_LEFT ? lowest : q.min_dim1;
+  b.hi = (q.kind == KIND_THREE || q.kind == KIND_LEFT) ? q.max_dim1 : highest;
+  b.min2 = q.min_dim2;
+  b.pad = T(0);
+  return b;
 }
 
-// Whether query q can hit a tree whose dim1 values lie in [lo1, hi1] and
-// whose largest dim2 is max2 (a necessary condition; NaN bounds fail it).
 template <typename T>
-__device__ inline bool may_hit(const Query<T>& q, T lo1, T hi1, T max2) {
-  if (!(q.min_dim2 <= max2)) return false;
-  if (q.kind == KIND_THREE) return q.min_dim1 <= hi1 && lo1 <= q.max_dim1;
-  if (q.kind == KIND_LEFT) return lo1 <= q.max_dim1;
-  return q.min_dim1 <= hi1;
+__device__ inline bool hit(const Box<T>& q, T d1, T d2) {
+  return (q.min2 <= d2) & (q.lo <= d1) & (d1 <= q.hi);
+}
+
+// Whether the query can hit a tree whose dim1 values lie in [lo1, hi1] and
+// whose largest dim2 is max2 (a necessary condition).
+template <typename T>
+__device__ inline bool may_hit(const Box<T>& q, T lo1, T hi1, T max2) {
+  return (q.min2 <= max2) & (q.lo <= hi1) & (lo1 <= q.hi);
 }
 
 struct ForestArgs {
@@ -83,69 +103,143 @@ struct ForestArgs {
   uint32_t nq;
 };
 
-// R records per lane: slot lane + 64 r of the tree.
+// Which bounds a 64-query pass compares: FORM 0 all three, FORM 1 no
+// lower dim1 bound (every live query's lo is the lowest value, which every
+// non-NaN dim1 meets and a NaN dim1 fails through d1 <= hi anyway), FORM 2
+// no upper dim1 bound (likewise).
+template <int FORM, typename T>
+__device__ inline bool hit_f(const Box<T>& q, T d1, T d2) {
+  if constexpr (FORM == 1) return (q.min2 <= d2) & (d1 <= q.hi);
+  if constexpr (FORM == 2) return (q.min2 <= d2) & (q.lo <= d1);
+  return hit(q, d1, d2);
+}
+
+template <typename T>
+__device__ inline int pass_form(bool mh, const Box<T>& q) {
+  constexpr T lowest = std::numeric_limits<T>::has_infinity ? -std::numeric_limits<T>::infinity()
+                                                              : std::numeric_limits<T>::lowest();
+  constexpr T highest = std::numeric_limits<T>::has_infinity ? std::numeric_limits<T>::infinity()
+                                                               : std::numeric_limits<T>::max();
+  if (!__ballot(mh && !(q.lo == lowest))) return 1;
+  if (!__ballot(mh && !(q.hi == highest))) return 2;
+  return 0;
+}
+
+template <typename T>
+struct Pair {
+  T d1, d2;
+};
+
+// A tree staged for the wave: the records of its slots that hold a point,
+// compacted in slot order, as (dim1, dim2) pairs and IDs in LDS.
 template <typename T, typename IS, int R>
-__device__ inline void load_tree(const Node<T, IS>* __restrict__ nodes, const ForestArgs& F, uint32_t tree,
-                                 Node<T, IS> (&rec)[R], bool (&valid)[R], T& lo1, T& hi1, T& max2) {
+struct TreeStage {
+  Pair<T> pr[kWave * R];
+  typename std::conditional<has_id_v<IS>, IS, uint32_t>::type id[kWave * R];
+};
+
+// Stages tree `tree` (R slots per lane: slot lane + 64 r; the last slot of
+// the 2^shift is never used) and returns its point count, its dim1 range
+// and its largest dim2 (the root's, by heap order). Slots holding no point
+// are not read.
+template <typename T, typename IS, int R>
+__device__ inline uint32_t stage_tree(const Node<T, IS>* __restrict__ nodes, const ForestArgs& F, uint32_t tree,
+                                      TreeStage<T, IS, R>& st, T& lo1, T& hi1, T& max2) {
   const uint32_t M = tree + 1 == F.ntrees ? F.last_M : F.full_M;
   const uint64_t base = static_cast<uint64_t>(tree) << F.shift;
   const uint32_t lane = lane_id();
   const uint32_t slots = 1u << F.shift;
+  Node<T, IS> rec[R];
+  bool v[R];
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const uint32_t s = lane + r * kWave;
-    if (s < slots) rec[r] = nodes[base + s];
+    v[r] = s + 1 < slots && subtree_size32(M, s) > 0;
+    if (v[r]) rec[r] = nodes[base + s];
   }
-  T mn = rec[0].dim1, mx = rec[0].dim1;  // slot 0 always holds a point (the root)
-  mn = __shfl(mn, 0, kWave);
-  mx = mn;
+  T mn = __shfl(rec[0].dim1, 0, kWave);  // slot 0 always holds a point (the root)
+  T mx = mn;
+  max2 = __shfl(rec[0].dim2, 0, kWave);
+  __builtin_amdgcn_wave_barrier();  // the previous tree's records are read
+  uint32_t nv = 0;
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    const uint32_t s = lane + r * kWave;
-    valid[r] = s + 1 < slots && subtree_size32(M, s) > 0;
-    if (valid[r]) {
+    const uint64_t m = __ballot(v[r]);
+    if (v[r]) {
       mn = rec[r].dim1 < mn ? rec[r].dim1 : mn;
       mx = rec[r].dim1 > mx ? rec[r].dim1 : mx;
+      const uint32_t k = nv + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+      st.pr[k] = Pair<T>{rec[r].dim1, rec[r].dim2};
+      if constexpr (has_id_v<IS>) st.id[k] = rec[r].id;
     }
+    nv += static_cast<uint32_t>(__builtin_popcountll(m));
   }
   lo1 = wave_min(mn);
   hi1 = wave_max(mx);
-  max2 = __shfl(rec[0].dim2, 0, kWave);
+  __builtin_amdgcn_wave_barrier();
+  return nv;
 }
 
-// Pass 1: segcnt[q * nseg + seg] = hits of query q in segment seg.
+// Queries per lane pass: lane l of pass p holds query 64 p + l.
+constexpr int kQPass = static_cast<int>(kForestDenseMax / kWave);
+
+template <int FORM, typename T>
+__device__ inline uint32_t count_run(const Box<T>& q, const Pair<T>* __restrict__ pr, uint32_t nv) {
+  uint32_t c = 0;
+#pragma unroll 8
+  for (uint32_t k = 0; k < nv; k++) {
+    const Pair<T> v = pr[k];
+    c += hit_f<FORM>(q, v.d1, v.d2) ? 1u : 0u;
+  }
+  return c;
+}
+
+// Pass 1: segcnt[q * nseg + seg] = hits of query q in segment seg. One
+// wave per segment, a lane per query: per tree, a 64-query pass runs only
+// when one of its queries may hit the tree; it then tests every staged
+// record (an LDS broadcast) against each lane's own query.
 template <typename T, typename IS, int R>
 __global__ __launch_bounds__(kFBlock) void k_forest_count(const Node<T, IS>* __restrict__ nodes, ForestArgs F,
                                                           const Query<T>* __restrict__ queries,
                                                           uint32_t* __restrict__ segcnt) {
-  extern __shared__ unsigned char smem[];
-  Query<T>* sq = reinterpret_cast<Query<T>*>(smem);
-  uint32_t* cnt_all = reinterpret_cast<uint32_t*>(sq + F.nq);
-  for (uint32_t i = threadIdx.x; i < F.nq; i += kFBlock) sq[i] = queries[i];
+  __shared__ TreeStage<T, IS, R> stage_all[kFWaves];
   const uint32_t wave = threadIdx.x / kWave, lane = lane_id();
-  uint32_t* cnt = cnt_all + wave * F.nq;
-  for (uint32_t i = lane; i < F.nq; i += kWave) cnt[i] = 0;
-  __syncthreads();
   const uint32_t seg = blockIdx.x * kFWaves + wave;
   if (seg >= F.nseg) return;
+  TreeStage<T, IS, R>& st = stage_all[wave];
+  Box<T> qb[kQPass];
+  bool live[kQPass];
+  uint32_t acc[kQPass];
+#pragma unroll
+  for (int p = 0; p < kQPass; p++) {
+    const uint32_t q = p * kWave + lane;
+    live[p] = q < F.nq;
+    qb[p] = box_of(queries[live[p] ? q : 0]);
+    acc[p] = 0;
+  }
+  const uint32_t npass = (F.nq + kWave - 1) / kWave;
   const uint32_t t0 = seg * F.seg_trees;
   const uint32_t t1 = std::min(F.ntrees, t0 + F.seg_trees);
   for (uint32_t tree = t0; tree < t1; tree++) {
-    Node<T, IS> rec[R];
-    bool valid[R];
     T lo1, hi1, max2;
-    load_tree<T, IS, R>(nodes, F, tree, rec, valid, lo1, hi1, max2);
-    for (uint32_t q = 0; q < F.nq; q++) {
-      const Query<T> Q = sq[q];
-      if (!may_hit(Q, lo1, hi1, max2)) continue;
-      uint32_t c = 0;
+    const uint32_t nv = stage_tree<T, IS, R>(nodes, F, tree, st, lo1, hi1, max2);
 #pragma unroll
-      for (int r = 0; r < R; r++) c += static_cast<uint32_t>(__builtin_popcountll(__ballot(valid[r] && hit(Q, rec[r].dim1, rec[r].dim2))));
-      if (lane == 0) cnt[q] += c;
+    for (int p = 0; p < kQPass; p++) {
+      if (static_cast<uint32_t>(p) >= npass) break;
+      const bool mh = live[p] && may_hit(qb[p], lo1, hi1, max2);
+      if (!__ballot(mh)) continue;
+      const int form = pass_form(mh, qb[p]);
+      uint32_t c;
+      if (form == 1) c = count_run<1>(qb[p], st.pr, nv);
+      else if (form == 2) c = count_run<2>(qb[p], st.pr, nv);
+      else c = count_run<0>(qb[p], st.pr, nv);
+      acc[p] += mh ? c : 0u;
     }
   }
-  __builtin_amdgcn_wave_barrier();
-  for (uint32_t q = lane; q < F.nq; q += kWave) segcnt[static_cast<uint64_t>(q) * F.nseg + seg] = cnt[q];
+#pragma unroll
+  for (int p = 0; p < kQPass; p++)
+    if (live[p]) segcnt[static_cast<uint64_t>(p * kWave + lane) * F.nseg + seg] = acc[p];
 }
 
 // Per query (one workgroup each): exclusive scan of its segment counts in
@@ -180,8 +274,36 @@ __global__ __launch_bounds__(kScanBlock) void k_forest_seg_scan(uint32_t* __rest
   if (tid == 0) totals[q] = carry_s;
 }
 
+// Writes the lane's hits among the staged records from position w on,
+// those at positions >= lim dropped; returns the new position.
+template <int FORM, bool RP, typename T, typename IS, int R, typename Rec>
+__device__ inline uint32_t write_run(const Box<T>& q, const TreeStage<T, IS, R>& st, uint32_t nv, Rec* __restrict__ o,
+                                     uint32_t w, uint32_t lim) {
+#pragma unroll 4
+  for (uint32_t k = 0; k < nv; k++) {
+    const Pair<T> v = st.pr[k];
+    if (hit_f<FORM>(q, v.d1, v.d2)) {
+      if (w < lim) {
+        if constexpr (RP) {
+          Point<T, IS> pt;
+          pt.dim1_val = v.d1;
+          pt.dim2_val = v.d2;
+          if constexpr (has_id_v<IS>) pt.id = st.id[k];
+          o[w] = pt;
+        } else {
+          o[w] = st.id[k];
+        }
+      }
+      w++;
+    }
+  }
+  return w;
+}
+
 // Pass 2: query q's hits of segment seg at out + offsets[q] + segoff[q][seg]
-// + running count, those at query positions >= the query's limit dropped.
+// + running count, in (tree, slot) order; positions at or past the query's
+// limit (its capacity, and the output's end) are dropped. A lane per query
+// as in pass 1: each lane writes its own query's run.
 template <typename T, typename IS, bool RP, int R>
 __global__ __launch_bounds__(kFBlock) void k_forest_write(const Node<T, IS>* __restrict__ nodes, ForestArgs F,
                                                           const Query<T>* __restrict__ queries,
@@ -191,67 +313,46 @@ __global__ __launch_bounds__(kFBlock) void k_forest_write(const Node<T, IS>* __r
                                                           unsigned long long cap_all, unsigned long long out_total,
                                                           void* __restrict__ out) {
   using Rec = typename std::conditional<RP, Point<T, IS>, IS>::type;
-  extern __shared__ unsigned char smem[];
-  Query<T>* sq = reinterpret_cast<Query<T>*>(smem);
-  unsigned long long* qoff = reinterpret_cast<unsigned long long*>(sq + F.nq);  // the query's output offset
-  uint32_t* pos_all = reinterpret_cast<uint32_t*>(qoff + F.nq);  // running position within the query
-  uint32_t* lim_all = pos_all + kFWaves * F.nq;                  // positions below this are written
-  for (uint32_t i = threadIdx.x; i < F.nq; i += kFBlock) {
-    sq[i] = queries[i];
-    qoff[i] = offsets ? offsets[i] : 0ull;
-  }
+  __shared__ TreeStage<T, IS, R> stage_all[kFWaves];
   const uint32_t wave = threadIdx.x / kWave, lane = lane_id();
   const uint32_t seg = blockIdx.x * kFWaves + wave;
-  uint32_t* pos = pos_all + wave * F.nq;
-  uint32_t* lim = lim_all + wave * F.nq;
-  Rec* o = static_cast<Rec*>(out);
-  if (seg < F.nseg) {
-    for (uint32_t q = lane; q < F.nq; q += kWave) {
-      // positions of this query that fit its capacity and the output
-      const unsigned long long off = offsets ? offsets[q] : 0ull;
-      const unsigned long long cap = caps ? caps[q] : cap_all;
-      const unsigned long long room = off < out_total ? out_total - off : 0ull;
-      const unsigned long long limit = cap < room ? cap : room;
-      pos[q] = segoff[static_cast<uint64_t>(q) * F.nseg + seg];
-      lim[q] = static_cast<uint32_t>(limit < 0xffffffffull ? limit : 0xffffffffull);
-    }
-  }
-  __syncthreads();
   if (seg >= F.nseg) return;
+  TreeStage<T, IS, R>& st = stage_all[wave];
+  Box<T> qb[kQPass];
+  bool live[kQPass];
+  Rec* qo[kQPass];                   // the query's first output record
+  uint32_t at[kQPass], lim[kQPass];  // position within the query; positions below lim are written
+#pragma unroll
+  for (int p = 0; p < kQPass; p++) {
+    const uint32_t q = p * kWave + lane;
+    live[p] = q < F.nq;
+    const uint32_t qq = live[p] ? q : 0;
+    qb[p] = box_of(queries[qq]);
+    const unsigned long long off = offsets ? offsets[qq] : 0ull;
+    const unsigned long long cap = caps ? caps[qq] : cap_all;
+    const unsigned long long room = off < out_total ? out_total - off : 0ull;
+    const unsigned long long limit = cap < room ? cap : room;
+    qo[p] = static_cast<Rec*>(out) + off;
+    at[p] = segoff[static_cast<uint64_t>(qq) * F.nseg + seg];
+    lim[p] = live[p] ? static_cast<uint32_t>(limit < 0xffffffffull ? limit : 0xffffffffull) : 0u;
+  }
+  const uint32_t npass = (F.nq + kWave - 1) / kWave;
   const uint32_t t0 = seg * F.seg_trees;
   const uint32_t t1 = std::min(F.ntrees, t0 + F.seg_trees);
   for (uint32_t tree = t0; tree < t1; tree++) {
-    Node<T, IS> rec[R];
-    bool valid[R];
     T lo1, hi1, max2;
-    load_tree<T, IS, R>(nodes, F, tree, rec, valid, lo1, hi1, max2);
-    for (uint32_t q = 0; q < F.nq; q++) {
-      const Query<T> Q = sq[q];
-      if (!may_hit(Q, lo1, hi1, max2)) continue;
-      uint32_t at = pos[q];
-      const uint32_t limit = lim[q];
-      const unsigned long long base = qoff[q];
+    const uint32_t nv = stage_tree<T, IS, R>(nodes, F, tree, st, lo1, hi1, max2);
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        const bool h = valid[r] && hit(Q, rec[r].dim1, rec[r].dim2);
-        const uint64_t b = __ballot(h);
-        if (h) {
-          const uint32_t p = at + lanes_below(b);
-          if (p < limit) {
-            if constexpr (RP) {
-              Point<T, IS> pt;
-              pt.dim1_val = rec[r].dim1;
-              pt.dim2_val = rec[r].dim2;
-              if constexpr (has_id_v<IS>) pt.id = rec[r].id;
-              o[base + p] = pt;
-            } else {
-              o[base + p] = rec[r].id;
-            }
-          }
-        }
-        at += static_cast<uint32_t>(__builtin_popcountll(b));
+    for (int p = 0; p < kQPass; p++) {
+      if (static_cast<uint32_t>(p) >= npass) break;
+      const bool mh = live[p] && may_hit(qb[p], lo1, hi1, max2);
+      if (!__ballot(mh)) continue;
+      const int form = pass_form(mh, qb[p]);
+      if (mh) {
+        if (form == 1) at[p] = write_run<1, RP>(qb[p], st, nv, qo[p], at[p], lim[p]);
+        else if (form == 2) at[p] = write_run<2, RP>(qb[p], st, nv, qo[p], at[p], lim[p]);
+        else at[p] = write_run<0, RP>(qb[p], st, nv, qo[p], at[p], lim[p]);
       }
-      if (lane == 0) pos[q] = at;
     }
   }
 }
@@ -284,8 +385,9 @@ ForestArgs forest_args(const SearchArgs& a, int cus) {
   F.ntrees = static_cast<uint32_t>((a.n + St - 1) / St);
   F.full_M = static_cast<uint32_t>(St + 1);
   F.last_M = static_cast<uint32_t>(a.n - (static_cast<uint64_t>(F.ntrees) - 1) * St + 1);
-  // about 16 waves per CU, each with a run of consecutive trees
-  const uint32_t want = static_cast<uint32_t>(std::max(1, cus)) * 16u;
+  // about 64 segments per CU (8 waves per SIMD resident, launched in
+  // several turns), each a run of consecutive trees
+  const uint32_t want = static_cast<uint32_t>(std::max(1, cus)) * 64u;
   F.seg_trees = std::max(1u, (F.ntrees + want - 1) / want);
   F.nseg = (F.ntrees + F.seg_trees - 1) / F.seg_trees;
   F.nq = static_cast<uint32_t>(a.nq);
@@ -320,22 +422,20 @@ Status forest_dense_search(const SearchArgs& a, void* segbuf, hipStream_t s, int
       const unsigned grid = (F.nseg + kFWaves - 1) / kFWaves;
       auto go = [&](auto rtag) -> Status {
         constexpr int R = decltype(rtag)::value;
-        const size_t lds_count = static_cast<size_t>(F.nq) * (sizeof(Query<T>) + kFWaves * 4);
-        const size_t lds_write = static_cast<size_t>(F.nq) * (sizeof(Query<T>) + 8 + kFWaves * 8);
         const bool count = a.mode == 0 || plan != 2;
         if (count) {
           // plan 2 writes from the segment offsets a plan 1 count pass kept
-          hipLaunchKernelGGL((k_forest_count<T, IS, R>), dim3(grid), dim3(kFBlock), lds_count, s, nodes, F, qs, segcnt);
+          hipLaunchKernelGGL((k_forest_count<T, IS, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, F, qs, segcnt);
           // write mode without kept offsets: the per-query totals go to the cursor
           hipLaunchKernelGGL(k_forest_seg_scan, dim3(F.nq), dim3(kScanBlock), 0, s, segcnt, F.nseg, a.counts);
         }
         if (a.mode == 0) return hip_status(hipGetLastError(), "Error in launching forest search");
         if (a.ret_point) {
-          hipLaunchKernelGGL((k_forest_write<T, IS, true, R>), dim3(grid), dim3(kFBlock), lds_write, s, nodes, F, qs,
+          hipLaunchKernelGGL((k_forest_write<T, IS, true, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, F, qs,
                              segcnt, a.offsets, a.caps, a.cap_all, a.out_total, a.out);
         } else {
           if constexpr (has_id_v<IS>) {
-            hipLaunchKernelGGL((k_forest_write<T, IS, false, R>), dim3(grid), dim3(kFBlock), lds_write, s, nodes,
+            hipLaunchKernelGGL((k_forest_write<T, IS, false, R>), dim3(grid), dim3(kFBlock), 0, s, nodes,
                                F, qs, segcnt, a.offsets, a.caps, a.cap_all, a.out_total, a.out);
           } else {
             return Status{1, "Error: IDs requested from a tree built without IDs"};

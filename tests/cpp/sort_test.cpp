@@ -71,7 +71,7 @@ int run_case(size_t n, int dist, uint64_t seed) {
   if (gin != keys) bad = 2;
   std::printf("sort u%zu n=%zu dist=%d: %s\n", sizeof(K) * 8, n, dist,
               bad == 0 ? "ok" : (bad == 1 ? "MISMATCH" : "INPUT CHANGED"));
-  hipFree(kin), hipFree(kout), hipFree(vin), hipFree(vout), hipFree(tmp);
+  (void)hipFree(kin), (void)hipFree(kout), (void)hipFree(vin), (void)hipFree(vout), (void)hipFree(tmp);
   return bad;
 }
 

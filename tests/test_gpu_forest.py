@@ -242,7 +242,7 @@ def test_forest_stationary_batches(torch, P, t, i, tpb, n):
         f.batch_search(dq, nq, counts, offs, out2, cap, ret=rp)
         torch.cuda.synchronize()
         raw2 = out2.cpu().numpy()
-        if nq <= 512:
+        if nq <= 512 and f.slots_per_tree <= 511:  # stationary passes: fixed output positions
             assert np.array_equal(raw2[:cap * rb], raw[:cap * rb])
         assert np.all(raw2[cap * rb:] == 0x5A)
         # separate count and write calls, each query capped at half its results
