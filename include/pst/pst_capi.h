@@ -332,6 +332,22 @@ int pst_gpu_batch_write(pst_tree* tree, const void* queries_d, size_t num_querie
 int pst_gpu_batch_search(pst_tree* tree, const void* queries_d, size_t num_queries, int ret,
                          unsigned long long* counts_d, unsigned long long* offsets_d, void* out_d,
                          size_t out_capacity, pst_stream_t stream);
+/* pst_gpu_batch_search through the level index with no device allocation
+ * at all: no counterpart in the reference; it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). Same arguments and results as
+ * pst_gpu_batch_search with ret = PST_RET_ID (IDs only; out_capacity = 0 is
+ * the sizing call), for a deep tree whose level index is built
+ * (pst_gpu_set_level_index; PST_ERR_INVALID otherwise, and for forests or
+ * trees without IDs). Its search buffers are one file-scope device array
+ * per GPU, 16.25 MiB, held from the library's load until the process ends,
+ * which limits a call to at most 8192 queries (PST_ERR_INVALID beyond).
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (other streams, other threads) race and must be serialised by the caller.
+ * Asynchronous on `stream`; nothing is cleared or synchronised. */
+int pst_gpu_batch_search_noalloc(pst_tree* tree, const void* queries_d, size_t num_queries,
+                                 unsigned long long* counts_d, unsigned long long* offsets_d, void* out_d,
+                                 size_t out_capacity, pst_stream_t stream);
 /* Tuning knobs of the search rounds (0 keeps the default). */
 int pst_gpu_set_search_tuning(pst_tree* tree, int rounds, int budget_small, int budget_large,
                               int waves_per_cu);

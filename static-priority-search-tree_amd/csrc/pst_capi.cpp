@@ -217,6 +217,60 @@ Status forest_seg_reserve(pst_tree* t, size_t nq) {
   return st;
 }
 
+// ---- the allocation-free batched search (pst_gpu_batch_search_noalloc) ----
+// Its search buffers live in one file-scope device array per GPU, sized for
+// kNoAllocMaxQueries queries through the level index: 128 ranges per query
+// (8-byte source + 4-byte output position), the range count, IDs written,
+// their prefix and the write cursor per query, and the copy chunk table.
+constexpr size_t kNoAllocMaxQueries = 8192;
+constexpr size_t kNoAllocRanges = 128;      // ranges per query of a level search
+constexpr size_t kNoAllocChunks = 1u << 20;  // chunk table entries of a level search
+constexpr size_t align256(size_t b) { return (b + 255) & ~size_t{255}; }
+constexpr size_t kNoAllocTsrc = 0;
+constexpr size_t kNoAllocTpre = kNoAllocTsrc + align256(kNoAllocMaxQueries * kNoAllocRanges * 8);
+constexpr size_t kNoAllocNtask = kNoAllocTpre + align256(kNoAllocMaxQueries * kNoAllocRanges * 4);
+constexpr size_t kNoAllocWq = kNoAllocNtask + align256(kNoAllocMaxQueries * 4);
+constexpr size_t kNoAllocWoff = kNoAllocWq + align256(kNoAllocMaxQueries * 8);
+constexpr size_t kNoAllocCursor = kNoAllocWoff + align256((kNoAllocMaxQueries + 1) * 8);
+constexpr size_t kNoAllocChunk = kNoAllocCursor + align256(kNoAllocMaxQueries * 8);
+constexpr size_t kNoAllocBytes = kNoAllocChunk + align256(kNoAllocChunks * 4);  // 16.25 MiB
+
+// In host code this name is only a stand-in for the symbol: its device
+// address comes from hipGetSymbolAddress (noalloc_scratch below).
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch;  // device ordinal -> that device's copy of `scratch`
+
+// The device address of `scratch` on `device` (looked up once per device,
+// with that device current); nullptr and a message in g_err on failure.
+unsigned char* noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch.find(device);
+  if (it != g_scratch.end()) return it->second;
+  int prev = 0;
+  hipGetDevice(&prev);
+  if (prev != device && hipSetDevice(device) != hipSuccess) {
+    g_err = "Error in selecting device " + std::to_string(device) + " for the search scratch";
+    return nullptr;
+  }
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (prev != device) hipSetDevice(prev);
+  if (e != hipSuccess || !ptr) {
+    g_err = std::string("Error in locating the search scratch: ") + hipGetErrorString(e);
+    return nullptr;
+  }
+  unsigned char* p = static_cast<unsigned char*>(ptr);
+  g_scratch[device] = p;
+  return p;
+}
+
+__global__ void k_zero_u64(unsigned long long* p, size_t n) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 }  // namespace
 
 namespace pst {
@@ -903,7 +957,66 @@ int pst_gpu_set_level_index(pst_tree* t, int on) {
   }
   t->lv_key = k;
   t->lv_id = i;
+  // set-up of the allocation-free batched search on this device
+  if (!noalloc_scratch(t->device)) return PST_ERR_HIP;
   return PST_OK;
+}
+
+int pst_gpu_batch_search_noalloc(pst_tree* t, const void* queries_d, size_t nq, unsigned long long* counts_d,
+                                 unsigned long long* offsets_d, void* out_d, size_t out_capacity,
+                                 pst_stream_t stream) {
+  if (!t || !counts_d || !offsets_d || (nq && !queries_d)) return fail(PST_ERR_INVALID, "Error: null argument");
+  if (nq == 0) return PST_OK;
+  if (nq > kNoAllocMaxQueries)
+    return fail(PST_ERR_INVALID, "Error: the allocation-free batched search holds at most " +
+                                     std::to_string(kNoAllocMaxQueries) + " queries per call");
+  if (t->slots_per_tree || t->id_dtype == PST_NO_ID)
+    return fail(PST_ERR_INVALID, "Error: the allocation-free batched search needs one deep tree with IDs");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (t->n == 0) {  // nothing to report: zero counts and offsets, with a kernel (capturable)
+    hipLaunchKernelGGL(k_zero_u64, dim3(static_cast<unsigned>((nq + 256) / 256)), dim3(256), 0, s, offsets_d,
+                       nq + 1);
+    hipLaunchKernelGGL(k_zero_u64, dim3(static_cast<unsigned>((nq + 255) / 256)), dim3(256), 0, s, counts_d, nq);
+    Status st = pst::hip_status(hipGetLastError(), "Error in launching the count reset");
+    return st.ok() ? PST_OK : fail(st);
+  }
+  if (!t->lv_key)
+    return fail(PST_ERR_INVALID, "Error: the allocation-free batched search runs through the level index; "
+                                 "build it first (pst_gpu_set_level_index)");
+  unsigned char* base = noalloc_scratch(t->device);
+  if (!base) return PST_ERR_HIP;
+  pst::LevelScratch ls{};
+  ls.tsrc = reinterpret_cast<uint64_t*>(base + kNoAllocTsrc);
+  ls.tpre = reinterpret_cast<uint32_t*>(base + kNoAllocTpre);
+  ls.ntask = reinterpret_cast<uint32_t*>(base + kNoAllocNtask);
+  ls.wq = reinterpret_cast<unsigned long long*>(base + kNoAllocWq);
+  ls.woff = reinterpret_cast<unsigned long long*>(base + kNoAllocWoff);
+  ls.chunk_task = reinterpret_cast<uint32_t*>(base + kNoAllocChunk);
+  ls.cap = kNoAllocMaxQueries;
+  ls.external = 1;
+  unsigned long long* cursor = reinterpret_cast<unsigned long long*>(base + kNoAllocCursor);
+  // count pass keeping each query's ranges, offsets, then the write step
+  // clips the kept ranges: every scratch entry a step reads was written by
+  // an earlier step of this call, so nothing needs clearing
+  pst::SearchArgs a = base_args(t);
+  a.queries = queries_d;
+  a.nq = nq;
+  a.mode = 0;
+  a.counts = counts_d;
+  Status st = pst::level_search(a, t->lv_key, t->lv_id, ls, s, 1);
+  if (st.ok()) st = pst::exclusive_scan(counts_d, offsets_d, nq, s);
+  if (!st.ok()) return fail(st);
+  if (out_capacity == 0) return PST_OK;
+  a.mode = 1;
+  a.ret_point = 0;
+  a.counts = cursor;
+  a.offsets = offsets_d;
+  a.caps = counts_d;
+  a.cap_all = ~0ull;
+  a.out = out_d;
+  a.out_total = out_capacity;
+  st = pst::level_search(a, t->lv_key, t->lv_id, ls, s, 2);
+  return st.ok() ? PST_OK : fail(st);
 }
 
 int pst_gpu_set_query_grouping(pst_tree* t, int queries_per_walk) {

@@ -44,7 +44,7 @@ EXPORTED = [
     "pst_interval_parallel_search",
     "pst_interval_parallel_search_into",
     "pst_gpu_save", "pst_gpu_load", "pst_read_vertices", "pst_read_vertices_to_device",
-    "pst_gpu_set_level_index", "pst_gpu_level_index_bytes",
+    "pst_gpu_set_level_index", "pst_gpu_level_index_bytes", "pst_gpu_batch_search_noalloc",
     "pst_isosurface_count", "pst_isosurface_write", "pst_isosurface",
     "pst_marching_cubes_count", "pst_marching_cubes_write", "pst_marching_cubes",
 ]
