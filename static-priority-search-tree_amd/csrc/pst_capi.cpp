@@ -23,6 +23,7 @@ struct pst_tree {
   void* submin = nullptr;  // subtree minimum of dim2 per record slot
   void* ids = nullptr;     // ID per record slot (whole-subtree copies read these)
   size_t slots_per_tree = 0;  // forest (StaticPSTGPUArr): slots of a full tree; 0 for one deep tree
+  void* tree_bounds = nullptr;  // forest: each tree's dim1 range (stationary batch passes)
   pst::SearchScratch sc;
   int rounds = 4, budget_small = 0, budget_large = 0, waves_per_cu = 0;
   int group = pst::kGroup;  // queries per walk in batched searches
@@ -121,6 +122,7 @@ pst::SearchArgs base_args(pst_tree* t) {
   a.submin = t->submin;
   a.ids = t->ids;
   a.slots_per_tree = t->slots_per_tree;
+  a.tree_bounds = t->tree_bounds;
   a.rounds = t->rounds;
   a.budget_small = t->budget_small;
   a.budget_large = t->budget_large;
@@ -381,6 +383,10 @@ int pst_gpu_destroy(pst_tree* t) {
     hipError_t e2 = hipFree(t->ids);
     if (e == hipSuccess) e = e2;
   }
+  if (t->tree_bounds) {
+    hipError_t e2 = hipFree(t->tree_bounds);
+    if (e == hipSuccess) e = e2;
+  }
   pst::scratch_release(t->sc);
   if (t->lv_key) hipFree(t->lv_key);
   if (t->lv_id) hipFree(t->lv_id);
@@ -421,6 +427,7 @@ int pst_gpu_arr_create(const void* pt_arr_d, size_t n, int dtype, int id_dtype, 
   hipError_t e = hipMalloc(&t->nodes, t->S * pst::node_size(dtype, id_dtype));
   if (e == hipSuccess) e = hipMalloc(&t->submin, t->S * pst::dtype_size(dtype));
   if (e == hipSuccess && id_size(id_dtype) > 0) e = hipMalloc(&t->ids, t->S * id_size(id_dtype));
+  if (e == hipSuccess) e = hipMalloc(&t->tree_bounds, pst::forest_bounds_bytes(dtype, n, t->slots_per_tree));
   if (e != hipSuccess) {
     std::string m = "Error in allocating array of PSTs on device " + std::to_string(dev_ind + 1) +
                     " (1-indexed) of " + std::to_string(num_devs) + ": " + hipGetErrorString(e);
@@ -430,6 +437,8 @@ int pst_gpu_arr_create(const void* pt_arr_d, size_t n, int dtype, int id_dtype, 
   Status st = pst::build_forest(dtype, id_dtype, pt_arr_d, n, t->slots_per_tree, t->nodes, t->stream);
   if (st.ok())
     st = pst::build_submin(dtype, id_dtype, n, t->slots_per_tree, t->nodes, t->submin, t->ids, t->stream);
+  if (st.ok())
+    st = pst::build_forest_bounds(dtype, id_dtype, n, t->slots_per_tree, t->nodes, t->tree_bounds, t->stream);
   if (st.ok()) st = pst::hip_status(hipStreamSynchronize(t->stream), "Error in synchronising after construction");
   if (!st.ok()) {
     pst_gpu_destroy(t);
@@ -472,6 +481,8 @@ int pst_gpu_load(const char* path, int dev_ind, int num_devs, pst_stream_t strea
   hipError_t e = hipMalloc(&t->nodes, t->S * pst::node_size(t->dtype, t->id_dtype));
   if (e == hipSuccess) e = hipMalloc(&t->submin, t->S * pst::dtype_size(t->dtype));
   if (e == hipSuccess && id_size(t->id_dtype) > 0) e = hipMalloc(&t->ids, t->S * id_size(t->id_dtype));
+  if (e == hipSuccess && t->slots_per_tree)
+    e = hipMalloc(&t->tree_bounds, pst::forest_bounds_bytes(t->dtype, t->n, t->slots_per_tree));
   if (e != hipSuccess) {
     std::string m = "Error in allocating priority search tree storage array" + dev_suffix(dev_ind, num_devs) +
                     hipGetErrorString(e);
@@ -481,6 +492,9 @@ int pst_gpu_load(const char* path, int dev_ind, int num_devs, pst_stream_t strea
   st = pst::load_tree_file(path, info, t->nodes, t->stream);
   if (st.ok())
     st = pst::build_submin(t->dtype, t->id_dtype, t->n, t->slots_per_tree, t->nodes, t->submin, t->ids, t->stream);
+  if (st.ok() && t->slots_per_tree)
+    st = pst::build_forest_bounds(t->dtype, t->id_dtype, t->n, t->slots_per_tree, t->nodes, t->tree_bounds,
+                                  t->stream);
   if (st.ok()) st = pst::hip_status(hipStreamSynchronize(t->stream), "Error in synchronising after loading");
   if (!st.ok()) {
     pst_gpu_destroy(t);
@@ -563,7 +577,8 @@ size_t pst_gpu_arr_calc_global_mem_needed(size_t n, unsigned threads_per_block, 
   const size_t St = forest_slots_per_tree(threads_per_block);
   const size_t ntrees = n ? (n + St - 1) / St : 0;
   const size_t tree =
-      ntrees * (St + 1) * (pst::node_size(dtype, id_dtype) + pst::dtype_size(dtype) + id_size(id_dtype));
+      ntrees * (St + 1) * (pst::node_size(dtype, id_dtype) + pst::dtype_size(dtype) + id_size(id_dtype)) +
+      pst::forest_bounds_bytes(dtype, n, St);
   // construction works in LDS; a search needs a result array of n points,
   // two 4M-entry work lists and a 4M-entry copy task list
   const size_t search = n * pst::point_size(dtype, id_dtype) + (3ull << 22) * 16;

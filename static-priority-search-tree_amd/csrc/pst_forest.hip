@@ -1,31 +1,34 @@
 // Batched searches of a forest of shallow trees (StaticPSTGPUArr) with the
-// trees held stationary: each wavefront owns a segment of consecutive trees
-// and evaluates every query of the batch against one tree at a time, so the
-// forest is read from HBM once per pass instead of once per query walk.
+// trees held stationary: each wavefront owns a segment of trees and
+// evaluates the batch's queries against one tree at a time, so the forest
+// is read from HBM once per pass instead of once per query walk.
 //
 // The reference searches a forest with one thread block per tree and query
-// (pst/static-pst-gpu-arr-search-functions.tu:10-196). Here a wave stages
-// one tree of at most 2^L - 1 <= 511 points at a time: its slots that hold
-// a point are read once (coalesced) and compacted, in slot order, into LDS
-// as (dim1, dim2) pairs and IDs. A lane then owns a query (up to 512 per
-// batch, 64 per pass), and per pass:
-//  - the tree is skipped when none of the pass's queries may hit it: the
-//    query's dim2 bound is above the root's dim2 (the tree's maximum, by
-//    heap order), or its dim1 bounds miss the tree's dim1 range;
-//  - otherwise every lane tests every staged record (LDS broadcast reads)
-//    against its query's predicate, which is exactly the set a PST walk of
-//    that tree reports (a walk reports every node whose dim1 meets the
-//    query's bounds and whose dim2 reaches min_dim2: heap order keeps such
-//    a node's ancestors unpruned, and the median splits steer the walk to
-//    it).
-// Pass 1 counts the hits of every (query, segment) in registers; a scan per
-// query turns them into segment offsets and the query's total; pass 2 has
-// each lane write its query's hits at its offset + segment offset + running
-// count, so output positions are fixed (no atomics) and results are in
-// tree, then slot order. Predicates follow the search kinds of the C ABI:
-// THREE min_dim1 <= dim1 <= max_dim1, LEFT dim1 <= max_dim1, any other kind
-// dim1 >= min_dim1 (a RIGHT search), each with dim2 >= min_dim2, compared
-// in T.
+// (pst/static-pst-gpu-arr-search-functions.tu:10-196). Here, per tree of at
+// most 2^L - 1 <= 511 points:
+//  - lanes first take a query each (64 per step, up to 512 per batch) to
+//    find the candidates, the queries that may hit the tree: those whose
+//    dim2 bound does not exceed the root's dim2 (the tree's largest, by
+//    heap order) and whose dim1 bounds meet the tree's dim1 range (kept per
+//    tree from construction);
+//  - lanes then take the tree's slots (slot 64 r + l in lane l, coalesced
+//    loads), and each candidate, its bounds read from its lane's registers,
+//    is compared with all of them at once. The hits are exactly what a PST
+//    walk of that tree reports (a walk reports every node whose dim1 meets
+//    the query's bounds and whose dim2 reaches min_dim2: heap order keeps
+//    such a node's ancestors unpruned, and the median splits steer the walk
+//    to it).
+// Pass 1 adds each candidate's hit count to its (query, segment) counter; a
+// scan per query turns the counters into segment offsets and the query's
+// total; pass 2 writes each candidate's hits in the tree at its offset +
+// segment offset + running count, the hit lanes storing their records at
+// consecutive positions. Output positions are fixed (no atomics); within a
+// query and segment, results come in (tree, slot) order. Segment s holds
+// trees s, s + nseg, s + 2 nseg, ... so that segments get alike work even
+// when hits cluster in one region of the input. Predicates follow the
+// search kinds of the C ABI: THREE min_dim1 <= dim1 <= max_dim1, LEFT
+// dim1 <= max_dim1, any other kind dim1 >= min_dim1 (a RIGHT search), each
+// with dim2 >= min_dim2, compared in T.
 #include <algorithm>
 #include <limits>
 
@@ -81,11 +84,6 @@ __device__ inline Box<T> box_of(const Query<T>& q) {
   return b;
 }
 
-template <typename T>
-__device__ inline bool hit(const Box<T>& q, T d1, T d2) {
-  return (q.min2 <= d2) & (q.lo <= d1) & (d1 <= q.hi);
-}
-
 // Whether the query can hit a tree whose dim1 values lie in [lo1, hi1] and
 // whose largest dim2 is max2 (a necessary condition).
 template <typename T>
@@ -93,153 +91,172 @@ __device__ inline bool may_hit(const Box<T>& q, T lo1, T hi1, T max2) {
   return (q.min2 <= max2) & (q.lo <= hi1) & (lo1 <= q.hi);
 }
 
+// A tree's dim1 range over its points (NaN values ignored).
+template <typename T>
+struct Range1 {
+  T lo, hi;
+};
+
 struct ForestArgs {
   uint32_t ntrees;      // trees of the forest
   uint32_t shift;       // record slots per tree = 2^shift
   uint32_t full_M;      // points + 1 of a full tree
   uint32_t last_M;      // points + 1 of the last tree
-  uint32_t nseg;        // segments (one wave each)
-  uint32_t seg_trees;   // trees per segment
+  uint32_t nseg;        // segments (one wave each); segment s holds trees s + k nseg
   uint32_t nq;
 };
 
-// Which bounds a 64-query pass compares: FORM 0 all three, FORM 1 no
-// lower dim1 bound (every live query's lo is the lowest value, which every
-// non-NaN dim1 meets and a NaN dim1 fails through d1 <= hi anyway), FORM 2
-// no upper dim1 bound (likewise).
-template <int FORM, typename T>
-__device__ inline bool hit_f(const Box<T>& q, T d1, T d2) {
-  if constexpr (FORM == 1) return (q.min2 <= d2) & (d1 <= q.hi);
-  if constexpr (FORM == 2) return (q.min2 <= d2) & (q.lo <= d1);
-  return hit(q, d1, d2);
+// Lane masks of a <= b (ordered: false where either is NaN), one vector
+// compare each.
+__device__ inline uint64_t le_mask(float a, float b) { return __builtin_amdgcn_fcmpf(a, b, 5); }
+__device__ inline uint64_t le_mask(double a, double b) { return __builtin_amdgcn_fcmp(a, b, 5); }
+__device__ inline uint64_t le_mask(int32_t a, int32_t b) { return __builtin_amdgcn_sicmp(a, b, 41); }
+__device__ inline uint64_t le_mask(int64_t a, int64_t b) {
+  return __builtin_amdgcn_sicmpl(static_cast<long long>(a), static_cast<long long>(b), 41);
 }
 
+// The box of a query slot past the batch's end (such slots are never
+// candidates; the box keeps their LDS entries defined).
 template <typename T>
-__device__ inline int pass_form(bool mh, const Box<T>& q) {
-  constexpr T lowest = std::numeric_limits<T>::has_infinity ? -std::numeric_limits<T>::infinity()
-                                                              : std::numeric_limits<T>::lowest();
-  constexpr T highest = std::numeric_limits<T>::has_infinity ? std::numeric_limits<T>::infinity()
-                                                               : std::numeric_limits<T>::max();
-  if (!__ballot(mh && !(q.lo == lowest))) return 1;
-  if (!__ballot(mh && !(q.hi == highest))) return 2;
-  return 0;
+__device__ inline Box<T> dead_box() {
+  Box<T> b;
+  b.lo = std::numeric_limits<T>::max();
+  b.hi = std::numeric_limits<T>::lowest();
+  b.min2 = std::numeric_limits<T>::has_quiet_NaN ? std::numeric_limits<T>::quiet_NaN() : std::numeric_limits<T>::max();
+  b.pad = T(0);
+  return b;
 }
 
+// The lanes whose slot of the tree chunk (d1, d2 in the lane) the query box
+// hits (a bound the query's kind leaves open is the lowest or highest value,
+// which every dim1 but NaN meets).
 template <typename T>
-struct Pair {
-  T d1, d2;
-};
+__device__ inline uint64_t hit_lanes(const Box<T>& q, T d1, T d2) {
+  return le_mask(q.min2, d2) & le_mask(q.lo, d1) & le_mask(d1, q.hi);
+}
 
-// A tree staged for the wave: the records of its slots that hold a point,
-// compacted in slot order, as (dim1, dim2) pairs and IDs in LDS.
-template <typename T, typename IS, int R>
-struct TreeStage {
-  Pair<T> pr[kWave * R];
-  typename std::conditional<has_id_v<IS>, IS, uint32_t>::type id[kWave * R];
-};
+// Lane j's value, as a uniform (SGPR) value.
+template <typename T>
+__device__ inline T bcast(T v, uint32_t j) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int32_t, v), j));
+  } else {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int32_t>(u), j);
+    const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int32_t>(u >> 32), j);
+    return __builtin_bit_cast(T, (static_cast<uint64_t>(hi) << 32) | lo);
+  }
+}
 
-// Stages tree `tree` (R slots per lane: slot lane + 64 r; the last slot of
-// the 2^shift is never used) and returns its point count, its dim1 range
-// and its largest dim2 (the root's, by heap order). Slots holding no point
-// are not read.
+// Lane ql's box, as uniform values.
+template <typename T>
+__device__ inline Box<T> lane_box(const Box<T>& b, uint32_t ql) {
+  Box<T> r;
+  r.lo = bcast(b.lo, ql);
+  r.hi = bcast(b.hi, ql);
+  r.min2 = bcast(b.min2, ql);
+  r.pad = T(0);
+  return r;
+}
+
+// Queries per pass over the batch: lane l of pass p tests query 64 p + l.
+constexpr uint32_t kQSlots = static_cast<uint32_t>(kForestDenseMax);
+
+// Stages the batch's query boxes in LDS (the whole block), query
+// slots past the batch's end up to a multiple of 64 getting dead boxes.
+template <typename T>
+__device__ inline void stage_boxes(const Query<T>* __restrict__ queries, uint32_t nq, Box<T>* sbox) {
+  const uint32_t padded = (nq + kWave - 1) / kWave * kWave;
+  for (uint32_t i = threadIdx.x; i < padded; i += kFBlock) sbox[i] = i < nq ? box_of(queries[i]) : dead_box<T>();
+}
+
+// A tree staged for the wave: slot 64 r + lane of the tree in lane `lane`
+// (coalesced vector loads), and per chunk r the slots that hold a point.
 template <typename T, typename IS, int R>
-__device__ inline uint32_t stage_tree(const Node<T, IS>* __restrict__ nodes, const ForestArgs& F, uint32_t tree,
-                                      TreeStage<T, IS, R>& st, T& lo1, T& hi1, T& max2) {
-  const uint32_t M = tree + 1 == F.ntrees ? F.last_M : F.full_M;
-  const uint64_t base = static_cast<uint64_t>(tree) << F.shift;
+__device__ inline void load_tree(const Node<T, IS>* __restrict__ tn, const ForestArgs& F, uint32_t M, T (&d1)[R],
+                                 T (&d2)[R], uint64_t (&vm)[R]) {
   const uint32_t lane = lane_id();
-  const uint32_t slots = 1u << F.shift;
-  Node<T, IS> rec[R];
-  bool v[R];
+  const uint32_t end = (1u << F.shift) - 1;  // the last slot of a tree's storage is never used
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    const uint32_t s = lane + r * kWave;
-    v[r] = s + 1 < slots && subtree_size32(M, s) > 0;
-    if (v[r]) rec[r] = nodes[base + s];
-  }
-  T mn = __shfl(rec[0].dim1, 0, kWave);  // slot 0 always holds a point (the root)
-  T mx = mn;
-  max2 = __shfl(rec[0].dim2, 0, kWave);
-  __builtin_amdgcn_wave_barrier();  // the previous tree's records are read
-  uint32_t nv = 0;
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    const uint64_t m = __ballot(v[r]);
-    if (v[r]) {
-      mn = rec[r].dim1 < mn ? rec[r].dim1 : mn;
-      mx = rec[r].dim1 > mx ? rec[r].dim1 : mx;
-      const uint32_t k = nv + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
-                                                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
-      st.pr[k] = Pair<T>{rec[r].dim1, rec[r].dim2};
-      if constexpr (has_id_v<IS>) st.id[k] = rec[r].id;
+    const uint32_t s = r * kWave + lane;
+    const bool v = s < end && (M == F.full_M || subtree_size32(M, s) > 0);
+    d1[r] = T(0);
+    d2[r] = T(0);
+    if (v) {
+      const Node<T, IS> nd = tn[s];
+      d1[r] = nd.dim1;
+      d2[r] = nd.dim2;
     }
-    nv += static_cast<uint32_t>(__builtin_popcountll(m));
+    vm[r] = __ballot(v);
   }
-  lo1 = wave_min(mn);
-  hi1 = wave_max(mx);
-  __builtin_amdgcn_wave_barrier();
-  return nv;
 }
 
-// Queries per lane pass: lane l of pass p holds query 64 p + l.
-constexpr int kQPass = static_cast<int>(kForestDenseMax / kWave);
-
-template <int FORM, typename T>
-__device__ inline uint32_t count_run(const Box<T>& q, const Pair<T>* __restrict__ pr, uint32_t nv) {
-  uint32_t c = 0;
-#pragma unroll 8
-  for (uint32_t k = 0; k < nv; k++) {
-    const Pair<T> v = pr[k];
-    c += hit_f<FORM>(q, v.d1, v.d2) ? 1u : 0u;
-  }
-  return c;
-}
 
 // Pass 1: segcnt[q * nseg + seg] = hits of query q in segment seg. One
-// wave per segment, a lane per query: per tree, a 64-query pass runs only
-// when one of its queries may hit the tree; it then tests every staged
-// record (an LDS broadcast) against each lane's own query.
+// wave per segment. Per tree, lanes first take a query each (64 per pass)
+// to find the candidates: the queries that may hit the tree (its dim1 range
+// and its root's dim2, the tree's largest by heap order). Lanes then take
+// the tree's slots, and each candidate is compared with all of them at
+// once; its hit count goes to its LDS counter.
 template <typename T, typename IS, int R>
-__global__ __launch_bounds__(kFBlock) void k_forest_count(const Node<T, IS>* __restrict__ nodes, ForestArgs F,
+__global__ __launch_bounds__(kFBlock) void k_forest_count(const Node<T, IS>* __restrict__ nodes,
+                                                          const Range1<T>* __restrict__ bounds, ForestArgs F,
                                                           const Query<T>* __restrict__ queries,
                                                           uint32_t* __restrict__ segcnt) {
-  __shared__ TreeStage<T, IS, R> stage_all[kFWaves];
-  const uint32_t wave = threadIdx.x / kWave, lane = lane_id();
-  const uint32_t seg = blockIdx.x * kFWaves + wave;
+  __shared__ Box<T> sbox[kQSlots];
+  __shared__ uint32_t cnt_all[kFWaves][kQSlots];
+  stage_boxes(queries, F.nq, sbox);
+  const uint32_t lane = lane_id();
+  uint32_t* cnt = cnt_all[threadIdx.x / kWave];
+  for (uint32_t i = lane; i < kQSlots; i += kWave) cnt[i] = 0;
+  __syncthreads();
+  // the wave's segment, known to the compiler as uniform
+  const uint32_t seg = __builtin_amdgcn_readfirstlane(blockIdx.x * kFWaves + threadIdx.x / kWave);
   if (seg >= F.nseg) return;
-  TreeStage<T, IS, R>& st = stage_all[wave];
-  Box<T> qb[kQPass];
-  bool live[kQPass];
-  uint32_t acc[kQPass];
-#pragma unroll
-  for (int p = 0; p < kQPass; p++) {
-    const uint32_t q = p * kWave + lane;
-    live[p] = q < F.nq;
-    qb[p] = box_of(queries[live[p] ? q : 0]);
-    acc[p] = 0;
-  }
   const uint32_t npass = (F.nq + kWave - 1) / kWave;
-  const uint32_t t0 = seg * F.seg_trees;
-  const uint32_t t1 = std::min(F.ntrees, t0 + F.seg_trees);
-  for (uint32_t tree = t0; tree < t1; tree++) {
-    T lo1, hi1, max2;
-    const uint32_t nv = stage_tree<T, IS, R>(nodes, F, tree, st, lo1, hi1, max2);
+  // segment seg holds trees seg, seg + nseg, seg + 2 nseg, ...: trees spread
+  // over the whole forest, so that the segments' work is alike even where
+  // hits cluster in some region of the input
+  for (uint32_t tree = seg; tree < F.ntrees; tree += F.nseg) {
+    const Node<T, IS>* tn = nodes + (static_cast<uint64_t>(tree) << F.shift);
+    const Range1<T> b = bounds[tree];
+    const T max2 = tn[0].dim2;
+    const uint32_t M = tree + 1 == F.ntrees ? F.last_M : F.full_M;
+    T d1[R], d2[R];
+    uint64_t vm[R];
+    bool loaded = false;
+    for (uint32_t p = 0; p < npass; p++) {
+      const Box<T> lb = sbox[p * kWave + lane];  // the lane's query of this pass
+      uint64_t cm = __ballot(p * kWave + lane < F.nq && may_hit(lb, b.lo, b.hi, max2));
+      if (!cm) continue;
+      if (!loaded) {
+        load_tree<T, IS, R>(tn, F, M, d1, d2, vm);
+        loaded = true;
+      }
+      // two candidates per step: their compare chains are independent
+      while (cm) {
+        const uint32_t qa = static_cast<uint32_t>(__builtin_ctzll(cm));
+        cm &= cm - 1;
+        const bool two = cm != 0;
+        const uint32_t qb = two ? static_cast<uint32_t>(__builtin_ctzll(cm)) : qa;
+        cm &= cm - (two ? 1 : 0);
+        const Box<T> ba = lane_box(lb, qa), bb = lane_box(lb, qb);  // from the candidates' lanes
+        uint32_t ca = 0, cb = 0;
 #pragma unroll
-    for (int p = 0; p < kQPass; p++) {
-      if (static_cast<uint32_t>(p) >= npass) break;
-      const bool mh = live[p] && may_hit(qb[p], lo1, hi1, max2);
-      if (!__ballot(mh)) continue;
-      const int form = pass_form(mh, qb[p]);
-      uint32_t c;
-      if (form == 1) c = count_run<1>(qb[p], st.pr, nv);
-      else if (form == 2) c = count_run<2>(qb[p], st.pr, nv);
-      else c = count_run<0>(qb[p], st.pr, nv);
-      acc[p] += mh ? c : 0u;
+        for (int r = 0; r < R; r++) {
+          ca += static_cast<uint32_t>(__builtin_popcountll(hit_lanes(ba, d1[r], d2[r]) & vm[r]));
+          cb += static_cast<uint32_t>(__builtin_popcountll(hit_lanes(bb, d1[r], d2[r]) & vm[r]));
+        }
+        if (lane == 0) {
+          if (ca) atomicAdd(&cnt[p * kWave + qa], ca);  // LDS adds, no return
+          if (two && cb) atomicAdd(&cnt[p * kWave + qb], cb);
+        }
+      }
     }
   }
-#pragma unroll
-  for (int p = 0; p < kQPass; p++)
-    if (live[p]) segcnt[static_cast<uint64_t>(p * kWave + lane) * F.nseg + seg] = acc[p];
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t q = lane; q < F.nq; q += kWave) segcnt[static_cast<uint64_t>(q) * F.nseg + seg] = cnt[q];
 }
 
 // Per query (one workgroup each): exclusive scan of its segment counts in
@@ -274,38 +291,16 @@ __global__ __launch_bounds__(kScanBlock) void k_forest_seg_scan(uint32_t* __rest
   if (tid == 0) totals[q] = carry_s;
 }
 
-// Writes the lane's hits among the staged records from position w on,
-// those at positions >= lim dropped; returns the new position.
-template <int FORM, bool RP, typename T, typename IS, int R, typename Rec>
-__device__ inline uint32_t write_run(const Box<T>& q, const TreeStage<T, IS, R>& st, uint32_t nv, Rec* __restrict__ o,
-                                     uint32_t w, uint32_t lim) {
-#pragma unroll 4
-  for (uint32_t k = 0; k < nv; k++) {
-    const Pair<T> v = st.pr[k];
-    if (hit_f<FORM>(q, v.d1, v.d2)) {
-      if (w < lim) {
-        if constexpr (RP) {
-          Point<T, IS> pt;
-          pt.dim1_val = v.d1;
-          pt.dim2_val = v.d2;
-          if constexpr (has_id_v<IS>) pt.id = st.id[k];
-          o[w] = pt;
-        } else {
-          o[w] = st.id[k];
-        }
-      }
-      w++;
-    }
-  }
-  return w;
-}
-
 // Pass 2: query q's hits of segment seg at out + offsets[q] + segoff[q][seg]
 // + running count, in (tree, slot) order; positions at or past the query's
-// limit (its capacity, and the output's end) are dropped. A lane per query
-// as in pass 1: each lane writes its own query's run.
+// limit (its capacity, and the output's end) are dropped. Candidates are
+// found as in pass 1; per candidate and chunk of 64 slots, the lanes whose
+// slot it hits write their record at consecutive positions (one contiguous
+// store per candidate and chunk).
 template <typename T, typename IS, bool RP, int R>
-__global__ __launch_bounds__(kFBlock) void k_forest_write(const Node<T, IS>* __restrict__ nodes, ForestArgs F,
+__global__ __launch_bounds__(kFBlock) void k_forest_write(const Node<T, IS>* __restrict__ nodes,
+                                                          const void* __restrict__ ids_v,
+                                                          const Range1<T>* __restrict__ bounds, ForestArgs F,
                                                           const Query<T>* __restrict__ queries,
                                                           const uint32_t* __restrict__ segoff,
                                                           const unsigned long long* __restrict__ offsets,
@@ -313,48 +308,141 @@ __global__ __launch_bounds__(kFBlock) void k_forest_write(const Node<T, IS>* __r
                                                           unsigned long long cap_all, unsigned long long out_total,
                                                           void* __restrict__ out) {
   using Rec = typename std::conditional<RP, Point<T, IS>, IS>::type;
-  __shared__ TreeStage<T, IS, R> stage_all[kFWaves];
-  const uint32_t wave = threadIdx.x / kWave, lane = lane_id();
-  const uint32_t seg = blockIdx.x * kFWaves + wave;
-  if (seg >= F.nseg) return;
-  TreeStage<T, IS, R>& st = stage_all[wave];
-  Box<T> qb[kQPass];
-  bool live[kQPass];
-  Rec* qo[kQPass];                   // the query's first output record
-  uint32_t at[kQPass], lim[kQPass];  // position within the query; positions below lim are written
-#pragma unroll
-  for (int p = 0; p < kQPass; p++) {
-    const uint32_t q = p * kWave + lane;
-    live[p] = q < F.nq;
-    const uint32_t qq = live[p] ? q : 0;
-    qb[p] = box_of(queries[qq]);
-    const unsigned long long off = offsets ? offsets[qq] : 0ull;
-    const unsigned long long cap = caps ? caps[qq] : cap_all;
+  using ISt = typename std::conditional<has_id_v<IS>, IS, uint32_t>::type;
+  const ISt* __restrict__ ids = static_cast<const ISt*>(ids_v);
+  __shared__ Box<T> sbox[kQSlots];
+  __shared__ uint32_t pos_all[kFWaves][kQSlots];  // the query's next position (within the query's results)
+  __shared__ unsigned long long soff[kQSlots];     // the query's first output record
+  __shared__ uint32_t slim[kQSlots];               // positions below this are written
+  stage_boxes(queries, F.nq, sbox);
+  for (uint32_t i = threadIdx.x; i < F.nq; i += kFBlock) {
+    const unsigned long long off = offsets ? offsets[i] : 0ull;
+    const unsigned long long cap = caps ? caps[i] : cap_all;
     const unsigned long long room = off < out_total ? out_total - off : 0ull;
     const unsigned long long limit = cap < room ? cap : room;
-    qo[p] = static_cast<Rec*>(out) + off;
-    at[p] = segoff[static_cast<uint64_t>(qq) * F.nseg + seg];
-    lim[p] = live[p] ? static_cast<uint32_t>(limit < 0xffffffffull ? limit : 0xffffffffull) : 0u;
+    soff[i] = off;
+    slim[i] = static_cast<uint32_t>(limit < 0xffffffffull ? limit : 0xffffffffull);
   }
+  const uint32_t lane = lane_id();
+  uint32_t* pos = pos_all[threadIdx.x / kWave];
+  const uint32_t seg = __builtin_amdgcn_readfirstlane(blockIdx.x * kFWaves + threadIdx.x / kWave);
+  if (seg < F.nseg)
+    for (uint32_t i = lane; i < F.nq; i += kWave) pos[i] = segoff[static_cast<uint64_t>(i) * F.nseg + seg];
+  __syncthreads();
+  if (seg >= F.nseg) return;
+  Rec* const o = static_cast<Rec*>(out);
   const uint32_t npass = (F.nq + kWave - 1) / kWave;
-  const uint32_t t0 = seg * F.seg_trees;
-  const uint32_t t1 = std::min(F.ntrees, t0 + F.seg_trees);
-  for (uint32_t tree = t0; tree < t1; tree++) {
-    T lo1, hi1, max2;
-    const uint32_t nv = stage_tree<T, IS, R>(nodes, F, tree, st, lo1, hi1, max2);
+  // segment seg holds trees seg, seg + nseg, seg + 2 nseg, ...: trees spread
+  // over the whole forest, so that the segments' work is alike even where
+  // hits cluster in some region of the input
+  for (uint32_t tree = seg; tree < F.ntrees; tree += F.nseg) {
+    const uint64_t base = static_cast<uint64_t>(tree) << F.shift;
+    const Node<T, IS>* tn = nodes + base;
+    const Range1<T> b = bounds[tree];
+    const T max2 = tn[0].dim2;
+    const uint32_t M = tree + 1 == F.ntrees ? F.last_M : F.full_M;
+    T d1[R], d2[R];
+    uint64_t vm[R];
+    Rec rv[R];  // lane l's record of chunk r: slot 64 r + l
+    bool vl[R];  // and whether that slot holds a point
+    bool loaded = false;
+    for (uint32_t p = 0; p < npass; p++) {
+      const Box<T> lb = sbox[p * kWave + lane];  // the lane's query of this pass
+      uint64_t cm = __ballot(p * kWave + lane < F.nq && may_hit(lb, b.lo, b.hi, max2));
+      if (!cm) continue;
+      uint32_t lpos = pos[p * kWave + lane];  // the lane's query's next position, offset and limit
+      const unsigned long long loff = soff[p * kWave + lane];
+      const uint32_t llim = slim[p * kWave + lane];
+      bool moved = false;
+      if (!loaded) {
+        load_tree<T, IS, R>(tn, F, M, d1, d2, vm);
 #pragma unroll
-    for (int p = 0; p < kQPass; p++) {
-      if (static_cast<uint32_t>(p) >= npass) break;
-      const bool mh = live[p] && may_hit(qb[p], lo1, hi1, max2);
-      if (!__ballot(mh)) continue;
-      const int form = pass_form(mh, qb[p]);
-      if (mh) {
-        if (form == 1) at[p] = write_run<1, RP>(qb[p], st, nv, qo[p], at[p], lim[p]);
-        else if (form == 2) at[p] = write_run<2, RP>(qb[p], st, nv, qo[p], at[p], lim[p]);
-        else at[p] = write_run<0, RP>(qb[p], st, nv, qo[p], at[p], lim[p]);
+        for (int r = 0; r < R; r++) {
+          rv[r] = Rec{};
+          vl[r] = (vm[r] >> lane) & 1ull;
+          if (vl[r]) {
+            if constexpr (RP) {
+              rv[r].dim1_val = d1[r];
+              rv[r].dim2_val = d2[r];
+              if constexpr (has_id_v<IS>) rv[r].id = tn[r * kWave + lane].id;
+            } else {
+              rv[r] = ids[base + r * kWave + lane];
+            }
+          }
+        }
+        loaded = true;
       }
+      // one candidate's hits in the tree: its lanes' records, in slot order,
+      // from its next position on
+      auto emit = [&](uint32_t ql, const bool (&mine)[R], const uint64_t (&S)[R]) {
+        Rec* const oq = o + bcast(loff, ql);
+        const uint32_t lim = __builtin_amdgcn_readlane(llim, ql);
+        uint32_t w = __builtin_amdgcn_readlane(lpos, ql);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const uint64_t m = S[r];
+          if (!m) continue;
+          const uint32_t at = w + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+          if (mine[r] && at < lim) oq[at] = rv[r];
+          w += static_cast<uint32_t>(__builtin_popcountll(m));
+        }
+        lpos = lane == ql ? w : lpos;
+      };
+      // two candidates per step: their compare chains are independent
+      while (cm) {
+        const uint32_t qa = static_cast<uint32_t>(__builtin_ctzll(cm));
+        cm &= cm - 1;
+        const bool two = cm != 0;
+        const uint32_t qb = two ? static_cast<uint32_t>(__builtin_ctzll(cm)) : qa;
+        cm &= cm - (two ? 1 : 0);
+        const Box<T> ba = lane_box(lb, qa), bb = lane_box(lb, qb);
+        bool ma[R], mb[R];  // the candidate hits the lane's slot of chunk r
+        uint64_t Sa[R], Sb[R];
+        uint64_t anya = 0, anyb = 0;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          ma[r] = vl[r] & (ba.min2 <= d2[r]) & (ba.lo <= d1[r]) & (d1[r] <= ba.hi);
+          mb[r] = vl[r] & two & (bb.min2 <= d2[r]) & (bb.lo <= d1[r]) & (d1[r] <= bb.hi);
+          Sa[r] = __ballot(ma[r]);
+          Sb[r] = __ballot(mb[r]);
+          anya |= Sa[r];
+          anyb |= Sb[r];
+        }
+        if (anya) emit(qa, ma, Sa);
+        if (anyb) emit(qb, mb, Sb);
+        moved = true;
+      }
+      if (moved) pos[p * kWave + lane] = lpos;
     }
   }
+}
+
+// bounds[tree] = the tree's dim1 range (one wave per tree); NaN values
+// compare false and so never widen it.
+template <typename T, typename IS>
+__global__ __launch_bounds__(kFBlock) void k_forest_bounds(const Node<T, IS>* __restrict__ nodes, ForestArgs F,
+                                                           Range1<T>* __restrict__ bounds) {
+  const uint32_t tree = blockIdx.x * kFWaves + threadIdx.x / kWave, lane = lane_id();
+  if (tree >= F.ntrees) return;
+  constexpr T lowest = std::numeric_limits<T>::has_infinity ? -std::numeric_limits<T>::infinity()
+                                                              : std::numeric_limits<T>::lowest();
+  constexpr T highest = std::numeric_limits<T>::has_infinity ? std::numeric_limits<T>::infinity()
+                                                               : std::numeric_limits<T>::max();
+  const uint32_t M = tree + 1 == F.ntrees ? F.last_M : F.full_M;
+  const Node<T, IS>* tn = nodes + (static_cast<uint64_t>(tree) << F.shift);
+  const uint32_t end = (1u << F.shift) - 1;
+  T mn = highest, mx = lowest;
+  for (uint32_t s = lane; s < end; s += kWave) {
+    if (subtree_size32(M, s) > 0) {
+      const T d1 = tn[s].dim1;
+      mn = d1 < mn ? d1 : mn;
+      mx = d1 > mx ? d1 : mx;
+    }
+  }
+  mn = wave_min(mn);
+  mx = wave_max(mx);
+  if (lane == 0) bounds[tree] = Range1<T>{mn, mx};
 }
 
 template <typename F>
@@ -385,11 +473,10 @@ ForestArgs forest_args(const SearchArgs& a, int cus) {
   F.ntrees = static_cast<uint32_t>((a.n + St - 1) / St);
   F.full_M = static_cast<uint32_t>(St + 1);
   F.last_M = static_cast<uint32_t>(a.n - (static_cast<uint64_t>(F.ntrees) - 1) * St + 1);
-  // about 64 segments per CU (8 waves per SIMD resident, launched in
-  // several turns), each a run of consecutive trees
-  const uint32_t want = static_cast<uint32_t>(std::max(1, cus)) * 64u;
-  F.seg_trees = std::max(1u, (F.ntrees + want - 1) / want);
-  F.nseg = (F.ntrees + F.seg_trees - 1) / F.seg_trees;
+  // 112 segments per CU: about four turns of resident waves, so that the
+  // hardware's dispatch evens out what the interleaving leaves uneven
+  const uint32_t want = static_cast<uint32_t>(std::max(1, cus)) * 112u;
+  F.nseg = std::min(F.ntrees, want);
   F.nq = static_cast<uint32_t>(a.nq);
   return F;
 }
@@ -408,10 +495,12 @@ size_t forest_dense_scratch_bytes(size_t n, size_t slots_per_tree, size_t nq, in
 Status forest_dense_search(const SearchArgs& a, void* segbuf, hipStream_t s, int plan) {
   if (a.nq == 0 || a.n == 0) return Status{};
   if (a.nq > kForestDenseMax) return Status{1, "Error: too many queries for a stationary forest pass"};
+  if (!a.tree_bounds) return Status{1, "Error: the forest has no per-tree dim1 ranges"};
   int dev = 0, cus = 0;
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   const ForestArgs F = forest_args(a, cus);
+  if (F.shift > 9) return Status{1, "Error: stationary forest passes hold trees of at most 511 points"};
   uint32_t* segcnt = static_cast<uint32_t*>(segbuf);
   return fwith_t(a.dtype, [&](auto tv) -> Status {
     using T = decltype(tv);
@@ -419,38 +508,62 @@ Status forest_dense_search(const SearchArgs& a, void* segbuf, hipStream_t s, int
       using IS = decltype(iv);
       const auto* nodes = static_cast<const Node<T, IS>*>(a.nodes);
       const auto* qs = static_cast<const Query<T>*>(a.queries);
+      const auto* bounds = static_cast<const Range1<T>*>(a.tree_bounds);
       const unsigned grid = (F.nseg + kFWaves - 1) / kFWaves;
       auto go = [&](auto rtag) -> Status {
         constexpr int R = decltype(rtag)::value;
-        const bool count = a.mode == 0 || plan != 2;
-        if (count) {
+        if (a.mode == 0 || plan != 2) {
           // plan 2 writes from the segment offsets a plan 1 count pass kept
-          hipLaunchKernelGGL((k_forest_count<T, IS, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, F, qs, segcnt);
+          hipLaunchKernelGGL((k_forest_count<T, IS, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, bounds, F, qs,
+                             segcnt);
           // write mode without kept offsets: the per-query totals go to the cursor
           hipLaunchKernelGGL(k_forest_seg_scan, dim3(F.nq), dim3(kScanBlock), 0, s, segcnt, F.nseg, a.counts);
         }
         if (a.mode == 0) return hip_status(hipGetLastError(), "Error in launching forest search");
         if (a.ret_point) {
-          hipLaunchKernelGGL((k_forest_write<T, IS, true, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, F, qs,
-                             segcnt, a.offsets, a.caps, a.cap_all, a.out_total, a.out);
+          hipLaunchKernelGGL((k_forest_write<T, IS, true, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, a.ids, bounds,
+                             F, qs, segcnt, a.offsets, a.caps, a.cap_all, a.out_total, a.out);
         } else {
           if constexpr (has_id_v<IS>) {
-            hipLaunchKernelGGL((k_forest_write<T, IS, false, R>), dim3(grid), dim3(kFBlock), 0, s, nodes,
-                               F, qs, segcnt, a.offsets, a.caps, a.cap_all, a.out_total, a.out);
-          } else {
-            return Status{1, "Error: IDs requested from a tree built without IDs"};
+            hipLaunchKernelGGL((k_forest_write<T, IS, false, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, a.ids,
+                               bounds, F, qs, segcnt, a.offsets, a.caps, a.cap_all, a.out_total, a.out);
           }
         }
         return hip_status(hipGetLastError(), "Error in launching forest search");
       };
+      if (!a.ret_point && !has_id_v<IS> && a.mode != 0)
+        return Status{1, "Error: IDs requested from a tree built without IDs"};
       switch (F.shift) {
-        case 0: case 1: case 2: case 3: case 4: case 5: case 6:
-          return go(std::integral_constant<int, 1>{});
         case 7: return go(std::integral_constant<int, 2>{});
         case 8: return go(std::integral_constant<int, 4>{});
         case 9: return go(std::integral_constant<int, 8>{});
+        default: return go(std::integral_constant<int, 1>{});
       }
-      return Status{1, "Error: stationary forest passes hold trees of at most 511 points"};
+    });
+  });
+}
+
+size_t forest_bounds_bytes(int dtype, size_t n, size_t slots_per_tree) {
+  if (n == 0 || slots_per_tree == 0) return 0;
+  return (n + slots_per_tree - 1) / slots_per_tree * 2 * dtype_size(dtype);
+}
+
+Status build_forest_bounds(int dtype, int id_dtype, size_t n, size_t slots_per_tree, const void* nodes_d,
+                           void* bounds_d, hipStream_t s) {
+  if (n == 0) return Status{};
+  SearchArgs a{};
+  a.n = n;
+  a.slots_per_tree = slots_per_tree;
+  a.nq = 1;
+  const ForestArgs F = forest_args(a, 1);
+  return fwith_t(dtype, [&](auto tv) -> Status {
+    using T = decltype(tv);
+    return fwith_is(id_dtype, [&](auto iv) -> Status {
+      using IS = decltype(iv);
+      const unsigned grid = (F.ntrees + kFWaves - 1) / kFWaves;
+      hipLaunchKernelGGL((k_forest_bounds<T, IS>), dim3(grid), dim3(kFBlock), 0, s,
+                         static_cast<const Node<T, IS>*>(nodes_d), F, static_cast<Range1<T>*>(bounds_d));
+      return hip_status(hipGetLastError(), "Error in launching the forest's per-tree ranges");
     });
   });
 }

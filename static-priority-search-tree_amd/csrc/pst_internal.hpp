@@ -127,6 +127,7 @@ struct SearchArgs {
   const void* single_query;  // host Query<T> for one query (queries ignored)
   unsigned long long* debug;  // optional per-round counters (8 per round)
   unsigned long long out_total = ~0ull;  // records `out` holds (~0: unlimited; 0: nothing is written)
+  const void* tree_bounds = nullptr;     // forest: each tree's dim1 range (T lo, hi)
 };
 Status run_search(const SearchArgs& a, SearchScratch& sc, hipStream_t stream);
 unsigned long long* search_error_flag(SearchScratch& sc);
@@ -145,6 +146,10 @@ size_t forest_dense_scratch_bytes(size_t n, size_t slots_per_tree, size_t nq, in
 // count of the same batch, otherwise the pass counts first (and stores each
 // query's total in counts). Asynchronous.
 Status forest_dense_search(const SearchArgs& a, void* segbuf, hipStream_t stream, int plan);
+// Each forest tree's dim1 range (two T per tree), read by the stationary passes.
+size_t forest_bounds_bytes(int dtype, size_t n, size_t slots_per_tree);
+Status build_forest_bounds(int dtype, int id_dtype, size_t n, size_t slots_per_tree, const void* nodes_d,
+                           void* bounds_d, hipStream_t stream);
 
 // ---- level index of a deep tree (pst_levels.hip) ----
 // Device bytes of the index: bit_length(n) levels of n (key, ID) pairs,
