@@ -180,22 +180,34 @@ __device__ inline void write_node(Node<T, IS>* nodes, uint64_t slot, const Point
 }
 
 // Emits the level's nodes and compacts A into the next level: the point at
-// local position x of segment j (x != q_j) moves to start_j - j + x - [x > q_j].
+// local position x of segment j (x != q_j) moves to start_j - j + x', with
+// x' = x - [x > q_j]. Unless level t + 1 is built in LDS, the same pass
+// finds the next level's minima: x' < floor(m_j / 2) lands in child 2j at
+// position x', the rest in child 2j + 1 at x' - floor(m_j / 2), and the
+// minimum of (rank << 32 | child position) per child goes to nextmin.
 template <typename T, typename IS>
 __global__ __launch_bounds__(kLevelBlock) void k_level_emit(
     const uint32_t* __restrict__ A, uint32_t* __restrict__ Anext, uint32_t n_level, uint32_t t,
     uint64_t M, const uint32_t* __restrict__ starts, const unsigned long long* __restrict__ segmin,
-    const uint32_t* __restrict__ perm2, const Point<T, IS>* __restrict__ pts, Node<T, IS>* __restrict__ nodes) {
+    unsigned long long* __restrict__ nextmin, const uint32_t* __restrict__ perm2,
+    const Point<T, IS>* __restrict__ pts, Node<T, IS>* __restrict__ nodes) {
+  constexpr uint32_t kChildSlots = 16;  // children per tile: see the bound where this kernel is launched
   const uint32_t nseg = 1u << t;
   __shared__ uint32_t seg0;
+  __shared__ unsigned long long cmin[kChildSlots];
   const uint32_t tile = blockIdx.x * kLevelTile;
   if (threadIdx.x == 0) seg0 = find_segment(starts, nseg, tile);
+  if (threadIdx.x < kChildSlots) cmin[threadIdx.x] = ~0ull;
   __syncthreads();
+  const uint32_t c0 = 2 * seg0;  // the tile's first child segment
   const uint32_t first = tile + threadIdx.x;
   uint32_t j = seg0;
   uint32_t st = starts[j];
   uint32_t next = (j + 1 < nseg) ? starts[j + 1] : n_level;
   unsigned long long mn = segmin[j];
+  uint32_t half = static_cast<uint32_t>(level_size(M, t, j)) >> 1;
+  unsigned long long best = ~0ull;  // running minimum of child bc
+  uint32_t bc = c0;
 #pragma unroll 4
   for (int e = 0; e < kLevelElems; e++) {
     const uint32_t P = first + e * kLevelBlock;
@@ -205,12 +217,13 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_emit(
       st = next;
       next = (j + 1 < nseg) ? starts[j + 1] : n_level;
       mn = segmin[j];
+      half = static_cast<uint32_t>(level_size(M, t, j)) >> 1;
     }
     const uint32_t x = P - st;
     const uint32_t q = static_cast<uint32_t>(mn);
+    const uint32_t a = A[P];
     if (x == q) {
-      const uint32_t m = static_cast<uint32_t>(level_size(M, t, j));
-      const uint32_t L = m >> 1;
+      const uint32_t L = half;
       const Point<T, IS> p = pts[perm2[static_cast<uint32_t>(mn >> 32)]];
       T med = p.dim1_val;
       if (L >= 1) {
@@ -219,9 +232,25 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_emit(
       }
       write_node(nodes, (1ull << t) - 1 + j, p, med);
     } else {
-      Anext[st - j + x - (x > q ? 1u : 0u)] = A[P];
+      const uint32_t xr = x - (x > q ? 1u : 0u);
+      Anext[st - j + xr] = a;
+      if (nextmin) {
+        const bool right = xr >= half;
+        const uint32_t c = 2 * j + (right ? 1u : 0u);
+        const unsigned long long key = (static_cast<unsigned long long>(a) << 32) | (right ? xr - half : xr);
+        if (c != bc) {
+          if (best != ~0ull) atomicMin(&cmin[bc - c0], best);
+          best = ~0ull;
+          bc = c;
+        }
+        best = key < best ? key : best;
+      }
     }
   }
+  if (!nextmin) return;
+  if (best != ~0ull) atomicMin(&cmin[bc - c0], best);
+  __syncthreads();
+  if (threadIdx.x < kChildSlots && cmin[threadIdx.x] != ~0ull) atomicMin(&nextmin[c0 + threadIdx.x], cmin[threadIdx.x]);
 }
 
 // ---------------------------------------------------------------------------
@@ -311,19 +340,28 @@ __device__ void lds_build(uint32_t* bufA, uint32_t* bufB, unsigned long long* sr
   }
 }
 
-// The subtree build of the deep tree's last levels. Every point of the
-// subtree is looked up once up front (rank -> input index -> dim1), all
-// lookups in flight together; the levels then work on 16-bit local
-// positions: the node of a segment is the position of least rank, med is
-// read from LDS, and only the node's own record is read from the input.
+// The subtree build of the deep tree's last levels. Every point's rank and
+// input index are looked up once up front, all lookups in flight together;
+// the levels then work on 16-bit local positions in LDS: the node of a
+// segment is the position of least rank, and the level only notes, per
+// point, the subtree slot it fills and its med point. The node records are
+// read from the input and written after the last level, all reads in
+// flight together, so that no level waits on global memory.
 template <typename T, typename IS>
-__device__ void lds_build_local(uint16_t* bufA, uint16_t* bufB, const uint32_t* rank, const uint32_t* idx,
-                                const T* d1, unsigned long long* sres, unsigned long long* wred, uint32_t m,
-                                uint64_t root_slot, const Point<T, IS>* __restrict__ pts,
-                                Node<T, IS>* __restrict__ nodes) {
+__device__ void lds_build_local(uint16_t* bufA, uint16_t* bufB, const uint32_t* rank, uint16_t* slot_of,
+                                uint16_t* med_of, unsigned long long* sres, unsigned long long* wred, uint32_t m) {
   const uint32_t tid = threadIdx.x;
   const uint32_t nthr = blockDim.x;
   const uint64_t Mp = static_cast<uint64_t>(m) + 1;
+  // node of segment k of level u: point bufA[st + q]; its subtree slot
+  // 2^u - 1 + k; med: the point at position L or L - 1 (L = floor(sz / 2)),
+  // or the node itself when L = 0
+  auto note = [&](uint32_t u, uint32_t k, uint32_t st, uint32_t sz, uint32_t q) {
+    const uint32_t L = sz >> 1;
+    const uint16_t x = bufA[st + q];
+    slot_of[x] = static_cast<uint16_t>((1u << u) - 1 + k);
+    med_of[x] = L >= 1 ? bufA[st + (q < L ? L : L - 1)] : x;
+  };
   for (uint32_t u = 0; (Mp >> u) != 0; u++) {
     const uint32_t nsub = 1u << u;
     if (nsub <= nthr) {
@@ -353,14 +391,8 @@ __device__ void lds_build_local(uint16_t* bufA, uint16_t* bufB, const uint32_t* 
       if (g == 0) sres[k] = best;
       __syncthreads();
       if (sz > 0) {
-        const unsigned long long mn = sres[k];
-        const uint32_t q = static_cast<uint32_t>(mn);
-        const uint32_t L = static_cast<uint32_t>(sz) >> 1;
-        if (g == 0) {
-          const Point<T, IS> p = pts[idx[bufA[st + q]]];
-          const T med = L >= 1 ? d1[bufA[st + (q < L ? L : L - 1)]] : p.dim1_val;
-          write_node(nodes, (root_slot + 1) * nsub - 1 + k, p, med);
-        }
+        const uint32_t q = static_cast<uint32_t>(sres[k]);
+        if (g == 0) note(u, k, st, static_cast<uint32_t>(sz), q);
         const uint32_t dst0 = static_cast<uint32_t>(sz > 1 ? level_start(Mp, u + 1, 2 * k) : 0);
         for (int64_t x = g; x < sz; x += G) {
           if (static_cast<uint32_t>(x) == q) continue;
@@ -379,10 +411,7 @@ __device__ void lds_build_local(uint16_t* bufA, uint16_t* bufB, const uint32_t* 
           best = key < best ? key : best;
         }
         const uint32_t q = static_cast<uint32_t>(best);
-        const uint32_t L = static_cast<uint32_t>(sz) >> 1;
-        const Point<T, IS> p = pts[idx[bufA[st + q]]];
-        const T med = L >= 1 ? d1[bufA[st + (q < L ? L : L - 1)]] : p.dim1_val;
-        write_node(nodes, (root_slot + 1) * nsub - 1 + k, p, med);
+        note(u, k, st, static_cast<uint32_t>(sz), q);
         if (sz > 1) {
           const uint32_t dst0 = static_cast<uint32_t>(level_start(Mp, u + 1, 2 * k));
           for (int64_t x = 0; x < sz; x++) {
@@ -407,16 +436,18 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish(
   __shared__ uint16_t bufB[kFinishTile];
   __shared__ uint32_t rank[kFinishTile];
   __shared__ uint32_t idx[kFinishTile];
-  __shared__ T d1[kFinishTile];
+  __shared__ uint16_t slot_of[kFinishTile];  // the subtree slot (heap order) each point fills
+  __shared__ uint16_t med_of[kFinishTile];   // and the point whose dim1 is its med
   __shared__ unsigned long long sres[kFinishBlock];  // one per segment of a level with <= kFinishBlock segments
   __shared__ unsigned long long wred[kFinishBlock / kWave];
   const uint32_t j = blockIdx.x;
   const int64_t m = level_size(M, t0, j);
   if (m <= 0) return;
   const uint32_t st = starts[j];
-  // rank -> input index -> dim1 for every point, all loads in flight together
+  const uint64_t root = (1ull << t0) - 1 + j;
+  // rank -> input index for every point, all loads in flight together
   constexpr int kPer = kFinishTile / kFinishBlock;
-  uint32_t r[kPer], ix[kPer];
+  uint32_t r[kPer];
 #pragma unroll
   for (int e = 0; e < kPer; e++) {
     const uint32_t x = threadIdx.x + e * kFinishBlock;
@@ -425,21 +456,35 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish(
 #pragma unroll
   for (int e = 0; e < kPer; e++) {
     const uint32_t x = threadIdx.x + e * kFinishBlock;
-    if (x < m) ix[e] = perm2[r[e]];
+    if (x < m) {
+      rank[x] = r[e];
+      idx[x] = perm2[r[e]];
+      bufA[x] = static_cast<uint16_t>(x);
+    }
+  }
+  __syncthreads();
+  lds_build_local<T, IS>(bufA, bufB, rank, slot_of, med_of, sres, wred, static_cast<uint32_t>(m));
+  // every point's node record: subtree slot h (level u, position k) is
+  // slot ((root + 1) << u) - 1 + k of the tree
+  Point<T, IS> p[kPer];
+  T med[kPer];
+#pragma unroll
+  for (int e = 0; e < kPer; e++) {
+    const uint32_t x = threadIdx.x + e * kFinishBlock;
+    if (x < m) {
+      p[e] = pts[idx[x]];
+      med[e] = pts[idx[med_of[x]]].dim1_val;
+    }
   }
 #pragma unroll
   for (int e = 0; e < kPer; e++) {
     const uint32_t x = threadIdx.x + e * kFinishBlock;
     if (x < m) {
-      d1[x] = pts[ix[e]].dim1_val;
-      rank[x] = r[e];
-      idx[x] = ix[e];
-      bufA[x] = static_cast<uint16_t>(x);
+      const uint32_t h = slot_of[x] + 1u;
+      const uint32_t u = 31u - static_cast<uint32_t>(__builtin_clz(h));
+      write_node(nodes, ((root + 1) << u) - 1 + (h - (1u << u)), p[e], med[e]);
     }
   }
-  __syncthreads();
-  lds_build_local<T, IS>(bufA, bufB, rank, idx, d1, sres, wred, static_cast<uint32_t>(m), (1ull << t0) - 1 + j,
-                         pts, nodes);
 }
 
 // ---------------------------------------------------------------------------
@@ -837,13 +882,19 @@ Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, 
   unsigned long long* segmin = reinterpret_cast<unsigned long long*>(base + BL.segmin);
   k_fill_u64<<<grid_for(nmin, B), B, 0, s>>>(segmin, nmin, ~0ull);
   const uint32_t* starts = starts_d;
+  // level 0's minimum by its own pass; each emit pass then finds the next
+  // level's minima while compacting. Level t < t0 - 1 has segments of more
+  // than kFinishTile points (level t0 - 1 the fewest, above kFinishTile), so
+  // an 8192-point tile meets at most 8192 / 2049 + 2 < 8 of them and at most
+  // 16 of their children.
+  if (t0 > 0)
+    k_level_argmin<<<grid_for(n, kLevelTile), kLevelBlock, 0, s>>>(Acur, n, 1u, starts, segmin);
   for (uint32_t t = 0; t < t0; t++) {
     const uint32_t n_level = n - ((1u << t) - 1);
     const unsigned tiles = grid_for(n_level, kLevelTile);
-    k_level_argmin<<<tiles, kLevelBlock, 0, s>>>(Acur, n_level, 1u << t, starts + (1u << t) - 1,
-                                                 segmin + (1u << t) - 1);
+    unsigned long long* nextmin = t + 1 < t0 ? segmin + (2u << t) - 1 : nullptr;
     k_level_emit<T, IS><<<tiles, kLevelBlock, 0, s>>>(Acur, Anxt, n_level, t, M, starts + (1u << t) - 1,
-                                                      segmin + (1u << t) - 1, perm2, ptsS, nodes);
+                                                      segmin + (1u << t) - 1, nextmin, perm2, ptsS, nodes);
     std::swap(Acur, Anxt);
   }
   mark(2);
