@@ -787,7 +787,7 @@ def main():
     # roofline of the write pass on the per-reported-ID model (21 B/ID)
     achieved = total * BYTES_PER_ID / (t_write * 1e-3) / 1e9
     traffic = traffic_src = None
-    tpath = os.path.join(ROOT, "profiles", "write_pass_k_search_round_k_copy_tasks_f32u32_2e26_batch2048.traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "walk_write_pass_f32u32_2e26_batch2048.traffic.json")
     if args.log2n == 26 and args.nq2 == 1024 and args.nq3 == 1024 and os.path.exists(tpath):
         with open(tpath) as f:
             tj = json.load(f)
