@@ -191,13 +191,13 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_emit(
     uint64_t M, const uint32_t* __restrict__ starts, const unsigned long long* __restrict__ segmin,
     unsigned long long* __restrict__ nextmin, const uint32_t* __restrict__ perm2,
     const Point<T, IS>* __restrict__ pts, Node<T, IS>* __restrict__ nodes) {
-  constexpr uint32_t kChildSlots = 16;  // children per tile: see the bound where this kernel is launched
+  constexpr uint32_t kChildSlots = 2 * (kLevelTile / kFinishTile + 2);  // children per tile (bound at the launch)
   const uint32_t nseg = 1u << t;
   __shared__ uint32_t seg0;
   __shared__ unsigned long long cmin[kChildSlots];
   const uint32_t tile = blockIdx.x * kLevelTile;
   if (threadIdx.x == 0) seg0 = find_segment(starts, nseg, tile);
-  if (threadIdx.x < kChildSlots) cmin[threadIdx.x] = ~0ull;
+  for (uint32_t i = threadIdx.x; i < kChildSlots; i += kLevelBlock) cmin[i] = ~0ull;
   __syncthreads();
   const uint32_t c0 = 2 * seg0;  // the tile's first child segment
   const uint32_t first = tile + threadIdx.x;
@@ -250,7 +250,8 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_emit(
   if (!nextmin) return;
   if (best != ~0ull) atomicMin(&cmin[bc - c0], best);
   __syncthreads();
-  if (threadIdx.x < kChildSlots && cmin[threadIdx.x] != ~0ull) atomicMin(&nextmin[c0 + threadIdx.x], cmin[threadIdx.x]);
+  for (uint32_t i = threadIdx.x; i < kChildSlots; i += kLevelBlock)
+    if (cmin[i] != ~0ull) atomicMin(&nextmin[c0 + i], cmin[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -349,18 +350,26 @@ __device__ void lds_build(uint32_t* bufA, uint32_t* bufB, unsigned long long* sr
 // flight together, so that no level waits on global memory.
 template <typename T, typename IS>
 __device__ void lds_build_local(uint16_t* bufA, uint16_t* bufB, const uint32_t* rank, uint16_t* slot_of,
-                                uint16_t* med_of, unsigned long long* sres, unsigned long long* wred, uint32_t m) {
+                                uint16_t* med_of, uint16_t* sst, unsigned long long* sres, unsigned long long* wred,
+                                uint32_t m) {
   const uint32_t tid = threadIdx.x;
   const uint32_t nthr = blockDim.x;
   const uint64_t Mp = static_cast<uint64_t>(m) + 1;
   // node of segment k of level u: point bufA[st + q]; its subtree slot
-  // 2^u - 1 + k; med: the point at position L or L - 1 (L = floor(sz / 2)),
-  // or the node itself when L = 0
+  // h = 2^u - 1 + k; med: the point at position L or L - 1 (L = floor(sz /
+  // 2)), or the node itself when L = 0. Segment starts are kept per slot
+  // (sst[h], sst[0] = 0 set by the caller): with the node removed, the
+  // children start at st - k and st - k + L of the next level's array.
   auto note = [&](uint32_t u, uint32_t k, uint32_t st, uint32_t sz, uint32_t q) {
     const uint32_t L = sz >> 1;
     const uint16_t x = bufA[st + q];
-    slot_of[x] = static_cast<uint16_t>((1u << u) - 1 + k);
+    const uint32_t h = (1u << u) - 1 + k;
+    slot_of[x] = static_cast<uint16_t>(h);
     med_of[x] = L >= 1 ? bufA[st + (q < L ? L : L - 1)] : x;
+    if (sz > 1) {
+      sst[2 * h + 1] = static_cast<uint16_t>(st - k);
+      sst[2 * h + 2] = static_cast<uint16_t>(st - k + L);
+    }
   };
   for (uint32_t u = 0; (Mp >> u) != 0; u++) {
     const uint32_t nsub = 1u << u;
@@ -368,7 +377,7 @@ __device__ void lds_build_local(uint16_t* bufA, uint16_t* bufB, const uint32_t* 
       const uint32_t G = nthr / nsub;
       const uint32_t k = tid / G, g = tid % G;
       const int64_t sz = level_size(Mp, u, k);
-      const uint32_t st = static_cast<uint32_t>(sz > 0 ? level_start(Mp, u, k) : 0);
+      const uint32_t st = sz > 0 ? sst[(1u << u) - 1 + k] : 0u;
       unsigned long long best = ~0ull;
       for (int64_t x = g; x < sz; x += G) {
         const unsigned long long key =
@@ -393,7 +402,7 @@ __device__ void lds_build_local(uint16_t* bufA, uint16_t* bufB, const uint32_t* 
       if (sz > 0) {
         const uint32_t q = static_cast<uint32_t>(sres[k]);
         if (g == 0) note(u, k, st, static_cast<uint32_t>(sz), q);
-        const uint32_t dst0 = static_cast<uint32_t>(sz > 1 ? level_start(Mp, u + 1, 2 * k) : 0);
+        const uint32_t dst0 = st - k;
         for (int64_t x = g; x < sz; x += G) {
           if (static_cast<uint32_t>(x) == q) continue;
           bufB[dst0 + x - (static_cast<uint32_t>(x) > q ? 1 : 0)] = bufA[st + x];
@@ -403,7 +412,7 @@ __device__ void lds_build_local(uint16_t* bufA, uint16_t* bufB, const uint32_t* 
       for (uint32_t k = tid; k < nsub; k += nthr) {
         const int64_t sz = level_size(Mp, u, k);
         if (sz <= 0) continue;
-        const uint32_t st = static_cast<uint32_t>(level_start(Mp, u, k));
+        const uint32_t st = sst[(1u << u) - 1 + k];
         unsigned long long best = ~0ull;
         for (int64_t x = 0; x < sz; x++) {
           const unsigned long long key =
@@ -413,7 +422,7 @@ __device__ void lds_build_local(uint16_t* bufA, uint16_t* bufB, const uint32_t* 
         const uint32_t q = static_cast<uint32_t>(best);
         note(u, k, st, static_cast<uint32_t>(sz), q);
         if (sz > 1) {
-          const uint32_t dst0 = static_cast<uint32_t>(level_start(Mp, u + 1, 2 * k));
+          const uint32_t dst0 = st - k;
           for (int64_t x = 0; x < sz; x++) {
             if (static_cast<uint32_t>(x) == q) continue;
             bufB[dst0 + x - (static_cast<uint32_t>(x) > q ? 1 : 0)] = bufA[st + x];
@@ -435,9 +444,9 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish(
   __shared__ uint16_t bufA[kFinishTile];
   __shared__ uint16_t bufB[kFinishTile];
   __shared__ uint32_t rank[kFinishTile];
-  __shared__ uint32_t idx[kFinishTile];
   __shared__ uint16_t slot_of[kFinishTile];  // the subtree slot (heap order) each point fills
   __shared__ uint16_t med_of[kFinishTile];   // and the point whose dim1 is its med
+  __shared__ uint16_t sst[2 * kFinishTile];  // segment start per subtree slot
   __shared__ unsigned long long sres[kFinishBlock];  // one per segment of a level with <= kFinishBlock segments
   __shared__ unsigned long long wred[kFinishBlock / kWave];
   const uint32_t j = blockIdx.x;
@@ -445,35 +454,38 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish(
   if (m <= 0) return;
   const uint32_t st = starts[j];
   const uint64_t root = (1ull << t0) - 1 + j;
-  // rank -> input index for every point, all loads in flight together
   constexpr int kPer = kFinishTile / kFinishBlock;
-  uint32_t r[kPer];
-#pragma unroll
-  for (int e = 0; e < kPer; e++) {
-    const uint32_t x = threadIdx.x + e * kFinishBlock;
-    if (x < m) r[e] = A[st + x];
-  }
 #pragma unroll
   for (int e = 0; e < kPer; e++) {
     const uint32_t x = threadIdx.x + e * kFinishBlock;
     if (x < m) {
-      rank[x] = r[e];
-      idx[x] = perm2[r[e]];
+      rank[x] = A[st + x];
       bufA[x] = static_cast<uint16_t>(x);
     }
   }
+  if (threadIdx.x == 0) sst[0] = 0;
   __syncthreads();
-  lds_build_local<T, IS>(bufA, bufB, rank, slot_of, med_of, sres, wred, static_cast<uint32_t>(m));
-  // every point's node record: subtree slot h (level u, position k) is
-  // slot ((root + 1) << u) - 1 + k of the tree
+  lds_build_local<T, IS>(bufA, bufB, rank, slot_of, med_of, sst, sres, wred, static_cast<uint32_t>(m));
+  // every point's node record (rank -> input index -> record; all loads in
+  // flight together): subtree slot h (level u, position k) is slot
+  // ((root + 1) << u) - 1 + k of the tree
+  uint32_t ix[kPer], im[kPer];
+#pragma unroll
+  for (int e = 0; e < kPer; e++) {
+    const uint32_t x = threadIdx.x + e * kFinishBlock;
+    if (x < m) {
+      ix[e] = perm2[rank[x]];
+      im[e] = perm2[rank[med_of[x]]];
+    }
+  }
   Point<T, IS> p[kPer];
   T med[kPer];
 #pragma unroll
   for (int e = 0; e < kPer; e++) {
     const uint32_t x = threadIdx.x + e * kFinishBlock;
     if (x < m) {
-      p[e] = pts[idx[x]];
-      med[e] = pts[idx[med_of[x]]].dim1_val;
+      p[e] = pts[ix[e]];
+      med[e] = pts[im[e]].dim1_val;
     }
   }
 #pragma unroll
@@ -883,10 +895,11 @@ Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, 
   k_fill_u64<<<grid_for(nmin, B), B, 0, s>>>(segmin, nmin, ~0ull);
   const uint32_t* starts = starts_d;
   // level 0's minimum by its own pass; each emit pass then finds the next
-  // level's minima while compacting. Level t < t0 - 1 has segments of more
-  // than kFinishTile points (level t0 - 1 the fewest, above kFinishTile), so
-  // an 8192-point tile meets at most 8192 / 2049 + 2 < 8 of them and at most
-  // 16 of their children.
+  // level's minima while compacting. Segment sizes within a level differ by
+  // at most one, and some segment of every level t < t0 holds more than
+  // kFinishTile points, so all hold at least kFinishTile: a tile of
+  // kLevelTile points meets at most kLevelTile / kFinishTile + 2 segments,
+  // twice as many children.
   if (t0 > 0)
     k_level_argmin<<<grid_for(n, kLevelTile), kLevelBlock, 0, s>>>(Acur, n, 1u, starts, segmin);
   for (uint32_t t = 0; t < t0; t++) {
