@@ -260,35 +260,44 @@ __global__ __launch_bounds__(kFBlock) void k_forest_count(const Node<T, IS>* __r
 }
 
 // Per query (one workgroup each): exclusive scan of its segment counts in
-// place; totals[q] = the query's hit count.
+// place; totals[q] = the query's hit count. Wave w takes a contiguous
+// chunk of the row: it sums the chunk (coalesced loads), the waves' sums
+// give each wave's prefix, and the wave then scans its chunk 64 counts at
+// a time from that prefix (the row stays in L2 between the two reads).
 __global__ __launch_bounds__(kScanBlock) void k_forest_seg_scan(uint32_t* __restrict__ segcnt, uint32_t nseg,
                                                                 unsigned long long* __restrict__ totals) {
-  __shared__ unsigned long long part[kScanBlock / kWave];
-  __shared__ unsigned long long carry_s;
+  constexpr uint32_t kWaves = kScanBlock / kWave;
+  __shared__ unsigned long long part[kWaves];
   const uint32_t q = blockIdx.x;
   uint32_t* row = segcnt + static_cast<uint64_t>(q) * nseg;
-  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
-  if (tid == 0) carry_s = 0;
+  const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+  const uint32_t chunk = (nseg + kWaves * kWave - 1) / (kWaves * kWave) * kWave;  // a multiple of 64
+  const uint32_t lo = std::min(nseg, wave * chunk), hi = std::min(nseg, lo + chunk);
+  unsigned long long sum = 0;
+  for (uint32_t i = lo + lane; i < hi; i += kWave) sum += row[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, kWave);
+  if (lane == 0) part[wave] = sum;
   __syncthreads();
-  for (uint32_t base = 0; base < nseg; base += kScanBlock) {
-    const uint32_t i = base + tid;
-    const unsigned long long v = i < nseg ? row[i] : 0ull;
+  unsigned long long run = 0;
+  for (uint32_t w = 0; w < wave; w++) run += part[w];
+  for (uint32_t i0 = lo; i0 < hi; i0 += kWave) {
+    const uint32_t i = i0 + lane;
+    const unsigned long long v = i < hi ? row[i] : 0ull;
     unsigned long long inc = v;
 #pragma unroll
     for (int off = 1; off < kWave; off <<= 1) {
       const unsigned long long o = __shfl_up(inc, off, kWave);
       if (static_cast<int>(lane) >= off) inc += o;
     }
-    if (lane == kWave - 1) part[wave] = inc;
-    __syncthreads();
-    unsigned long long before = carry_s;
-    for (uint32_t w = 0; w < wave; w++) before += part[w];
-    if (i < nseg) row[i] = static_cast<uint32_t>(before + inc - v);
-    __syncthreads();
-    if (tid == kScanBlock - 1) carry_s = before + inc;
-    __syncthreads();
+    if (i < hi) row[i] = static_cast<uint32_t>(run + inc - v);
+    run += __shfl(inc, kWave - 1, kWave);
   }
-  if (tid == 0) totals[q] = carry_s;
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (uint32_t w = 0; w < kWaves; w++) t += part[w];
+    totals[q] = t;
+  }
 }
 
 // Pass 2: query q's hits of segment seg at out + offsets[q] + segoff[q][seg]
