@@ -7,7 +7,7 @@
 // ord_key), sorted 8 bits per pass, least significant digit first. Every
 // pass is stable, which construction relies on for its index tie-breaks.
 //
-// One pass over tiles of 4096 keys (four waves of 1024, each wave's keys
+// One pass over tiles of 8192 keys (four waves of 2048, each wave's keys
 // striped so that lane l of round r holds key r * 64 + l of the wave):
 //   k_upsweep    per tile, the count of every digit (LDS atomics);
 //   k_scan_rows  per digit, an exclusive scan of its tile counts in tile
@@ -37,9 +37,9 @@ constexpr int kDigitBits = 8;
 constexpr int kDigits = 1 << kDigitBits;
 constexpr int kSortBlock = 256;
 constexpr int kSortWaves = kSortBlock / kWave;
-constexpr int kSortPer = 16;                                    // keys per lane
-constexpr uint32_t kWaveKeys = kWave * kSortPer;                // 1024
-constexpr uint32_t kSortTile = kSortBlock * kSortPer;           // 4096
+constexpr int kSortPer = 32;                                    // keys per lane
+constexpr uint32_t kWaveKeys = kWave * kSortPer;                // 2048
+constexpr uint32_t kSortTile = kSortBlock * kSortPer;           // 8192
 constexpr int kScanThreads = 1024;
 
 __global__ void k_zero_u32_sort(uint32_t* __restrict__ p, uint32_t count) {
@@ -51,20 +51,29 @@ __device__ inline uint32_t digit_of(K k, uint32_t shift) {
   return static_cast<uint32_t>(k >> shift) & (kDigits - 1);
 }
 
+// LDS digit counters are kept in kSub copies, lane l adding to copy l %
+// kSub: lanes of a wave that share a digit (frequent in the high digits of
+// clustered keys) then collide on one counter kSub times less often.
+constexpr int kSub = 4;
+
 // Digit totals of every pass: hist[p * 256 + d].
 template <typename K>
 __global__ __launch_bounds__(kSortBlock) void k_hist_all(const K* __restrict__ keys, uint32_t n, uint32_t passes,
                                                          uint32_t* __restrict__ hist) {
-  __shared__ uint32_t h[8][kDigits];
-  for (uint32_t i = threadIdx.x; i < passes * kDigits; i += kSortBlock) h[i / kDigits][i % kDigits] = 0;
+  __shared__ uint32_t h[8][kSub][kDigits];
+  for (uint32_t i = threadIdx.x; i < passes * kSub * kDigits; i += kSortBlock) (&h[0][0][0])[i] = 0;
   __syncthreads();
+  const uint32_t sub = threadIdx.x % kSub;
   for (uint32_t i = blockIdx.x * kSortBlock + threadIdx.x; i < n; i += gridDim.x * kSortBlock) {
     const K k = keys[i];
-    for (uint32_t p = 0; p < passes; p++) atomicAdd(&h[p][digit_of(k, p * kDigitBits)], 1u);
+    for (uint32_t p = 0; p < passes; p++) atomicAdd(&h[p][sub][digit_of(k, p * kDigitBits)], 1u);
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < passes * kDigits; i += kSortBlock) {
-    const uint32_t c = h[i / kDigits][i % kDigits];
+    const uint32_t p = i / kDigits, d = i % kDigits;
+    uint32_t c = 0;
+#pragma unroll
+    for (int u = 0; u < kSub; u++) c += h[p][u][d];
     if (c) atomicAdd(&hist[i], c);
   }
 }
@@ -91,9 +100,11 @@ __global__ __launch_bounds__(kDigits) void k_digit_starts(uint32_t* __restrict__
 template <typename K>
 __global__ __launch_bounds__(kSortBlock) void k_upsweep(const K* __restrict__ keys, uint32_t n, uint32_t shift,
                                                         uint32_t ntiles, uint32_t* __restrict__ counts) {
-  __shared__ uint32_t h[kDigits];
-  h[threadIdx.x] = 0;
+  __shared__ uint32_t hs[kSub][kDigits];
+#pragma unroll
+  for (int u = 0; u < kSub; u++) hs[u][threadIdx.x] = 0;
   __syncthreads();
+  uint32_t* h = hs[threadIdx.x % kSub];
   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSortTile;
   const uint32_t valid = static_cast<uint32_t>(std::min<uint64_t>(kSortTile, n - base));
   K k[kSortPer];
@@ -108,7 +119,10 @@ __global__ __launch_bounds__(kSortBlock) void k_upsweep(const K* __restrict__ ke
     if (i < valid) atomicAdd(&h[digit_of(k[e], shift)], 1u);
   }
   __syncthreads();
-  counts[static_cast<uint64_t>(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
+  uint32_t c = 0;
+#pragma unroll
+  for (int u = 0; u < kSub; u++) c += hs[u][threadIdx.x];
+  counts[static_cast<uint64_t>(threadIdx.x) * ntiles + blockIdx.x] = c;
 }
 
 // Per digit (one block each): exclusive scan of its tile counts, offset by

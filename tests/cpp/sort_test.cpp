@@ -1,6 +1,6 @@
 // GPU check of the library's radix sorts (pst::radix_sort_u32 / _u64)
 // against std::stable_sort on the host: keys with many duplicates, all-equal
-// keys, sizes around the 4096-key tile, values carried along (iota), inputs
+// keys, sizes around the 8192-key tile, values carried along (iota), inputs
 // left unchanged. Prints one line per case and exits non-zero on the first
 // mismatch. Built by __graft_entry__.build(); run by tests/test_gpu_sort.py.
 #include <hip/hip_runtime.h>
@@ -76,7 +76,7 @@ int run_case(size_t n, int dist, uint64_t seed) {
 }
 
 int main() {
-  const size_t sizes[] = {1, 2, 100, 4095, 4096, 4097, 8191, 100003, (1u << 20) + 3, 1u << 24};
+  const size_t sizes[] = {1, 2, 100, 4095, 4096, 4097, 8191, 8192, 8193, 16385, 100003, (1u << 20) + 3, 1u << 24};
   int rc = 0;
   for (size_t n : sizes)
     for (int dist = 0; dist < 4; dist++) {
