@@ -11,8 +11,8 @@
 // striped so that lane l of round r holds key r * 64 + l of the wave):
 //   k_upsweep    per tile, the count of every digit (LDS atomics);
 //   k_scan_rows  per digit, an exclusive scan of its tile counts in tile
-//                order plus the digit's global start (the digit totals of
-//                all passes come from one histogram pass over the keys);
+//                order, and the digit's total;
+//   k_digit_starts  the digits' starts: an exclusive scan of the totals;
 //   k_downsweep  per tile, a stable rank for every key: within a wave by
 //                ballot matching of the 8 digit bits (lanes with the same
 //                digit in a round), across rounds by per-wave digit
@@ -42,10 +42,6 @@ constexpr uint32_t kWaveKeys = kWave * kSortPer;                // 2048
 constexpr uint32_t kSortTile = kSortBlock * kSortPer;           // 8192
 constexpr int kScanThreads = 1024;
 
-__global__ void k_zero_u32_sort(uint32_t* __restrict__ p, uint32_t count) {
-  for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) p[i] = 0;
-}
-
 template <typename K>
 __device__ inline uint32_t digit_of(K k, uint32_t shift) {
   return static_cast<uint32_t>(k >> shift) & (kDigits - 1);
@@ -56,29 +52,7 @@ __device__ inline uint32_t digit_of(K k, uint32_t shift) {
 // clustered keys) then collide on one counter kSub times less often.
 constexpr int kSub = 4;
 
-// Digit totals of every pass: hist[p * 256 + d].
-template <typename K>
-__global__ __launch_bounds__(kSortBlock) void k_hist_all(const K* __restrict__ keys, uint32_t n, uint32_t passes,
-                                                         uint32_t* __restrict__ hist) {
-  __shared__ uint32_t h[8][kSub][kDigits];
-  for (uint32_t i = threadIdx.x; i < passes * kSub * kDigits; i += kSortBlock) (&h[0][0][0])[i] = 0;
-  __syncthreads();
-  const uint32_t sub = threadIdx.x % kSub;
-  for (uint32_t i = blockIdx.x * kSortBlock + threadIdx.x; i < n; i += gridDim.x * kSortBlock) {
-    const K k = keys[i];
-    for (uint32_t p = 0; p < passes; p++) atomicAdd(&h[p][sub][digit_of(k, p * kDigitBits)], 1u);
-  }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < passes * kDigits; i += kSortBlock) {
-    const uint32_t p = i / kDigits, d = i % kDigits;
-    uint32_t c = 0;
-#pragma unroll
-    for (int u = 0; u < kSub; u++) c += h[p][u][d];
-    if (c) atomicAdd(&hist[i], c);
-  }
-}
-
-// Exclusive scan of each pass's 256 digit totals, in place (one block).
+// Exclusive scan of each of `passes` rows of 256 digit totals, in place (one block).
 __global__ __launch_bounds__(kDigits) void k_digit_starts(uint32_t* __restrict__ hist, uint32_t passes) {
   __shared__ uint32_t s[kDigits];
   for (uint32_t p = 0; p < passes; p++) {
@@ -125,16 +99,16 @@ __global__ __launch_bounds__(kSortBlock) void k_upsweep(const K* __restrict__ ke
   counts[static_cast<uint64_t>(threadIdx.x) * ntiles + blockIdx.x] = c;
 }
 
-// Per digit (one block each): exclusive scan of its tile counts, offset by
-// the digit's start, in place.
+// Per digit (one block each): exclusive scan of its tile counts in place;
+// totals[d] = the digit's count over all tiles.
 __global__ __launch_bounds__(kScanThreads) void k_scan_rows(uint32_t* __restrict__ counts, uint32_t ntiles,
-                                                            const uint32_t* __restrict__ starts) {
+                                                            uint32_t* __restrict__ totals) {
   __shared__ uint32_t part[kScanThreads / kWave];
   __shared__ uint32_t carry;
   const uint32_t d = blockIdx.x;
   uint32_t* row = counts + static_cast<uint64_t>(d) * ntiles;
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
-  if (tid == 0) carry = starts[d];
+  if (tid == 0) carry = 0;
   __syncthreads();
   for (uint32_t b = 0; b < ntiles; b += kScanThreads) {
     const uint32_t i = b + tid;
@@ -154,13 +128,15 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_rows(uint32_t* __restrict
     if (tid == kScanThreads - 1) carry = before + inc;
     __syncthreads();
   }
+  if (tid == 0) totals[d] = carry;
 }
 
 template <typename K>
 __global__ __launch_bounds__(kSortBlock) void k_downsweep(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                           K* __restrict__ kout, uint32_t* __restrict__ vout,
                                                           uint32_t n, uint32_t shift, uint32_t ntiles,
-                                                          const uint32_t* __restrict__ offs) {
+                                                          const uint32_t* __restrict__ offs,
+                                                          const uint32_t* __restrict__ starts) {
   __shared__ K skey[kSortTile];
   __shared__ uint32_t sval[kSortTile];
   __shared__ uint32_t wcnt[kSortWaves][kDigits];  // per-wave digit counters, then their wave prefix
@@ -172,7 +148,7 @@ __global__ __launch_bounds__(kSortBlock) void k_downsweep(const K* __restrict__ 
   const uint32_t valid = static_cast<uint32_t>(std::min<uint64_t>(kSortTile, n - base));
 #pragma unroll
   for (int w = 0; w < kSortWaves; w++) wcnt[w][tid] = 0;
-  goff[tid] = offs[static_cast<uint64_t>(tid) * ntiles + tile];
+  goff[tid] = starts[tid] + offs[static_cast<uint64_t>(tid) * ntiles + tile];
   K k[kSortPer];
   uint32_t v[kSortPer];
   const uint32_t wbase = wave * kWaveKeys;
@@ -269,7 +245,7 @@ hipError_t radix_sort(void* tmp, size_t& tmp_bytes, const K* kin, K* kout, const
   const size_t need_keys = align256(n * sizeof(K));
   const size_t need_vals = align256(n * 4);
   const size_t need_counts = align256(static_cast<size_t>(ntiles) * kDigits * 4);
-  const size_t need_hist = align256(passes * kDigits * 4);
+  const size_t need_hist = align256(kDigits * 4);
   const size_t need = need_keys + need_vals + need_counts + need_hist;
   if (!tmp) {
     tmp_bytes = std::max<size_t>(need, 256);
@@ -284,14 +260,6 @@ hipError_t radix_sort(void* tmp, size_t& tmp_bytes, const K* kin, K* kout, const
   uint32_t* counts = reinterpret_cast<uint32_t*>(p + need_keys + need_vals);
   uint32_t* hist = reinterpret_cast<uint32_t*>(p + need_keys + need_vals + need_counts);
   const uint32_t n32 = static_cast<uint32_t>(n);
-  int dev = 0, cus = 0;
-  hipGetDevice(&dev);
-  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  // the histogram buffer is cleared by a kernel (capturable, no memset node)
-  hipLaunchKernelGGL(k_zero_u32_sort, dim3(1), dim3(kSortBlock), 0, s, hist, passes * kDigits);
-  const uint32_t hblocks = static_cast<uint32_t>(std::min<uint64_t>(ntiles, static_cast<uint64_t>(cus) * 4));
-  hipLaunchKernelGGL((k_hist_all<K>), dim3(std::max(1u, hblocks)), dim3(kSortBlock), 0, s, kin, n32, passes, hist);
-  hipLaunchKernelGGL(k_digit_starts, dim3(1), dim3(kDigits), 0, s, hist, passes);
   // passes alternate so that the last one writes kout: an even number of
   // passes starts into the scratch, an odd number into kout
   const K* ksrc = kin;
@@ -302,9 +270,11 @@ hipError_t radix_sort(void* tmp, size_t& tmp_bytes, const K* kin, K* kout, const
     uint32_t* vdst = to_out ? vout : vt;
     const uint32_t shift = pass * kDigitBits;
     hipLaunchKernelGGL((k_upsweep<K>), dim3(ntiles), dim3(kSortBlock), 0, s, ksrc, n32, shift, ntiles, counts);
-    hipLaunchKernelGGL(k_scan_rows, dim3(kDigits), dim3(kScanThreads), 0, s, counts, ntiles, hist + pass * kDigits);
+    // digit totals from the tile counts, then the digits' starts
+    hipLaunchKernelGGL(k_scan_rows, dim3(kDigits), dim3(kScanThreads), 0, s, counts, ntiles, hist);
+    hipLaunchKernelGGL(k_digit_starts, dim3(1), dim3(kDigits), 0, s, hist, 1u);
     hipLaunchKernelGGL((k_downsweep<K>), dim3(ntiles), dim3(kSortBlock), 0, s, ksrc, vsrc, kdst, vdst, n32, shift,
-                       ntiles, counts);
+                       ntiles, counts, hist);
     ksrc = kdst;
     vsrc = vdst;
   }
