@@ -124,25 +124,16 @@ __global__ void k_fill_u64(unsigned long long* p, size_t count, unsigned long lo
 // ---------------------------------------------------------------------------
 // Level passes over A (all segments of level t are non-empty here).
 // ---------------------------------------------------------------------------
-__device__ inline uint32_t find_segment(const uint32_t* __restrict__ starts, uint32_t nseg, uint32_t pos) {
-  uint32_t lo = 0, hi = nseg;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (starts[mid] <= pos) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
 
 // Per segment: min over (rank << 32 | local position), i.e. the max-key2
 // point and where it sits in the segment.
 __global__ __launch_bounds__(kLevelBlock) void k_level_argmin(
     const uint32_t* __restrict__ A, uint32_t n_level, uint32_t nseg,
-    const uint32_t* __restrict__ starts, unsigned long long* __restrict__ segmin) {
+    const uint32_t* __restrict__ starts, const uint32_t* __restrict__ tileseg,
+    unsigned long long* __restrict__ segmin) {
   __shared__ unsigned long long smin[8];
-  __shared__ uint32_t seg0;
   const uint32_t tile = blockIdx.x * kLevelTile;
-  if (threadIdx.x == 0) seg0 = find_segment(starts, nseg, tile);
+  const uint32_t seg0 = tileseg[blockIdx.x];  // the segment holding the tile's first point
   if (threadIdx.x < 8) smin[threadIdx.x] = ~0ull;
   __syncthreads();
   const uint32_t first = tile + threadIdx.x;
@@ -188,15 +179,14 @@ __device__ inline void write_node(Node<T, IS>* nodes, uint64_t slot, const Point
 template <typename T, typename IS>
 __global__ __launch_bounds__(kLevelBlock) void k_level_emit(
     const uint32_t* __restrict__ A, uint32_t* __restrict__ Anext, uint32_t n_level, uint32_t t,
-    uint64_t M, const uint32_t* __restrict__ starts, const unsigned long long* __restrict__ segmin,
-    unsigned long long* __restrict__ nextmin, const uint32_t* __restrict__ perm2,
-    const Point<T, IS>* __restrict__ pts, Node<T, IS>* __restrict__ nodes) {
+    uint64_t M, const uint32_t* __restrict__ starts, const uint32_t* __restrict__ tileseg,
+    const unsigned long long* __restrict__ segmin, unsigned long long* __restrict__ nextmin,
+    const uint32_t* __restrict__ perm2, const Point<T, IS>* __restrict__ pts, Node<T, IS>* __restrict__ nodes) {
   constexpr uint32_t kChildSlots = 2 * (kLevelTile / kFinishTile + 2);  // children per tile (bound at the launch)
   const uint32_t nseg = 1u << t;
-  __shared__ uint32_t seg0;
   __shared__ unsigned long long cmin[kChildSlots];
   const uint32_t tile = blockIdx.x * kLevelTile;
-  if (threadIdx.x == 0) seg0 = find_segment(starts, nseg, tile);
+  const uint32_t seg0 = tileseg[blockIdx.x];  // the segment holding the tile's first point
   for (uint32_t i = threadIdx.x; i < kChildSlots; i += kLevelBlock) cmin[i] = ~0ull;
   __syncthreads();
   const uint32_t c0 = 2 * seg0;  // the tile's first child segment
@@ -731,7 +721,8 @@ inline unsigned grid_for(size_t n, unsigned block) { return static_cast<unsigned
 // arrays, four 32-bit arrays (sort values, the key2 order, the level arrays
 // and the key2 ranks), flags, radix sort storage, level tables.
 struct BuildLayout {
-  size_t key0, key1, v0, v1, v2, r2, flags, tmp, starts, segmin, total;
+  size_t key0, key1, v0, v1, v2, r2, flags, tmp, starts, segmin, tileseg, total;
+  size_t tile_stride;  // tileseg entries per level
   size_t tmp_bytes;
   uint32_t t0;
 };
@@ -777,6 +768,8 @@ BuildLayout build_layout(size_t n) {
   L.tmp = take(tb);
   L.starts = take(nstart * 4);
   L.segmin = take(nmin * 8);
+  L.tile_stride = (n + kLevelTile - 1) / kLevelTile;
+  L.tileseg = take(static_cast<size_t>(L.t0) * L.tile_stride * 4);
   L.total = o;
   return L;
 }
@@ -890,6 +883,24 @@ Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, 
   uint32_t* starts_d = reinterpret_cast<uint32_t*>(base + BL.starts);
   PST_TRY(hipMemcpyAsync(starts_d, hstarts.data(), nstart * 4, hipMemcpyHostToDevice, s),
           "Error in copying level tables");
+  // per level t < t0 and tile of kLevelTile points: the segment holding the
+  // tile's first point (so that no workgroup searches the starts itself)
+  uint32_t* tileseg_d = reinterpret_cast<uint32_t*>(base + BL.tileseg);
+  if (t0 > 0) {
+    std::vector<uint32_t> htile(static_cast<size_t>(t0) * BL.tile_stride, 0u);
+    for (uint32_t t = 0; t < t0; t++) {
+      const uint32_t* lv = hstarts.data() + (1u << t) - 1;
+      const uint32_t n_level = n - ((1u << t) - 1);
+      const uint32_t tiles = (n_level + kLevelTile - 1) / kLevelTile;
+      for (uint32_t b = 0; b < tiles; b++) {
+        const uint32_t pos = b * kLevelTile;
+        htile[static_cast<size_t>(t) * BL.tile_stride + b] =
+            static_cast<uint32_t>(std::upper_bound(lv, lv + (1u << t), pos) - lv) - 1u;
+      }
+    }
+    PST_TRY(hipMemcpyAsync(tileseg_d, htile.data(), htile.size() * 4, hipMemcpyHostToDevice, s),
+            "Error in copying level tables");
+  }
   const size_t nmin = t0 ? (1ull << t0) - 1 : 1;
   unsigned long long* segmin = reinterpret_cast<unsigned long long*>(base + BL.segmin);
   k_fill_u64<<<grid_for(nmin, B), B, 0, s>>>(segmin, nmin, ~0ull);
@@ -901,12 +912,13 @@ Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, 
   // kLevelTile points meets at most kLevelTile / kFinishTile + 2 segments,
   // twice as many children.
   if (t0 > 0)
-    k_level_argmin<<<grid_for(n, kLevelTile), kLevelBlock, 0, s>>>(Acur, n, 1u, starts, segmin);
+    k_level_argmin<<<grid_for(n, kLevelTile), kLevelBlock, 0, s>>>(Acur, n, 1u, starts, tileseg_d, segmin);
   for (uint32_t t = 0; t < t0; t++) {
     const uint32_t n_level = n - ((1u << t) - 1);
     const unsigned tiles = grid_for(n_level, kLevelTile);
     unsigned long long* nextmin = t + 1 < t0 ? segmin + (2u << t) - 1 : nullptr;
     k_level_emit<T, IS><<<tiles, kLevelBlock, 0, s>>>(Acur, Anxt, n_level, t, M, starts + (1u << t) - 1,
+                                                      tileseg_d + static_cast<size_t>(t) * BL.tile_stride,
                                                       segmin + (1u << t) - 1, nextmin, perm2, ptsS, nodes);
     std::swap(Acur, Anxt);
   }
