@@ -23,6 +23,7 @@
 // pst/static-pst-cpu-iter.tpp:488-598); the tree is bit-identical to the
 // reference's CPU iterative tree on the same input.
 #include <algorithm>
+#include <functional>
 #include <vector>
 
 #include "pst_internal.hpp"
@@ -116,6 +117,41 @@ __global__ void k_scatter_rank(const uint32_t* __restrict__ perm2, uint32_t n, u
   if (p < n) r2[perm2[p]] = p;
 }
 
+// Segment starts of levels 0..t0, slot-indexed (level t at offset 2^t - 1):
+// the closed form of pst_common.hpp, clamped at 0.
+__global__ void k_level_starts(uint64_t M, uint32_t nstart, uint32_t* __restrict__ starts) {
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nstart) return;
+  const uint32_t t = depth_of(idx);
+  const uint32_t j = idx + 1u - (1u << t);
+  const int64_t v = level_start(M, t, j);
+  starts[idx] = static_cast<uint32_t>(v > 0 ? v : 0);
+}
+
+// Per level t < t0 and tile b of kLevelTile points: the segment holding the
+// tile's first point, i.e. the last j with starts[j] <= b * kLevelTile.
+__global__ void k_tile_segments(const uint32_t* __restrict__ starts, uint32_t n, uint32_t t0, uint32_t stride,
+                                uint32_t* __restrict__ tileseg) {
+  const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= static_cast<uint64_t>(t0) * stride) return;
+  const uint32_t t = static_cast<uint32_t>(g / stride), b = static_cast<uint32_t>(g % stride);
+  const uint32_t n_level = n - ((1u << t) - 1u);
+  const uint32_t tiles = (n_level + kLevelTile - 1) / kLevelTile;
+  uint32_t seg = 0;
+  if (b < tiles) {
+    const uint32_t* lv = starts + (1u << t) - 1u;
+    const uint32_t pos = b * kLevelTile;
+    uint32_t lo = 0, hi = 1u << t;  // lv[lo] <= pos (lv[0] = 0); answer in [lo, hi)
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (lv[mid] <= pos) lo = mid;
+      else hi = mid;
+    }
+    seg = lo;
+  }
+  tileseg[g] = seg;
+}
+
 __global__ void k_fill_u64(unsigned long long* p, size_t count, unsigned long long v) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < count) p[i] = v;
@@ -125,39 +161,112 @@ __global__ void k_fill_u64(unsigned long long* p, size_t count, unsigned long lo
 // Level passes over A (all segments of level t are non-empty here).
 // ---------------------------------------------------------------------------
 
-// Per segment: min over (rank << 32 | local position), i.e. the max-key2
-// point and where it sits in the segment.
+// Segments met by one tile of kLevelTile points: every segment of a level
+// walked by these passes holds at least kFinishTile points.
+constexpr uint32_t kSegSlots = kLevelTile / kFinishTile + 2;
+
+// The tile's elements, each thread's kLevelElems loaded before any is used
+// (position tile + threadIdx.x + e * kLevelBlock).
+__device__ inline void load_tile(const uint32_t* __restrict__ A, uint32_t n_level, uint32_t tile,
+                                 uint32_t (&av)[kLevelElems]) {
+  const uint32_t* p = A + tile + threadIdx.x;
+  if (tile + kLevelTile <= n_level) {
+#pragma unroll
+    for (int e = 0; e < kLevelElems; e++) av[e] = p[e * kLevelBlock];
+  } else {
+#pragma unroll
+    for (int e = 0; e < kLevelElems; e++)
+      if (tile + threadIdx.x + e * kLevelBlock < n_level) av[e] = p[e * kLevelBlock];
+  }
+}
+
+// The starts of the tile's segments seg0 + k, k <= kSegSlots (n_level past
+// the level's last segment), into LDS. Call before a barrier.
+__device__ inline void load_seg_starts(const uint32_t* __restrict__ starts, uint32_t seg0, uint32_t nseg,
+                                       uint32_t n_level, uint32_t* sst) {
+  if (threadIdx.x <= kSegSlots) {
+    const uint32_t j = seg0 + threadIdx.x;
+    sst[threadIdx.x] = j < nseg ? starts[j] : n_level;
+  }
+}
+
+// Per tile and segment met: min over (rank << 32 | local position), i.e.
+// the max-key2 point and where it sits in the segment, into tmin (per tile
+// kSegSlots entries, combined per segment by k_tile_min_reduce).
 __global__ __launch_bounds__(kLevelBlock) void k_level_argmin(
     const uint32_t* __restrict__ A, uint32_t n_level, uint32_t nseg,
     const uint32_t* __restrict__ starts, const uint32_t* __restrict__ tileseg,
-    unsigned long long* __restrict__ segmin) {
-  __shared__ unsigned long long smin[8];
+    unsigned long long* __restrict__ tmin) {
+  __shared__ unsigned long long smin[kSegSlots];
+  __shared__ uint32_t sst[kSegSlots + 1];
   const uint32_t tile = blockIdx.x * kLevelTile;
   const uint32_t seg0 = tileseg[blockIdx.x];  // the segment holding the tile's first point
-  if (threadIdx.x < 8) smin[threadIdx.x] = ~0ull;
+  uint32_t av[kLevelElems];
+  load_tile(A, n_level, tile, av);
+  if (threadIdx.x < kSegSlots) smin[threadIdx.x] = ~0ull;
+  load_seg_starts(starts, seg0, nseg, n_level, sst);
   __syncthreads();
   const uint32_t first = tile + threadIdx.x;
-  uint32_t j = seg0;
-  uint32_t next = (j + 1 < nseg) ? starts[j + 1] : n_level;
-  uint32_t st = starts[j];
+  uint32_t k = 0;
+  uint32_t st = sst[0], next = sst[1];
   unsigned long long best = ~0ull;
-#pragma unroll 4
+#pragma unroll
   for (int e = 0; e < kLevelElems; e++) {
     const uint32_t P = first + e * kLevelBlock;
-    if (P >= n_level) break;
-    while (P >= next) {
-      if (best != ~0ull) atomicMin(&smin[j - seg0], best);
-      best = ~0ull;
-      j++;
-      st = next;
-      next = (j + 1 < nseg) ? starts[j + 1] : n_level;
+    if (P < n_level) {
+      while (P >= next) {
+        if (best != ~0ull) atomicMin(&smin[k], best);
+        best = ~0ull;
+        k++;
+        st = next;
+        next = sst[k + 1];
+      }
+      const unsigned long long key = (static_cast<unsigned long long>(av[e]) << 32) | (P - st);
+      best = key < best ? key : best;
     }
-    const unsigned long long key = (static_cast<unsigned long long>(A[P]) << 32) | (P - st);
-    best = key < best ? key : best;
   }
-  if (best != ~0ull) atomicMin(&smin[j - seg0], best);
+  if (best != ~0ull) atomicMin(&smin[k], best);
   __syncthreads();
-  if (threadIdx.x < 8 && smin[threadIdx.x] != ~0ull) atomicMin(&segmin[seg0 + threadIdx.x], smin[threadIdx.x]);
+  if (threadIdx.x < kSegSlots) tmin[static_cast<size_t>(blockIdx.x) * kSegSlots + threadIdx.x] = smin[threadIdx.x];
+}
+
+// out[i] for i < gridDim.x (one workgroup each): the minimum of slot
+// i - (tileseg[b] << sh) of the per-tile minima of every tile b that meets
+// segment i >> sh of the level (slots: entries per tile). For the levels
+// with few segments, where the tiles' global atomics would queue on a few
+// addresses.
+__global__ __launch_bounds__(kLevelBlock) void k_tile_min_reduce(
+    const unsigned long long* __restrict__ tmin, uint32_t slots, const uint32_t* __restrict__ tileseg,
+    const uint32_t* __restrict__ starts, uint32_t nseg, uint32_t n_level, uint32_t sh,
+    unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long wbest[kLevelBlock / kWave];
+  const uint32_t i = blockIdx.x;
+  const uint32_t j = i >> sh;
+  const uint32_t lo = starts[j];
+  const uint32_t hi = j + 1 < nseg ? starts[j + 1] : n_level;
+  unsigned long long best = ~0ull;
+  if (hi > lo) {
+    const uint32_t b0 = lo / kLevelTile, b1 = (hi - 1) / kLevelTile;
+    for (uint32_t b = b0 + threadIdx.x; b <= b1; b += kLevelBlock) {
+      const uint32_t slot = i - (tileseg[b] << sh);
+      if (slot < slots) {
+        const unsigned long long v = tmin[static_cast<size_t>(b) * slots + slot];
+        best = v < best ? v : best;
+      }
+    }
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(best, off, kWave);
+    best = o < best ? o : best;
+  }
+  if (lane_id() == 0) wbest[threadIdx.x / kWave] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = wbest[0];
+    for (int w = 1; w < kLevelBlock / kWave; w++) m = wbest[w] < m ? wbest[w] : m;
+    out[i] = m;
+  }
 }
 
 template <typename T, typename IS>
@@ -176,72 +285,89 @@ __device__ inline void write_node(Node<T, IS>* nodes, uint64_t slot, const Point
 // finds the next level's minima: x' < floor(m_j / 2) lands in child 2j at
 // position x', the rest in child 2j + 1 at x' - floor(m_j / 2), and the
 // minimum of (rank << 32 | child position) per child goes to nextmin.
+// The tile's elements are loaded first and its segments' starts, minima and
+// halves staged in LDS, so that the pass streams A at full occupancy.
 template <typename T, typename IS>
 __global__ __launch_bounds__(kLevelBlock) void k_level_emit(
     const uint32_t* __restrict__ A, uint32_t* __restrict__ Anext, uint32_t n_level, uint32_t t,
     uint64_t M, const uint32_t* __restrict__ starts, const uint32_t* __restrict__ tileseg,
     const unsigned long long* __restrict__ segmin, unsigned long long* __restrict__ nextmin,
-    const uint32_t* __restrict__ perm2, const Point<T, IS>* __restrict__ pts, Node<T, IS>* __restrict__ nodes) {
-  constexpr uint32_t kChildSlots = 2 * (kLevelTile / kFinishTile + 2);  // children per tile (bound at the launch)
+    unsigned long long* __restrict__ tmin, const uint32_t* __restrict__ perm2, const Point<T, IS>* __restrict__ pts,
+    Node<T, IS>* __restrict__ nodes) {
+  constexpr uint32_t kChildSlots = 2 * kSegSlots;  // children per tile
   const uint32_t nseg = 1u << t;
   __shared__ unsigned long long cmin[kChildSlots];
+  __shared__ unsigned long long smn[kSegSlots];
+  __shared__ uint32_t sst[kSegSlots + 1];
+  __shared__ uint32_t shalf[kSegSlots];
   const uint32_t tile = blockIdx.x * kLevelTile;
   const uint32_t seg0 = tileseg[blockIdx.x];  // the segment holding the tile's first point
-  for (uint32_t i = threadIdx.x; i < kChildSlots; i += kLevelBlock) cmin[i] = ~0ull;
+  uint32_t av[kLevelElems];
+  load_tile(A, n_level, tile, av);
+  if (threadIdx.x < kChildSlots) cmin[threadIdx.x] = ~0ull;
+  load_seg_starts(starts, seg0, nseg, n_level, sst);
+  if (threadIdx.x < kSegSlots) {
+    const uint32_t j = seg0 + threadIdx.x;
+    smn[threadIdx.x] = j < nseg ? segmin[j] : ~0ull;
+    shalf[threadIdx.x] = j < nseg ? static_cast<uint32_t>(level_size(M, t, j)) >> 1 : 0u;
+  }
   __syncthreads();
   const uint32_t c0 = 2 * seg0;  // the tile's first child segment
   const uint32_t first = tile + threadIdx.x;
-  uint32_t j = seg0;
-  uint32_t st = starts[j];
-  uint32_t next = (j + 1 < nseg) ? starts[j + 1] : n_level;
-  unsigned long long mn = segmin[j];
-  uint32_t half = static_cast<uint32_t>(level_size(M, t, j)) >> 1;
+  uint32_t k = 0;
+  uint32_t st = sst[0], next = sst[1];
+  uint32_t q = static_cast<uint32_t>(smn[0]);
+  uint32_t half = shalf[0];
   unsigned long long best = ~0ull;  // running minimum of child bc
-  uint32_t bc = c0;
-#pragma unroll 4
+  uint32_t bc = 0;
+#pragma unroll
   for (int e = 0; e < kLevelElems; e++) {
     const uint32_t P = first + e * kLevelBlock;
-    if (P >= n_level) break;
-    while (P >= next) {
-      j++;
-      st = next;
-      next = (j + 1 < nseg) ? starts[j + 1] : n_level;
-      mn = segmin[j];
-      half = static_cast<uint32_t>(level_size(M, t, j)) >> 1;
-    }
-    const uint32_t x = P - st;
-    const uint32_t q = static_cast<uint32_t>(mn);
-    const uint32_t a = A[P];
-    if (x == q) {
-      const uint32_t L = half;
-      const Point<T, IS> p = pts[perm2[static_cast<uint32_t>(mn >> 32)]];
-      T med = p.dim1_val;
-      if (L >= 1) {
-        const uint32_t pm = q < L ? L : L - 1;
-        med = pts[perm2[A[st + pm]]].dim1_val;
+    if (P < n_level) {
+      while (P >= next) {
+        k++;
+        st = next;
+        next = sst[k + 1];
+        q = static_cast<uint32_t>(smn[k]);
+        half = shalf[k];
       }
-      write_node(nodes, (1ull << t) - 1 + j, p, med);
-    } else {
-      const uint32_t xr = x - (x > q ? 1u : 0u);
-      Anext[st - j + xr] = a;
-      if (nextmin) {
-        const bool right = xr >= half;
-        const uint32_t c = 2 * j + (right ? 1u : 0u);
-        const unsigned long long key = (static_cast<unsigned long long>(a) << 32) | (right ? xr - half : xr);
-        if (c != bc) {
-          if (best != ~0ull) atomicMin(&cmin[bc - c0], best);
-          best = ~0ull;
-          bc = c;
+      const uint32_t x = P - st;
+      const uint32_t a = av[e];
+      const uint32_t j = seg0 + k;
+      if (x == q) {
+        const uint32_t L = half;
+        const Point<T, IS> p = pts[perm2[a]];
+        T med = p.dim1_val;
+        if (L >= 1) {
+          const uint32_t pm = q < L ? L : L - 1;
+          med = pts[perm2[A[st + pm]]].dim1_val;
         }
-        best = key < best ? key : best;
+        write_node(nodes, (1ull << t) - 1 + j, p, med);
+      } else {
+        const uint32_t xr = x - (x > q ? 1u : 0u);
+        Anext[st - j + xr] = a;
+        if (nextmin) {
+          const bool right = xr >= half;
+          const uint32_t c = 2 * k + (right ? 1u : 0u);
+          const unsigned long long key = (static_cast<unsigned long long>(a) << 32) | (right ? xr - half : xr);
+          if (c != bc) {
+            if (best != ~0ull) atomicMin(&cmin[bc], best);
+            best = ~0ull;
+            bc = c;
+          }
+          best = key < best ? key : best;
+        }
       }
     }
   }
   if (!nextmin) return;
-  if (best != ~0ull) atomicMin(&cmin[bc - c0], best);
+  if (best != ~0ull) atomicMin(&cmin[bc], best);
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < kChildSlots; i += kLevelBlock)
-    if (cmin[i] != ~0ull) atomicMin(&nextmin[c0 + i], cmin[i]);
+  if (tmin) {  // per-tile minima, reduced by k_tile_min_reduce
+    if (threadIdx.x < kChildSlots) tmin[static_cast<size_t>(blockIdx.x) * kChildSlots + threadIdx.x] = cmin[threadIdx.x];
+  } else if (threadIdx.x < kChildSlots && cmin[threadIdx.x] != ~0ull) {
+    atomicMin(&nextmin[c0 + threadIdx.x], cmin[threadIdx.x]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -721,7 +847,7 @@ inline unsigned grid_for(size_t n, unsigned block) { return static_cast<unsigned
 // arrays, four 32-bit arrays (sort values, the key2 order, the level arrays
 // and the key2 ranks), flags, radix sort storage, level tables.
 struct BuildLayout {
-  size_t key0, key1, v0, v1, v2, r2, flags, tmp, starts, segmin, tileseg, total;
+  size_t key0, key1, v0, v1, v2, r2, flags, tmp, starts, segmin, tileseg, tmin, total;
   size_t tile_stride;  // tileseg entries per level
   size_t tmp_bytes;
   uint32_t t0;
@@ -770,6 +896,7 @@ BuildLayout build_layout(size_t n) {
   L.segmin = take(nmin * 8);
   L.tile_stride = (n + kLevelTile - 1) / kLevelTile;
   L.tileseg = take(static_cast<size_t>(L.t0) * L.tile_stride * 4);
+  L.tmin = take(L.tile_stride * 2 * kSegSlots * 8);
   L.total = o;
   return L;
 }
@@ -780,7 +907,8 @@ using BuildKey = typename std::conditional<(sizeof(T) == 8 || (has_id_v<I> && si
                                            uint32_t>::type;
 
 template <typename T, typename I>
-Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, BuildTimes* times) {
+Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, BuildTimes* times,
+                   const std::function<Status()>* then) {
   using IS = typename IdStore<I>::type;
   using K = BuildKey<T, I>;
   Status st;
@@ -822,12 +950,28 @@ Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, 
   uint32_t* r2 = reinterpret_cast<uint32_t*>(base + BL.r2);
   int* flags = reinterpret_cast<int*>(base + BL.flags);
   void* tmp = base + BL.tmp;
-  PST_TRY(hipMemsetAsync(flags, 0, 2 * sizeof(int), s), "Error in clearing construction flags");
+  // flags[0]: NaN seen; flags[1]: IDs not in index order; flags[2]: a sort's wait ran out
+  PST_TRY(hipMemsetAsync(flags, 0, 3 * sizeof(int), s), "Error in clearing construction flags");
+  // level tables, which depend on n alone: segment starts of levels 0..t0
+  // and, per level t < t0 and tile of kLevelTile points, the segment
+  // holding the tile's first point (so that no workgroup searches the
+  // starts itself)
+  {
+    const uint32_t nstart = (2u << BL.t0) - 1u;
+    uint32_t* st_d = reinterpret_cast<uint32_t*>(base + BL.starts);
+    k_level_starts<<<grid_for(nstart, 256), 256, 0, s>>>(n64 + 1, nstart, st_d);
+    if (BL.t0 > 0) {
+      const uint64_t cells = static_cast<uint64_t>(BL.t0) * BL.tile_stride;
+      k_tile_segments<<<grid_for(cells, 256), 256, 0, s>>>(st_d, static_cast<uint32_t>(n64), BL.t0,
+                                                           static_cast<uint32_t>(BL.tile_stride),
+                                                           reinterpret_cast<uint32_t*>(base + BL.tileseg));
+    }
+  }
   // stable radix sort of (key, value) pairs: kin/vin to kout/vout
   auto sortK = [&](const K* kin, K* kout, const uint32_t* vin, uint32_t* vout) -> hipError_t {
     size_t tb = BL.tmp_bytes;
-    if constexpr (sizeof(K) == 4) return radix_sort_u32(tmp, tb, kin, kout, vin, vout, n, s);
-    else return radix_sort_u64(tmp, tb, kin, kout, vin, vout, n, s);
+    if constexpr (sizeof(K) == 4) return radix_sort_u32(tmp, tb, kin, kout, vin, vout, n, s, flags + 2);
+    else return radix_sort_u64(tmp, tb, kin, kout, vin, vout, n, s, flags + 2);
   };
   const unsigned B = 256;
   const unsigned G = grid_for(n, B);
@@ -872,35 +1016,10 @@ Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, 
   uint32_t* Acur = v2;
   uint32_t* Anxt = v0;
 
-  // ---- level schedule
+  // ---- level schedule (tables computed above)
   const uint32_t t0 = BL.t0;
-  // segment starts of levels 0..t0, slot-indexed (level t at offset 2^t - 1)
-  const size_t nstart = (1ull << (t0 + 1)) - 1;
-  std::vector<uint32_t> hstarts(nstart);
-  for (uint32_t t = 0; t <= t0; t++)
-    for (uint32_t j = 0; j < (1u << t); j++)
-      hstarts[(1u << t) - 1 + j] = static_cast<uint32_t>(std::max<int64_t>(0, level_start(M, t, j)));
   uint32_t* starts_d = reinterpret_cast<uint32_t*>(base + BL.starts);
-  PST_TRY(hipMemcpyAsync(starts_d, hstarts.data(), nstart * 4, hipMemcpyHostToDevice, s),
-          "Error in copying level tables");
-  // per level t < t0 and tile of kLevelTile points: the segment holding the
-  // tile's first point (so that no workgroup searches the starts itself)
   uint32_t* tileseg_d = reinterpret_cast<uint32_t*>(base + BL.tileseg);
-  if (t0 > 0) {
-    std::vector<uint32_t> htile(static_cast<size_t>(t0) * BL.tile_stride, 0u);
-    for (uint32_t t = 0; t < t0; t++) {
-      const uint32_t* lv = hstarts.data() + (1u << t) - 1;
-      const uint32_t n_level = n - ((1u << t) - 1);
-      const uint32_t tiles = (n_level + kLevelTile - 1) / kLevelTile;
-      for (uint32_t b = 0; b < tiles; b++) {
-        const uint32_t pos = b * kLevelTile;
-        htile[static_cast<size_t>(t) * BL.tile_stride + b] =
-            static_cast<uint32_t>(std::upper_bound(lv, lv + (1u << t), pos) - lv) - 1u;
-      }
-    }
-    PST_TRY(hipMemcpyAsync(tileseg_d, htile.data(), htile.size() * 4, hipMemcpyHostToDevice, s),
-            "Error in copying level tables");
-  }
   const size_t nmin = t0 ? (1ull << t0) - 1 : 1;
   unsigned long long* segmin = reinterpret_cast<unsigned long long*>(base + BL.segmin);
   k_fill_u64<<<grid_for(nmin, B), B, 0, s>>>(segmin, nmin, ~0ull);
@@ -911,23 +1030,43 @@ Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, 
   // kFinishTile points, so all hold at least kFinishTile: a tile of
   // kLevelTile points meets at most kLevelTile / kFinishTile + 2 segments,
   // twice as many children.
-  if (t0 > 0)
-    k_level_argmin<<<grid_for(n, kLevelTile), kLevelBlock, 0, s>>>(Acur, n, 1u, starts, tileseg_d, segmin);
+  // Levels with at most kReduceSegs segments combine the tiles' minima by a
+  // reduction instead of global atomics.
+  constexpr uint32_t kReduceSegs = 32;
+  unsigned long long* tmin = reinterpret_cast<unsigned long long*>(base + BL.tmin);
+  if (t0 > 0) {
+    k_level_argmin<<<grid_for(n, kLevelTile), kLevelBlock, 0, s>>>(Acur, n, 1u, starts, tileseg_d, tmin);
+    k_tile_min_reduce<<<1, kLevelBlock, 0, s>>>(tmin, kSegSlots, tileseg_d, starts, 1u, n, 0u, segmin);
+  }
   for (uint32_t t = 0; t < t0; t++) {
     const uint32_t n_level = n - ((1u << t) - 1);
     const unsigned tiles = grid_for(n_level, kLevelTile);
     unsigned long long* nextmin = t + 1 < t0 ? segmin + (2u << t) - 1 : nullptr;
-    k_level_emit<T, IS><<<tiles, kLevelBlock, 0, s>>>(Acur, Anxt, n_level, t, M, starts + (1u << t) - 1,
-                                                      tileseg_d + static_cast<size_t>(t) * BL.tile_stride,
-                                                      segmin + (1u << t) - 1, nextmin, perm2, ptsS, nodes);
+    const bool reduce = nextmin && (1u << t) <= kReduceSegs;
+    const uint32_t* st_t = starts + (1u << t) - 1;
+    const uint32_t* ts_t = tileseg_d + static_cast<size_t>(t) * BL.tile_stride;
+    k_level_emit<T, IS><<<tiles, kLevelBlock, 0, s>>>(Acur, Anxt, n_level, t, M, st_t, ts_t, segmin + (1u << t) - 1,
+                                                      nextmin, reduce ? tmin : nullptr, perm2, ptsS, nodes);
+    if (reduce)
+      k_tile_min_reduce<<<2u << t, kLevelBlock, 0, s>>>(tmin, 2 * kSegSlots, ts_t, st_t, 1u << t, n_level, 1u,
+                                                        nextmin);
     std::swap(Acur, Anxt);
   }
   mark(2);
   k_finish<T, IS><<<1u << t0, kFinishBlock, 0, s>>>(Acur, t0, M, starts + (1u << t0) - 1, perm2, ptsS, nodes);
   mark(3);
   PST_TRY(hipGetLastError(), "Error in launching construction kernels");
-  int hflag = 0;
-  PST_TRY(hipMemcpyAsync(&hflag, nan_flag, sizeof(int), hipMemcpyDeviceToHost, s),
+  // the caller's work on the finished nodes goes into the stream before the
+  // one synchronisation (and before this function's buffers are freed)
+  if (then) {
+    Status st2 = (*then)();
+    if (!st2.ok()) {
+      (void)hipStreamSynchronize(s);
+      return st2;
+    }
+  }
+  int hflag[3] = {0, 0, 0};
+  PST_TRY(hipMemcpyAsync(hflag, flags, sizeof(hflag), hipMemcpyDeviceToHost, s),
           "Error in reading construction flag");
   PST_TRY(hipStreamSynchronize(s), "Error in synchronising after construction");
   if (times) {
@@ -936,7 +1075,10 @@ Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, 
     hipEventElapsedTime(&times->finish_ms, ev[2], ev[3]);
     for (auto& x : ev) hipEventDestroy(x);
   }
-  if (hflag) {
+  if (hflag[2]) {
+    st.code = 5;
+    st.msg = "Error: a construction sort pass waited too long for earlier tiles";
+  } else if (hflag[0]) {
     st.code = 3;
     st.msg = "Error: input contains NaN values, which have no order";
   }
@@ -1028,10 +1170,10 @@ Status with_id(int idt, F&& f) {
 }  // namespace
 
 Status build_tree(int dtype, int id_dtype, const void* pts_d, size_t n, void* nodes_d, hipStream_t stream,
-                  BuildTimes* times) {
+                  BuildTimes* times, const std::function<Status()>* then) {
   return with_t(dtype, [&](auto t) {
     return with_id(id_dtype, [&](auto i) {
-      return build_typed<decltype(t), decltype(i)>(pts_d, n, nodes_d, stream, times);
+      return build_typed<decltype(t), decltype(i)>(pts_d, n, nodes_d, stream, times, then);
     });
   });
 }

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <functional>
 #include <string>
 
 #include "pst_internal.hpp"
@@ -358,9 +359,10 @@ int pst_gpu_create(const void* pt_arr_d, size_t n, int dtype, int id_dtype, int 
     pst_gpu_destroy(t);
     return fail(PST_ERR_HIP, m);
   }
-  Status st = pst::build_tree(dtype, id_dtype, pt_arr_d, n, t->nodes, t->stream, nullptr);
-  if (st.ok()) st = pst::build_submin(dtype, id_dtype, n, 0, t->nodes, t->submin, t->ids, t->stream);
-  if (st.ok()) st = pst::hip_status(hipStreamSynchronize(t->stream), "Error in synchronising after construction");
+  const std::function<pst::Status()> submin = [&] {
+    return pst::build_submin(dtype, id_dtype, n, 0, t->nodes, t->submin, t->ids, t->stream);
+  };
+  Status st = pst::build_tree(dtype, id_dtype, pt_arr_d, n, t->nodes, t->stream, nullptr, &submin);
   if (!st.ok()) {
     pst_gpu_destroy(t);
     return fail(st);

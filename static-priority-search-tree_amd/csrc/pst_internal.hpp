@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <string>
 
 #include "pst_common.hpp"
@@ -12,10 +13,12 @@
 namespace pst {
 
 // ---- sorting (pst_sort.hip) ----
+// `fail` (device int, optional) is set to 1 if a pass's bounded wait for
+// earlier tiles ran out, which leaves the output undefined.
 hipError_t radix_sort_u32(void* tmp, size_t& tmp_bytes, const uint32_t* kin, uint32_t* kout,
-                          const uint32_t* vin, uint32_t* vout, size_t n, hipStream_t s);
+                          const uint32_t* vin, uint32_t* vout, size_t n, hipStream_t s, int* fail = nullptr);
 hipError_t radix_sort_u64(void* tmp, size_t& tmp_bytes, const uint64_t* kin, uint64_t* kout,
-                          const uint32_t* vin, uint32_t* vout, size_t n, hipStream_t s);
+                          const uint32_t* vin, uint32_t* vout, size_t n, hipStream_t s, int* fail = nullptr);
 
 // Error carrier: a HIP status plus the reference-style context message
 // ("Error in ... on device X (1-indexed) of Y: " + hipGetErrorString).
@@ -36,9 +39,11 @@ struct BuildTimes {
 };
 // Builds the tree for n AoS points (device pointer) into `nodes`, which
 // holds S = 2^ceil(log2(n+1)) - 1 records. Returns PST_ERR_NAN if a NaN dim
-// (or floating ID) is found.
+// (or floating ID) is found. `then`, if given, is called once the
+// construction kernels are enqueued, to enqueue work on the nodes before the
+// single synchronisation that ends the construction.
 Status build_tree(int dtype, int id_dtype, const void* pts_d, size_t n, void* nodes_d, hipStream_t stream,
-                  BuildTimes* times);
+                  BuildTimes* times, const std::function<Status()>* then = nullptr);
 
 // Device bytes one construction of n points allocates (freed when it ends).
 size_t build_scratch_bytes(int dtype, int id_dtype, size_t n);

@@ -14,9 +14,9 @@
 
 namespace pst {
 hipError_t radix_sort_u32(void* tmp, size_t& tmp_bytes, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
-                          uint32_t* vout, size_t n, hipStream_t s);
+                          uint32_t* vout, size_t n, hipStream_t s, int* fail);
 hipError_t radix_sort_u64(void* tmp, size_t& tmp_bytes, const uint64_t* kin, uint64_t* kout, const uint32_t* vin,
-                          uint32_t* vout, size_t n, hipStream_t s);
+                          uint32_t* vout, size_t n, hipStream_t s, int* fail);
 }  // namespace pst
 
 #define CHECK(x)                                                                      \
@@ -54,12 +54,20 @@ int run_case(size_t n, int dist, uint64_t seed) {
   CHECK(hipMemcpy(vin, vals.data(), n * 4, hipMemcpyHostToDevice));
   size_t tb = 0;
   void* tmp = nullptr;
-  if constexpr (sizeof(K) == 4) CHECK(pst::radix_sort_u32(nullptr, tb, nullptr, nullptr, nullptr, nullptr, n, 0));
-  else CHECK(pst::radix_sort_u64(nullptr, tb, nullptr, nullptr, nullptr, nullptr, n, 0));
+  int* fail;
+  CHECK(hipMalloc(&fail, sizeof(int)));
+  CHECK(hipMemset(fail, 0, sizeof(int)));
+  if constexpr (sizeof(K) == 4) CHECK(pst::radix_sort_u32(nullptr, tb, nullptr, nullptr, nullptr, nullptr, n, 0, nullptr));
+  else CHECK(pst::radix_sort_u64(nullptr, tb, nullptr, nullptr, nullptr, nullptr, n, 0, nullptr));
   CHECK(hipMalloc(&tmp, tb));
-  if constexpr (sizeof(K) == 4) CHECK(pst::radix_sort_u32(tmp, tb, kin, kout, vin, vout, n, 0));
-  else CHECK(pst::radix_sort_u64(tmp, tb, kin, kout, vin, vout, n, 0));
+  // twice with the same scratch: the second sort must not see the first one's state
+  for (int rep = 0; rep < 2; rep++) {
+    if constexpr (sizeof(K) == 4) CHECK(pst::radix_sort_u32(tmp, tb, kin, kout, vin, vout, n, 0, fail));
+    else CHECK(pst::radix_sort_u64(tmp, tb, kin, kout, vin, vout, n, 0, fail));
+  }
   CHECK(hipDeviceSynchronize());
+  int hfail = 0;
+  CHECK(hipMemcpy(&hfail, fail, sizeof(int), hipMemcpyDeviceToHost));
   std::vector<K> gk(n), gin(n);
   std::vector<uint32_t> gv(n);
   CHECK(hipMemcpy(gk.data(), kout, n * sizeof(K), hipMemcpyDeviceToHost));
@@ -69,9 +77,10 @@ int run_case(size_t n, int dist, uint64_t seed) {
   for (size_t i = 0; i < n && !bad; i++)
     if (gv[i] != order[i] || gk[i] != keys[order[i]]) bad = 1;
   if (gin != keys) bad = 2;
+  if (hfail) bad = 3;
   std::printf("sort u%zu n=%zu dist=%d: %s\n", sizeof(K) * 8, n, dist,
-              bad == 0 ? "ok" : (bad == 1 ? "MISMATCH" : "INPUT CHANGED"));
-  (void)hipFree(kin), (void)hipFree(kout), (void)hipFree(vin), (void)hipFree(vout), (void)hipFree(tmp);
+              bad == 0 ? "ok" : (bad == 1 ? "MISMATCH" : (bad == 2 ? "INPUT CHANGED" : "WAIT RAN OUT")));
+  (void)hipFree(kin), (void)hipFree(kout), (void)hipFree(vin), (void)hipFree(vout), (void)hipFree(tmp), (void)hipFree(fail);
   return bad;
 }
 
