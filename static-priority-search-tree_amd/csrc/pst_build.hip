@@ -23,7 +23,6 @@
 // pst/static-pst-cpu-iter.tpp:488-598); the tree is bit-identical to the
 // reference's CPU iterative tree on the same input.
 #include <algorithm>
-#include <functional>
 #include <vector>
 
 #include "pst_internal.hpp"
@@ -553,16 +552,26 @@ __device__ void lds_build_local(uint16_t* bufA, uint16_t* bufB, const uint32_t* 
   }
 }
 
+// One workgroup per level-t0 segment: its whole subtree in LDS, then every
+// node record; with `submin` also the subtree minima of dim2 and the IDs of
+// the subtree's slots (the bottom levels of what k_submin_level computes).
 template <typename T, typename IS>
 __global__ __launch_bounds__(kFinishBlock) void k_finish(
     const uint32_t* __restrict__ A, uint32_t t0, uint64_t M, const uint32_t* __restrict__ starts,
-    const uint32_t* __restrict__ perm2, const Point<T, IS>* __restrict__ pts, Node<T, IS>* __restrict__ nodes) {
-  __shared__ uint16_t bufA[kFinishTile];
-  __shared__ uint16_t bufB[kFinishTile];
+    const uint32_t* __restrict__ perm2, const Point<T, IS>* __restrict__ pts, Node<T, IS>* __restrict__ nodes,
+    T* __restrict__ submin, IS* __restrict__ ids) {
+  union FinishLds {
+    struct {
+      uint16_t bufA[kFinishTile];
+      uint16_t bufB[kFinishTile];
+      uint16_t sst[2 * kFinishTile];  // segment start per subtree slot
+    } b;
+    T d2[2 * kFinishTile];  // after the build: dim2 per subtree slot, then its subtree minimum
+  };
+  __shared__ FinishLds U;
   __shared__ uint32_t rank[kFinishTile];
   __shared__ uint16_t slot_of[kFinishTile];  // the subtree slot (heap order) each point fills
   __shared__ uint16_t med_of[kFinishTile];   // and the point whose dim1 is its med
-  __shared__ uint16_t sst[2 * kFinishTile];  // segment start per subtree slot
   __shared__ unsigned long long sres[kFinishBlock];  // one per segment of a level with <= kFinishBlock segments
   __shared__ unsigned long long wred[kFinishBlock / kWave];
   const uint32_t j = blockIdx.x;
@@ -576,12 +585,12 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish(
     const uint32_t x = threadIdx.x + e * kFinishBlock;
     if (x < m) {
       rank[x] = A[st + x];
-      bufA[x] = static_cast<uint16_t>(x);
+      U.b.bufA[x] = static_cast<uint16_t>(x);
     }
   }
-  if (threadIdx.x == 0) sst[0] = 0;
+  if (threadIdx.x == 0) U.b.sst[0] = 0;
   __syncthreads();
-  lds_build_local<T, IS>(bufA, bufB, rank, slot_of, med_of, sst, sres, wred, static_cast<uint32_t>(m));
+  lds_build_local<T, IS>(U.b.bufA, U.b.bufB, rank, slot_of, med_of, U.b.sst, sres, wred, static_cast<uint32_t>(m));
   // every point's node record (rank -> input index -> record; all loads in
   // flight together): subtree slot h (level u, position k) is slot
   // ((root + 1) << u) - 1 + k of the tree
@@ -604,14 +613,48 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish(
       med[e] = pts[im[e]].dim1_val;
     }
   }
+  __syncthreads();  // the build's LDS arrays are free from here
 #pragma unroll
   for (int e = 0; e < kPer; e++) {
     const uint32_t x = threadIdx.x + e * kFinishBlock;
     if (x < m) {
       const uint32_t h = slot_of[x] + 1u;
       const uint32_t u = 31u - static_cast<uint32_t>(__builtin_clz(h));
-      write_node(nodes, ((root + 1) << u) - 1 + (h - (1u << u)), p[e], med[e]);
+      const uint64_t slot = ((root + 1) << u) - 1 + (h - (1u << u));
+      write_node(nodes, slot, p[e], med[e]);
+      if (submin) {
+        U.d2[h - 1] = p[e].dim2_val;
+        if constexpr (has_id_v<IS>) {
+          if (ids) ids[slot] = p[e].id;
+        }
+      }
     }
+  }
+  if (!submin) return;
+  // subtree minima, bottom level first: min(dim2(s), minima of the children
+  // that exist); a slot of size ms has a left child if ms >= 2, a right one
+  // if ms >= 3
+  __syncthreads();
+  const uint64_t Mp = static_cast<uint64_t>(m) + 1;
+  const int depth = 32 - __builtin_clz(static_cast<uint32_t>(m));
+  for (int u = depth - 1; u >= 0; u--) {
+    for (uint32_t k = threadIdx.x; k < (1u << u); k += kFinishBlock) {
+      const int64_t ms = level_size(Mp, static_cast<uint32_t>(u), k);
+      if (ms < 1) continue;
+      const uint32_t sl = (1u << u) - 1 + k;
+      T v = U.d2[sl];
+      if (ms >= 2) {
+        const T c = U.d2[2 * sl + 1];
+        v = c < v ? c : v;
+      }
+      if (ms >= 3) {
+        const T c = U.d2[2 * sl + 2];
+        v = c < v ? c : v;
+      }
+      U.d2[sl] = v;
+      submin[((root + 1) << u) - 1 + k] = v;
+    }
+    __syncthreads();
   }
 }
 
@@ -908,7 +951,7 @@ using BuildKey = typename std::conditional<(sizeof(T) == 8 || (has_id_v<I> && si
 
 template <typename T, typename I>
 Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, BuildTimes* times,
-                   const std::function<Status()>* then) {
+                   void* submin_v, void* ids_v) {
   using IS = typename IdStore<I>::type;
   using K = BuildKey<T, I>;
   Status st;
@@ -1053,18 +1096,16 @@ Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, 
     std::swap(Acur, Anxt);
   }
   mark(2);
-  k_finish<T, IS><<<1u << t0, kFinishBlock, 0, s>>>(Acur, t0, M, starts + (1u << t0) - 1, perm2, ptsS, nodes);
+  k_finish<T, IS><<<1u << t0, kFinishBlock, 0, s>>>(Acur, t0, M, starts + (1u << t0) - 1, perm2, ptsS, nodes,
+                                                    static_cast<T*>(submin_v), static_cast<IS*>(ids_v));
+  // subtree minima and IDs of the slots above the LDS subtrees
+  if (submin_v && t0 > 0) {
+    const uint32_t M32 = static_cast<uint32_t>(M);
+    Status st2 = submin_typed<T, IS>(nodes, submin_v, ids_v, t0, 31u, 1u, M32, M32, s);
+    if (!st2.ok()) return st2;
+  }
   mark(3);
   PST_TRY(hipGetLastError(), "Error in launching construction kernels");
-  // the caller's work on the finished nodes goes into the stream before the
-  // one synchronisation (and before this function's buffers are freed)
-  if (then) {
-    Status st2 = (*then)();
-    if (!st2.ok()) {
-      (void)hipStreamSynchronize(s);
-      return st2;
-    }
-  }
   int hflag[3] = {0, 0, 0};
   PST_TRY(hipMemcpyAsync(hflag, flags, sizeof(hflag), hipMemcpyDeviceToHost, s),
           "Error in reading construction flag");
@@ -1170,10 +1211,10 @@ Status with_id(int idt, F&& f) {
 }  // namespace
 
 Status build_tree(int dtype, int id_dtype, const void* pts_d, size_t n, void* nodes_d, hipStream_t stream,
-                  BuildTimes* times, const std::function<Status()>* then) {
+                  BuildTimes* times, void* submin_d, void* ids_d) {
   return with_t(dtype, [&](auto t) {
     return with_id(id_dtype, [&](auto i) {
-      return build_typed<decltype(t), decltype(i)>(pts_d, n, nodes_d, stream, times, then);
+      return build_typed<decltype(t), decltype(i)>(pts_d, n, nodes_d, stream, times, submin_d, ids_d);
     });
   });
 }

@@ -9,7 +9,6 @@
 #include <cstring>
 #include <map>
 #include <mutex>
-#include <functional>
 #include <string>
 
 #include "pst_internal.hpp"
@@ -359,10 +358,7 @@ int pst_gpu_create(const void* pt_arr_d, size_t n, int dtype, int id_dtype, int 
     pst_gpu_destroy(t);
     return fail(PST_ERR_HIP, m);
   }
-  const std::function<pst::Status()> submin = [&] {
-    return pst::build_submin(dtype, id_dtype, n, 0, t->nodes, t->submin, t->ids, t->stream);
-  };
-  Status st = pst::build_tree(dtype, id_dtype, pt_arr_d, n, t->nodes, t->stream, nullptr, &submin);
+  Status st = pst::build_tree(dtype, id_dtype, pt_arr_d, n, t->nodes, t->stream, nullptr, t->submin, t->ids);
   if (!st.ok()) {
     pst_gpu_destroy(t);
     return fail(st);

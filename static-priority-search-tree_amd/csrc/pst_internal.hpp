@@ -5,7 +5,6 @@
 
 #include <cstddef>
 #include <cstdint>
-#include <functional>
 #include <string>
 
 #include "pst_common.hpp"
@@ -39,11 +38,10 @@ struct BuildTimes {
 };
 // Builds the tree for n AoS points (device pointer) into `nodes`, which
 // holds S = 2^ceil(log2(n+1)) - 1 records. Returns PST_ERR_NAN if a NaN dim
-// (or floating ID) is found. `then`, if given, is called once the
-// construction kernels are enqueued, to enqueue work on the nodes before the
-// single synchronisation that ends the construction.
+// (or floating ID) is found. With `submin_d` (and `ids_d`, null for trees
+// without IDs) also fills what build_submin(.., slots_per_tree = 0, ..) does.
 Status build_tree(int dtype, int id_dtype, const void* pts_d, size_t n, void* nodes_d, hipStream_t stream,
-                  BuildTimes* times, const std::function<Status()>* then = nullptr);
+                  BuildTimes* times, void* submin_d = nullptr, void* ids_d = nullptr);
 
 // Device bytes one construction of n points allocates (freed when it ends).
 size_t build_scratch_bytes(int dtype, int id_dtype, size_t n);
