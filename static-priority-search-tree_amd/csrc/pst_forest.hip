@@ -193,6 +193,23 @@ __device__ inline void load_tree(const Node<T, IS>* __restrict__ tn, const Fores
 }
 
 
+// The records of the lanes in m, in lane order, at oq[w], oq[w + 1], ...
+// (positions at or past lim dropped); returns w + |m|. The mask is the
+// lanes' predicate as it stands; the per-lane limit test only runs when
+// the run crosses the limit.
+template <typename Rec>
+__device__ inline uint32_t emit_run(Rec* oq, uint32_t w, uint32_t lim, uint64_t m, const Rec& rec) {
+  const uint32_t at = w + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+  const uint32_t pop = static_cast<uint32_t>(__builtin_popcountll(m));
+  if (w + pop <= lim) {
+    if (__builtin_amdgcn_inverse_ballot_w64(m)) oq[at] = rec;
+  } else {
+    if (__builtin_amdgcn_inverse_ballot_w64(m) && at < lim) oq[at] = rec;
+  }
+  return w + pop;
+}
+
 // Pass 1: segcnt[q * nseg + seg] = hits of query q in segment seg. One
 // wave per segment. Per tree, lanes first take a query each (64 per pass)
 // to find the candidates: the queries that may hit the tree (its dim1 range
@@ -203,7 +220,9 @@ template <typename T, typename IS, int R>
 __global__ __launch_bounds__(kFBlock) void k_forest_count(const Node<T, IS>* __restrict__ nodes,
                                                           const Range1<T>* __restrict__ bounds, ForestArgs F,
                                                           const Query<T>* __restrict__ queries,
+                                                          const uint32_t* __restrict__ stab_flag,
                                                           uint32_t* __restrict__ segcnt) {
+  if (*stab_flag) return;  // a stabbing batch: k_stab_count
   __shared__ Box<T> sbox[kQSlots];
   __shared__ uint32_t cnt_all[kFWaves][kQSlots];
   stage_boxes(queries, F.nq, sbox);
@@ -311,11 +330,13 @@ __global__ __launch_bounds__(kFBlock) void k_forest_write(const Node<T, IS>* __r
                                                           const void* __restrict__ ids_v,
                                                           const Range1<T>* __restrict__ bounds, ForestArgs F,
                                                           const Query<T>* __restrict__ queries,
+                                                          const uint32_t* __restrict__ stab_flag,
                                                           const uint32_t* __restrict__ segoff,
                                                           const unsigned long long* __restrict__ offsets,
                                                           const unsigned long long* __restrict__ caps,
                                                           unsigned long long cap_all, unsigned long long out_total,
                                                           void* __restrict__ out) {
+  if (*stab_flag) return;  // a stabbing batch: k_stab_write
   using Rec = typename std::conditional<RP, Point<T, IS>, IS>::type;
   using ISt = typename std::conditional<has_id_v<IS>, IS, uint32_t>::type;
   const ISt* __restrict__ ids = static_cast<const ISt*>(ids_v);
@@ -383,7 +404,7 @@ __global__ __launch_bounds__(kFBlock) void k_forest_write(const Node<T, IS>* __r
       }
       // one candidate's hits in the tree: its lanes' records, in slot order,
       // from its next position on
-      auto emit = [&](uint32_t ql, const bool (&mine)[R], const uint64_t (&S)[R]) {
+      auto emit = [&](uint32_t ql, const uint64_t (&S)[R]) {
         Rec* const oq = o + bcast(loff, ql);
         const uint32_t lim = __builtin_amdgcn_readlane(llim, ql);
         uint32_t w = __builtin_amdgcn_readlane(lpos, ql);
@@ -391,10 +412,7 @@ __global__ __launch_bounds__(kFBlock) void k_forest_write(const Node<T, IS>* __r
         for (int r = 0; r < R; r++) {
           const uint64_t m = S[r];
           if (!m) continue;
-          const uint32_t at = w + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
-                                                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
-          if (mine[r] && at < lim) oq[at] = rv[r];
-          w += static_cast<uint32_t>(__builtin_popcountll(m));
+          w = emit_run(oq, w, lim, m, rv[r]);
         }
         lpos = lane == ql ? w : lpos;
       };
@@ -418,11 +436,262 @@ __global__ __launch_bounds__(kFBlock) void k_forest_write(const Node<T, IS>* __r
           anya |= Sa[r];
           anyb |= Sb[r];
         }
-        if (anya) emit(qa, ma, Sa);
-        if (anyb) emit(qb, mb, Sb);
+        if (anya) emit(qa, Sa);
+        if (anyb) emit(qb, Sb);
         moved = true;
       }
       if (moved) pos[p * kWave + lane] = lpos;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stabbing batches: every query a LEFT search whose dim1 bound equals its
+// dim2 bound v (an isovalue query: dim1 <= v <= dim2). With the batch's
+// values sorted, the queries that hit a point form one run of sorted
+// positions, [a, b) with a = #values < dim1 and b = #values <= dim2, so a
+// pass counts hits with two binary searches per point and a difference
+// array per segment, and the write pass tests a candidate against a slot
+// with two integer compares. NaN values hit nothing and sort last.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct StabPrep {
+  uint32_t all_stab;      // 1: every query of the batch is a stabbing query
+  uint32_t ns;            // non-NaN values
+  uint32_t pad[2];
+  T sv[kQSlots];          // the values in ascending order (ties by query index)
+  uint32_t sidx[kQSlots];  // sorted position -> query index (NaN values after ns)
+};
+
+template <typename T>
+__device__ inline bool is_stab(const Query<T>& q) {
+  return q.kind == KIND_LEFT && (q.max_dim1 == q.min_dim2 || (is_nan(q.max_dim1) && is_nan(q.min_dim2)));
+}
+
+// One workgroup of kQSlots threads: the flag, then the sort (rank of each
+// value among the batch's, ties by index).
+template <typename T>
+__global__ __launch_bounds__(kQSlots) void k_stab_prep(const Query<T>* __restrict__ queries, uint32_t nq,
+                                                       StabPrep<T>* __restrict__ prep) {
+  __shared__ T v[kQSlots];
+  __shared__ uint32_t ok[kQSlots];
+  const uint32_t i = threadIdx.x;
+  bool stab = true, num = false;
+  T x = T(0);
+  if (i < nq) {
+    const Query<T> q = queries[i];
+    stab = is_stab(q);
+    x = q.max_dim1;
+    num = !is_nan(x);
+  }
+  v[i] = x;
+  ok[i] = num ? 1u : 0u;
+  const int all = __syncthreads_and(stab ? 1 : 0);
+  const uint32_t ns = static_cast<uint32_t>(__syncthreads_count(num ? 1 : 0));
+  if (i < nq) {
+    uint32_t r = 0;
+    if (num) {
+      for (uint32_t j = 0; j < nq; j++) r += ok[j] & ((v[j] < x) | (v[j] == x && j < i) ? 1u : 0u);
+    } else {
+      r = ns;
+      for (uint32_t j = 0; j < i; j++) r += ok[j] ^ 1u;
+    }
+    if (num) prep->sv[r] = x;
+    prep->sidx[r] = i;
+  }
+  if (i == 0) {
+    prep->all_stab = all ? 1u : 0u;
+    prep->ns = ns;
+  }
+}
+
+// #values of sv[0..ns) below x (LT) or at most x (!LT); the same step
+// sequence in every lane.
+template <bool LT, typename T>
+__device__ inline uint32_t rank_in(const T* sv, uint32_t ns, uint32_t top, T x) {
+  uint32_t pos = 0;
+  for (uint32_t step = top; step > 0; step >>= 1) {
+    if (pos + step <= ns) {
+      const T y = sv[pos + step - 1];
+      if (LT ? (y < x) : (y <= x)) pos += step;
+    }
+  }
+  return pos;
+}
+
+__device__ inline uint32_t top_step(uint32_t ns) { return ns ? 1u << (31 - __builtin_clz(ns)) : 0u; }
+
+// Pass 1 of a stabbing batch: segcnt[q * nseg + seg] = hits of query q in
+// segment seg, from each point's run [a, b) of sorted queries (+1 at a,
+// -1 at b in the wave's difference array, then a prefix sum).
+template <typename T, typename IS, int R>
+__global__ __launch_bounds__(kFBlock) void k_stab_count(const Node<T, IS>* __restrict__ nodes, ForestArgs F,
+                                                        const StabPrep<T>* __restrict__ prep,
+                                                        uint32_t* __restrict__ segcnt) {
+  if (!prep->all_stab) return;
+  __shared__ T ssv[kQSlots];
+  __shared__ int32_t diff_all[kFWaves][kQSlots + 1];
+  const uint32_t ns = prep->ns;
+  for (uint32_t i = threadIdx.x; i < ns; i += kFBlock) ssv[i] = prep->sv[i];
+  const uint32_t lane = lane_id();
+  int32_t* diff = diff_all[threadIdx.x / kWave];
+  for (uint32_t i = lane; i <= kQSlots; i += kWave) diff[i] = 0;
+  __syncthreads();
+  const uint32_t seg = __builtin_amdgcn_readfirstlane(blockIdx.x * kFWaves + threadIdx.x / kWave);
+  if (seg >= F.nseg) return;
+  const uint32_t top = top_step(ns);
+  for (uint32_t tree = seg; tree < F.ntrees; tree += F.nseg) {
+    const Node<T, IS>* tn = nodes + (static_cast<uint64_t>(tree) << F.shift);
+    const uint32_t M = tree + 1 == F.ntrees ? F.last_M : F.full_M;
+    T d1[R], d2[R];
+    uint64_t vm[R];
+    load_tree<T, IS, R>(tn, F, M, d1, d2, vm);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if ((vm[r] >> lane) & 1ull) {
+        const uint32_t a = rank_in<true>(ssv, ns, top, d1[r]);
+        const uint32_t b = rank_in<false>(ssv, ns, top, d2[r]);
+        if (a < b) {
+          atomicAdd(&diff[a], 1);
+          atomicAdd(&diff[b], -1);
+        }
+      }
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  // prefix sum over sorted positions, 64 at a time
+  int32_t run = 0;
+  const uint32_t nq = F.nq;
+  for (uint32_t k0 = 0; k0 < nq; k0 += kWave) {
+    const uint32_t k = k0 + lane;
+    const int32_t d = k < ns ? diff[k] : 0;
+    int32_t inc = d;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int32_t o = __shfl_up(inc, off, kWave);
+      if (static_cast<int>(lane) >= off) inc += o;
+    }
+    if (k < nq) segcnt[static_cast<uint64_t>(prep->sidx[k]) * F.nseg + seg] = k < ns ? static_cast<uint32_t>(run + inc) : 0u;
+    run += __shfl(inc, kWave - 1, kWave);
+  }
+}
+
+// Pass 2 of a stabbing batch, with the output positions of k_forest_write:
+// per tree, the candidates are the sorted positions [kmin, kmax) that some
+// point's run covers, taken 64 at a time (lane l holding position w0 + l's
+// next position, offset and limit); a candidate hits the lanes whose run
+// [a, b) holds it.
+template <typename T, typename IS, bool RP, int R>
+__global__ __launch_bounds__(kFBlock) void k_stab_write(const Node<T, IS>* __restrict__ nodes,
+                                                        const void* __restrict__ ids_v, ForestArgs F,
+                                                        const StabPrep<T>* __restrict__ prep,
+                                                        const uint32_t* __restrict__ segoff,
+                                                        const unsigned long long* __restrict__ offsets,
+                                                        const unsigned long long* __restrict__ caps,
+                                                        unsigned long long cap_all, unsigned long long out_total,
+                                                        void* __restrict__ out) {
+  if (!prep->all_stab) return;
+  using Rec = typename std::conditional<RP, Point<T, IS>, IS>::type;
+  using ISt = typename std::conditional<has_id_v<IS>, IS, uint32_t>::type;
+  const ISt* __restrict__ ids = static_cast<const ISt*>(ids_v);
+  __shared__ T ssv[kQSlots];
+  __shared__ uint32_t spos_all[kFWaves][kQSlots];  // by sorted position: the query's next position
+  __shared__ unsigned long long soff[kQSlots];      // by sorted position: the query's first output record
+  __shared__ uint32_t slim[kQSlots];                // and the positions below which records are written
+  const uint32_t ns = prep->ns;
+  for (uint32_t k = threadIdx.x; k < ns; k += kFBlock) {
+    ssv[k] = prep->sv[k];
+    const uint32_t q = prep->sidx[k];
+    const unsigned long long off = offsets ? offsets[q] : 0ull;
+    const unsigned long long cap = caps ? caps[q] : cap_all;
+    const unsigned long long room = off < out_total ? out_total - off : 0ull;
+    const unsigned long long limit = cap < room ? cap : room;
+    soff[k] = off;
+    slim[k] = static_cast<uint32_t>(limit < 0xffffffffull ? limit : 0xffffffffull);
+  }
+  const uint32_t lane = lane_id();
+  uint32_t* spos = spos_all[threadIdx.x / kWave];
+  const uint32_t seg = __builtin_amdgcn_readfirstlane(blockIdx.x * kFWaves + threadIdx.x / kWave);
+  if (seg < F.nseg)
+    for (uint32_t k = lane; k < ns; k += kWave) spos[k] = segoff[static_cast<uint64_t>(prep->sidx[k]) * F.nseg + seg];
+  __syncthreads();
+  if (seg >= F.nseg) return;
+  Rec* const o = static_cast<Rec*>(out);
+  const uint32_t top = top_step(ns);
+  for (uint32_t tree = seg; tree < F.ntrees; tree += F.nseg) {
+    const uint64_t base = static_cast<uint64_t>(tree) << F.shift;
+    const Node<T, IS>* tn = nodes + base;
+    const uint32_t M = tree + 1 == F.ntrees ? F.last_M : F.full_M;
+    T d1[R], d2[R];
+    uint64_t vm[R];
+    load_tree<T, IS, R>(tn, F, M, d1, d2, vm);
+    uint32_t a[R], b[R];  // the lane's slot of chunk r is hit by sorted positions [a, b)
+    uint32_t kmin = ns, kmax = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      a[r] = 0;
+      b[r] = 0;
+      if ((vm[r] >> lane) & 1ull) {
+        a[r] = rank_in<true>(ssv, ns, top, d1[r]);
+        b[r] = rank_in<false>(ssv, ns, top, d2[r]);
+        if (a[r] < b[r]) {
+          kmin = a[r] < kmin ? a[r] : kmin;
+          kmax = b[r] > kmax ? b[r] : kmax;
+        }
+      }
+    }
+    kmin = wave_min(kmin);
+    kmax = wave_max(kmax);
+    kmin = __builtin_amdgcn_readfirstlane(kmin);
+    kmax = __builtin_amdgcn_readfirstlane(kmax);
+    if (kmin >= kmax) continue;
+    Rec rv[R];  // lane l's record of chunk r: slot 64 r + l
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      rv[r] = Rec{};
+      if ((vm[r] >> lane) & 1ull) {
+        if constexpr (RP) {
+          rv[r].dim1_val = d1[r];
+          rv[r].dim2_val = d2[r];
+          if constexpr (has_id_v<IS>) rv[r].id = tn[r * kWave + lane].id;
+        } else {
+          rv[r] = ids[base + r * kWave + lane];
+        }
+      }
+    }
+    for (uint32_t w0 = kmin & ~(kWave - 1u); w0 < kmax; w0 += kWave) {
+      const uint32_t kl = w0 + lane;  // the lane's sorted position of this window
+      uint32_t lpos = 0, llim = 0;
+      unsigned long long loff = 0;
+      if (kl < ns) {
+        lpos = spos[kl];
+        loff = soff[kl];
+        llim = slim[kl];
+      }
+      const uint32_t k_lo = kmin > w0 ? kmin : w0;
+      const uint32_t k_hi = kmax < w0 + kWave ? kmax : w0 + kWave;
+      for (uint32_t k = k_lo; k < k_hi; k++) {
+        uint64_t S[R];
+        uint64_t any = 0;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          S[r] = __builtin_amdgcn_uicmp(a[r], k, 37) & __builtin_amdgcn_uicmp(b[r], k, 34);  // a <= k < b
+          any |= S[r];
+        }
+        if (!any) continue;
+        const uint32_t ql = k - w0;
+        Rec* const oq = o + bcast(loff, ql);
+        const uint32_t lim = __builtin_amdgcn_readlane(llim, ql);
+        uint32_t w = __builtin_amdgcn_readlane(lpos, ql);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const uint64_t m = S[r];
+          if (!m) continue;
+          w = emit_run(oq, w, lim, m, rv[r]);
+        }
+        lpos = lane == ql ? w : lpos;
+      }
+      if (kl < ns && kl >= k_lo && kl < k_hi) spos[kl] = lpos;
     }
   }
 }
@@ -490,6 +759,9 @@ ForestArgs forest_args(const SearchArgs& a, int cus) {
   return F;
 }
 
+// The per (query, segment) counters, then the stabbing-batch table.
+size_t stab_prep_at(const ForestArgs& F) { return (static_cast<size_t>(F.nseg) * F.nq * 4 + 255) & ~size_t{255}; }
+
 }  // namespace
 
 size_t forest_dense_scratch_bytes(size_t n, size_t slots_per_tree, size_t nq, int cus) {
@@ -498,7 +770,7 @@ size_t forest_dense_scratch_bytes(size_t n, size_t slots_per_tree, size_t nq, in
   a.slots_per_tree = slots_per_tree;
   a.nq = nq;
   const ForestArgs F = forest_args(a, cus);
-  return static_cast<size_t>(F.nseg) * nq * 4;
+  return stab_prep_at(F) + sizeof(StabPrep<double>);
 }
 
 Status forest_dense_search(const SearchArgs& a, void* segbuf, hipStream_t s, int plan) {
@@ -518,24 +790,35 @@ Status forest_dense_search(const SearchArgs& a, void* segbuf, hipStream_t s, int
       const auto* nodes = static_cast<const Node<T, IS>*>(a.nodes);
       const auto* qs = static_cast<const Query<T>*>(a.queries);
       const auto* bounds = static_cast<const Range1<T>*>(a.tree_bounds);
+      auto* prep = reinterpret_cast<StabPrep<T>*>(static_cast<unsigned char*>(segbuf) + stab_prep_at(F));
+      const uint32_t* stab = &prep->all_stab;
       const unsigned grid = (F.nseg + kFWaves - 1) / kFWaves;
       auto go = [&](auto rtag) -> Status {
         constexpr int R = decltype(rtag)::value;
+        // whether the batch is all stabbing queries, and their sorted values;
+        // of each pair of passes below, the one for the other kind of batch
+        // returns at once
+        hipLaunchKernelGGL((k_stab_prep<T>), dim3(1), dim3(kQSlots), 0, s, qs, F.nq, prep);
         if (a.mode == 0 || plan != 2) {
           // plan 2 writes from the segment offsets a plan 1 count pass kept
           hipLaunchKernelGGL((k_forest_count<T, IS, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, bounds, F, qs,
-                             segcnt);
+                             stab, segcnt);
+          hipLaunchKernelGGL((k_stab_count<T, IS, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, F, prep, segcnt);
           // write mode without kept offsets: the per-query totals go to the cursor
           hipLaunchKernelGGL(k_forest_seg_scan, dim3(F.nq), dim3(kScanBlock), 0, s, segcnt, F.nseg, a.counts);
         }
         if (a.mode == 0) return hip_status(hipGetLastError(), "Error in launching forest search");
         if (a.ret_point) {
           hipLaunchKernelGGL((k_forest_write<T, IS, true, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, a.ids, bounds,
-                             F, qs, segcnt, a.offsets, a.caps, a.cap_all, a.out_total, a.out);
+                             F, qs, stab, segcnt, a.offsets, a.caps, a.cap_all, a.out_total, a.out);
+          hipLaunchKernelGGL((k_stab_write<T, IS, true, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, a.ids, F, prep,
+                             segcnt, a.offsets, a.caps, a.cap_all, a.out_total, a.out);
         } else {
           if constexpr (has_id_v<IS>) {
             hipLaunchKernelGGL((k_forest_write<T, IS, false, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, a.ids,
-                               bounds, F, qs, segcnt, a.offsets, a.caps, a.cap_all, a.out_total, a.out);
+                               bounds, F, qs, stab, segcnt, a.offsets, a.caps, a.cap_all, a.out_total, a.out);
+            hipLaunchKernelGGL((k_stab_write<T, IS, false, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, a.ids, F,
+                               prep, segcnt, a.offsets, a.caps, a.cap_all, a.out_total, a.out);
           }
         }
         return hip_status(hipGetLastError(), "Error in launching forest search");

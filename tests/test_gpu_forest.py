@@ -262,3 +262,106 @@ def test_forest_stationary_batches(torch, P, t, i, tpb, n):
             gap = out3.cpu().numpy()[(o[k] + want[k] // 2) * rb:(o[k] + want[k]) * rb]
             assert np.all(gap == 0x5A), k
     f.close()
+
+
+def _stab_batch(rng, t, nq, lo, hi):
+    """Stabbing (isovalue) queries: LEFT searches with max_dim1 = min_dim2 =
+    v, on the points' grid (ties), with NaN, infinite, repeated and
+    out-of-range values; min_dim1 (unused by LEFT) is arbitrary."""
+    import pst_amd as P
+    qs = np.zeros(nq, P.query_dtype(t))
+    if t in ("f32", "f64"):
+        v = rng.integers(int(lo * 8) - 8, int(hi * 8) + 8, nq).astype(np.float64) / 8
+    else:
+        v = rng.integers(lo - 5, hi + 5, nq)
+    if nq > 8:
+        v[4] = v[5]
+        if t in ("f32", "f64"):
+            v[0], v[1], v[2] = np.nan, np.inf, -np.inf
+        else:
+            info = np.iinfo(np.int32 if t == "i32" else np.int64)
+            v[0], v[1] = info.min, info.max
+    qs["max_dim1"] = qs["min_dim2"] = v
+    qs["min_dim1"] = rng.uniform(lo, hi, nq).astype(qs["min_dim1"].dtype)
+    qs["kind"] = 1
+    return qs
+
+
+# plus whole 8-byte points (records staged like IDs)
+STAB_CASES = STATIONARY_CASES + [("f32", None, 64, 5000), ("i32", None, 31, 3001)]
+
+
+@pytest.mark.parametrize("t,i,tpb,n", STAB_CASES)
+def test_forest_stabbing_batches(torch, P, t, i, tpb, n):
+    """Batches of stabbing (isovalue) queries, which stationary forest
+    passes take through sorted values and per-point runs of them: counts,
+    one-call writes, truncating capacities and separate count and write
+    calls equal brute force; and the records land at the same positions,
+    byte for byte, as through the general passes (the same batch plus a
+    final query with no hits, which is not a stabbing query)."""
+    rng = np.random.default_rng(tpb * 11 + n)
+    p = _rand(rng, t, i, n)
+    f, keep = _forest(P, p, t, i, tpb)
+    lo, hi = (-1, 8) if t in ("f32", "f64") else (-60, 80)
+    rec = np.dtype(O.NP[i]) if i is not None else P.point_dtype(t, i)
+    rp = P.RET_POINT if i is None else P.RET_ID
+    rb = rec.itemsize
+    for nq in (9, 200, 511):
+        print(f"stabbing forest {t}/{i} tpb={tpb} n={n} nq={nq}", flush=True)
+        qs = _stab_batch(rng, t, nq, lo, hi)
+        exp = _expect_batch(p, qs)
+        want = np.array([len(e) for e in exp], np.int64)
+        total = int(want.sum())
+        dq = P.to_device(qs)
+        counts = torch.zeros(nq, dtype=torch.int64, device="cuda")
+        offs = torch.zeros(nq + 1, dtype=torch.int64, device="cuda")
+        f.batch_count(dq, nq, counts)
+        torch.cuda.synchronize()
+        assert np.array_equal(counts.cpu().numpy(), want)
+        out = torch.full((total * rb + 64,), 0x5A, dtype=torch.uint8, device="cuda")
+        f.batch_search(dq, nq, counts, offs, out, total, ret=rp)
+        torch.cuda.synchronize()
+        o = offs.cpu().numpy()
+        raw = out.cpu().numpy()
+        res = raw[:total * rb].view(rec)
+        for k in range(nq):
+            seg = res[o[k]:o[k] + want[k]]
+            if rp == P.RET_POINT:
+                assert G.canon(seg, True, i is not None) == G.canon(p[exp[k]], True, i is not None), k
+            else:
+                assert np.array_equal(np.sort(seg), np.sort(p["id"][exp[k]])), k
+        assert np.all(raw[total * rb:] == 0x5A)
+        # the general passes put every record at the same position
+        q2 = np.concatenate([qs, np.zeros(1, qs.dtype)])
+        q2["kind"][-1] = 0
+        q2["min_dim1"][-1], q2["max_dim1"][-1] = hi + 100, lo - 100  # an empty dim1 range
+        d2 = P.to_device(q2)
+        c2 = torch.zeros(nq + 1, dtype=torch.int64, device="cuda")
+        o2 = torch.zeros(nq + 2, dtype=torch.int64, device="cuda")
+        out_g = torch.full_like(out, 0x5A)
+        f.batch_search(d2, nq + 1, c2, o2, out_g, total, ret=rp)
+        torch.cuda.synchronize()
+        assert int(c2[-1]) == 0
+        if f.slots_per_tree <= 511:  # stationary passes: fixed output positions
+            assert np.array_equal(out_g.cpu().numpy(), raw)
+        # a truncating capacity writes exactly the full call's first records
+        cap = total // 3
+        out2 = torch.full_like(out, 0x5A)
+        f.batch_search(dq, nq, counts, offs, out2, cap, ret=rp)
+        torch.cuda.synchronize()
+        raw2 = out2.cpu().numpy()
+        if f.slots_per_tree <= 511:  # stationary passes: fixed output positions
+            assert np.array_equal(raw2[:cap * rb], raw[:cap * rb])
+        assert np.all(raw2[cap * rb:] == 0x5A)
+        # separate count and write calls, each query capped at half its results
+        caps = torch.from_numpy(want // 2).to("cuda")
+        out3 = torch.full_like(out, 0x5A)
+        f.batch_write(dq, nq, offs, out3, ret=rp, caps_d=caps)
+        torch.cuda.synchronize()
+        raw3 = out3.cpu().numpy()
+        for k in range(nq):
+            h = int(want[k] // 2)
+            if f.slots_per_tree <= 511:
+                assert np.array_equal(raw3[o[k] * rb:(o[k] + h) * rb], raw[o[k] * rb:(o[k] + h) * rb]), k
+            assert np.all(raw3[(o[k] + h) * rb:(o[k] + want[k]) * rb] == 0x5A), k
+    f.close()
