@@ -88,7 +88,7 @@ __global__ __launch_bounds__(kHistBlock) void k_sort_hist(const K* __restrict__ 
                                                           uint32_t* __restrict__ hist) {
   constexpr int P = static_cast<int>(sizeof(K));
   __shared__ uint32_t h[kSub][P][kDigits];
-  const uint32_t tid = threadIdx.x, lane = lane_id();
+  const uint32_t tid = threadIdx.x;
   for (uint32_t i = tid; i < kSub * P * kDigits; i += kHistBlock) (&h[0][0][0])[i] = 0u;
   __syncthreads();
   uint32_t(*mine)[kDigits] = h[tid % kSub];
@@ -103,19 +103,19 @@ __global__ __launch_bounds__(kHistBlock) void k_sort_hist(const K* __restrict__ 
 #pragma unroll
     for (int e = 0; e < kHistPer; e++) {
       const bool ok = b + static_cast<uint64_t>(e) * kHistBlock + tid < n;
-      const uint64_t act = __ballot(ok);
-      if (act == 0) continue;
+      if (ok) {
 #pragma unroll
-      for (int p = 0; p < P; p++) {
-        const uint32_t d = digit_of(k[e], p * kDigitBits);
-        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-        // a wave whose keys all share the digit adds once
-        if (__ballot(ok && d == d0) == act) {
-          if (lane == static_cast<uint32_t>(__builtin_ctzll(act)))
-            atomicAdd(&h[0][p][d0], static_cast<uint32_t>(__builtin_popcountll(act)));
-        } else if (ok) {
-          atomicAdd(&mine[p][d], 1u);
-        }
+        for (int p = 0; p < P - 1; p++) atomicAdd(&mine[p][digit_of(k[e], p * kDigitBits)], 1u);
+      }
+      // the top digit, often shared by a whole wave of clustered keys: such
+      // a wave adds once
+      const uint32_t d = digit_of(k[e], (P - 1) * kDigitBits);
+      const uint64_t act = __ballot(ok);
+      const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+      if (act == ~0ull && __ballot(d != d0) == 0ull) {
+        if (tid % kWave == 0) atomicAdd(&h[0][P - 1][d0], static_cast<uint32_t>(kWave));
+      } else if (ok) {
+        atomicAdd(&mine[P - 1][d], 1u);
       }
     }
   }
