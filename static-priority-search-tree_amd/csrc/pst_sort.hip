@@ -56,6 +56,7 @@ constexpr int kHistPer = 16;
 constexpr uint32_t kHistChunk = kHistBlock * kHistPer;
 constexpr unsigned kHistMaxBlocks = 1024;
 constexpr uint32_t kLookbackSpins = 1u << 24;             // about a second
+constexpr uint32_t kLookAhead = 4;                        // look-back words read at once
 
 template <typename K>
 __device__ inline uint32_t digit_of(K k, uint32_t shift) {
@@ -261,21 +262,35 @@ __device__ __forceinline__ void sort_tile(PassLds<K>& L, const K* __restrict__ k
     uint32_t excl = 0;
     if (tile > 0) {
       const uint32_t ta = tag_count(pass), tp = tag_prefix(pass);
-      uint32_t j = tile - 1;
+      uint32_t j = tile - 1;  // the next tile to read; tiles above j are summed
       uint32_t spins = 0;
-      while (true) {
-        const unsigned long long w = __hip_atomic_load(&status[static_cast<size_t>(j) * kDigits + tid],
-                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t tag = static_cast<uint32_t>(w >> 32);
-        if (tag == tp) {
-          excl += static_cast<uint32_t>(w);
-          break;
+      bool done = false;
+      while (!done) {
+        // kLookAhead words at a time (independent loads), then taken in
+        // order: a prefix ends the walk, a count is added, a word not yet
+        // published stops this step
+        unsigned long long w[kLookAhead];
+#pragma unroll
+        for (uint32_t i = 0; i < kLookAhead; i++)
+          w[i] = i <= j ? __hip_atomic_load(&status[static_cast<size_t>(j - i) * kDigits + tid], __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT)
+                        : 0ull;
+        bool stalled = false;
+#pragma unroll
+        for (uint32_t i = 0; i < kLookAhead; i++) {
+          if (done || stalled) continue;
+          const uint32_t tag = static_cast<uint32_t>(w[i] >> 32);
+          if (tag == tp) {
+            excl += static_cast<uint32_t>(w[i]);
+            done = true;
+          } else if (tag == ta && j > 0) {  // tile 0 publishes a prefix
+            excl += static_cast<uint32_t>(w[i]);
+            j--;
+          } else {
+            stalled = true;
+          }
         }
-        if (tag == ta && j > 0) {  // tile 0 publishes a prefix
-          excl += static_cast<uint32_t>(w);
-          j--;
-          continue;
-        }
+        if (done || !stalled) continue;
         if (++spins > kLookbackSpins) {
           *fail = 1;
           break;
