@@ -210,6 +210,29 @@ __device__ inline uint32_t emit_run(Rec* oq, uint32_t w, uint32_t lim, uint64_t 
   return w + pop;
 }
 
+// Two chunks' runs, chunk a's records then chunk b's, from w on: one store
+// takes every lane with a hit (its chunk-a record where it has one, else
+// its chunk-b record), a second only the lanes that hit in both chunks.
+// Returns w + |ma| + |mb|.
+template <typename Rec>
+__device__ inline uint32_t emit_two(Rec* oq, uint32_t w, uint32_t lim, uint64_t ma, uint64_t mb, const Rec& ra,
+                                    const Rec& rb) {
+  const uint32_t pa = static_cast<uint32_t>(__builtin_popcountll(ma));
+  const uint32_t pb = static_cast<uint32_t>(__builtin_popcountll(mb));
+  if (w + pa + pb > lim) return emit_run(oq, emit_run(oq, w, lim, ma, ra), lim, mb, rb);
+  const uint32_t at_a = w + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(ma >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(ma), 0u));
+  const uint32_t at_b = w + pa + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mb >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mb), 0u));
+  if (__builtin_amdgcn_inverse_ballot_w64(ma | mb)) {
+    const bool in_a = __builtin_amdgcn_inverse_ballot_w64(ma);
+    oq[in_a ? at_a : at_b] = in_a ? ra : rb;
+  }
+  const uint64_t both = ma & mb;
+  if (both && __builtin_amdgcn_inverse_ballot_w64(both)) oq[at_b] = rb;
+  return w + pa + pb;
+}
+
 // Pass 1: segcnt[q * nseg + seg] = hits of query q in segment seg. One
 // wave per segment. Per tree, lanes first take a query each (64 per pass)
 // to find the candidates: the queries that may hit the tree (its dim1 range
@@ -409,10 +432,9 @@ __global__ __launch_bounds__(kFBlock) void k_forest_write(const Node<T, IS>* __r
         const uint32_t lim = __builtin_amdgcn_readlane(llim, ql);
         uint32_t w = __builtin_amdgcn_readlane(lpos, ql);
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-          const uint64_t m = S[r];
-          if (!m) continue;
-          w = emit_run(oq, w, lim, m, rv[r]);
+        for (int r = 0; r + 1 < R; r += 2) w = emit_two(oq, w, lim, S[r], S[r + 1], rv[r], rv[r + 1]);
+        if constexpr (R % 2 == 1) {
+          if (S[R - 1]) w = emit_run(oq, w, lim, S[R - 1], rv[R - 1]);
         }
         lpos = lane == ql ? w : lpos;
       };
@@ -684,10 +706,9 @@ __global__ __launch_bounds__(kFBlock) void k_stab_write(const Node<T, IS>* __res
         const uint32_t lim = __builtin_amdgcn_readlane(llim, ql);
         uint32_t w = __builtin_amdgcn_readlane(lpos, ql);
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-          const uint64_t m = S[r];
-          if (!m) continue;
-          w = emit_run(oq, w, lim, m, rv[r]);
+        for (int r = 0; r + 1 < R; r += 2) w = emit_two(oq, w, lim, S[r], S[r + 1], rv[r], rv[r + 1]);
+        if constexpr (R % 2 == 1) {
+          if (S[R - 1]) w = emit_run(oq, w, lim, S[R - 1], rv[R - 1]);
         }
         lpos = lane == ql ? w : lpos;
       }
