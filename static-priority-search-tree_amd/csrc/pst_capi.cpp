@@ -578,8 +578,9 @@ size_t pst_gpu_arr_calc_global_mem_needed(size_t n, unsigned threads_per_block, 
       ntrees * (St + 1) * (pst::node_size(dtype, id_dtype) + pst::dtype_size(dtype) + id_size(id_dtype)) +
       pst::forest_bounds_bytes(dtype, n, St);
   // construction works in LDS; a search needs a result array of n points,
-  // two 4M-entry work lists and a 4M-entry copy task list
-  const size_t search = n * pst::point_size(dtype, id_dtype) + (3ull << 22) * 16;
+  // two 4M-entry work lists and a 4M-entry copy task list; a stationary
+  // batch keeps 4 B per slot (each slot's run of sorted stabbing values)
+  const size_t search = n * pst::point_size(dtype, id_dtype) + (3ull << 22) * 16 + ntrees * (St + 1) * 4;
   return tree + search;
 }
 

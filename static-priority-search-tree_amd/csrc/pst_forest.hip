@@ -545,11 +545,12 @@ __device__ inline uint32_t top_step(uint32_t ns) { return ns ? 1u << (31 - __bui
 
 // Pass 1 of a stabbing batch: segcnt[q * nseg + seg] = hits of query q in
 // segment seg, from each point's run [a, b) of sorted queries (+1 at a,
-// -1 at b in the wave's difference array, then a prefix sum).
+// -1 at b in the wave's difference array, then a prefix sum); each slot's
+// run is stored (a | b << 16, 0 for an empty slot) for pass 2.
 template <typename T, typename IS, int R>
 __global__ __launch_bounds__(kFBlock) void k_stab_count(const Node<T, IS>* __restrict__ nodes, ForestArgs F,
                                                         const StabPrep<T>* __restrict__ prep,
-                                                        uint32_t* __restrict__ segcnt) {
+                                                        uint32_t* __restrict__ segcnt, uint32_t* __restrict__ runs) {
   if (!prep->all_stab) return;
   __shared__ T ssv[kQSlots];
   __shared__ int32_t diff_all[kFWaves][kQSlots + 1];
@@ -568,16 +569,20 @@ __global__ __launch_bounds__(kFBlock) void k_stab_count(const Node<T, IS>* __res
     T d1[R], d2[R];
     uint64_t vm[R];
     load_tree<T, IS, R>(tn, F, M, d1, d2, vm);
+    uint32_t* const tr = runs + (static_cast<uint64_t>(tree) << F.shift);
 #pragma unroll
     for (int r = 0; r < R; r++) {
+      uint32_t a = 0, b = 0;
       if ((vm[r] >> lane) & 1ull) {
-        const uint32_t a = rank_in<true>(ssv, ns, top, d1[r]);
-        const uint32_t b = rank_in<false>(ssv, ns, top, d2[r]);
+        a = rank_in<true>(ssv, ns, top, d1[r]);
+        b = rank_in<false>(ssv, ns, top, d2[r]);
         if (a < b) {
           atomicAdd(&diff[a], 1);
           atomicAdd(&diff[b], -1);
         }
       }
+      const uint32_t s = r * kWave + lane;
+      if (s < (1u << F.shift)) tr[s] = a | (b << 16);  // the write pass's runs (empty: 0)
     }
   }
   __builtin_amdgcn_wave_barrier();
@@ -599,14 +604,15 @@ __global__ __launch_bounds__(kFBlock) void k_stab_count(const Node<T, IS>* __res
 }
 
 // Pass 2 of a stabbing batch, with the output positions of k_forest_write:
-// per tree, the candidates are the sorted positions [kmin, kmax) that some
-// point's run covers, taken 64 at a time (lane l holding position w0 + l's
-// next position, offset and limit); a candidate hits the lanes whose run
-// [a, b) holds it.
+// per tree, each slot's run [a, b) as pass 1 stored it; the candidates are
+// the sorted positions [kmin, kmax) that some run covers, taken 64 at a
+// time (lane l holding position w0 + l's next position, offset and limit);
+// a candidate hits the lanes whose run holds it.
 template <typename T, typename IS, bool RP, int R>
 __global__ __launch_bounds__(kFBlock) void k_stab_write(const Node<T, IS>* __restrict__ nodes,
                                                         const void* __restrict__ ids_v, ForestArgs F,
                                                         const StabPrep<T>* __restrict__ prep,
+                                                        const uint32_t* __restrict__ runs,
                                                         const uint32_t* __restrict__ segoff,
                                                         const unsigned long long* __restrict__ offsets,
                                                         const unsigned long long* __restrict__ caps,
@@ -616,13 +622,11 @@ __global__ __launch_bounds__(kFBlock) void k_stab_write(const Node<T, IS>* __res
   using Rec = typename std::conditional<RP, Point<T, IS>, IS>::type;
   using ISt = typename std::conditional<has_id_v<IS>, IS, uint32_t>::type;
   const ISt* __restrict__ ids = static_cast<const ISt*>(ids_v);
-  __shared__ T ssv[kQSlots];
   __shared__ uint32_t spos_all[kFWaves][kQSlots];  // by sorted position: the query's next position
   __shared__ unsigned long long soff[kQSlots];      // by sorted position: the query's first output record
   __shared__ uint32_t slim[kQSlots];                // and the positions below which records are written
   const uint32_t ns = prep->ns;
   for (uint32_t k = threadIdx.x; k < ns; k += kFBlock) {
-    ssv[k] = prep->sv[k];
     const uint32_t q = prep->sidx[k];
     const unsigned long long off = offsets ? offsets[q] : 0ull;
     const unsigned long long cap = caps ? caps[q] : cap_all;
@@ -639,27 +643,20 @@ __global__ __launch_bounds__(kFBlock) void k_stab_write(const Node<T, IS>* __res
   __syncthreads();
   if (seg >= F.nseg) return;
   Rec* const o = static_cast<Rec*>(out);
-  const uint32_t top = top_step(ns);
   for (uint32_t tree = seg; tree < F.ntrees; tree += F.nseg) {
     const uint64_t base = static_cast<uint64_t>(tree) << F.shift;
     const Node<T, IS>* tn = nodes + base;
-    const uint32_t M = tree + 1 == F.ntrees ? F.last_M : F.full_M;
-    T d1[R], d2[R];
-    uint64_t vm[R];
-    load_tree<T, IS, R>(tn, F, M, d1, d2, vm);
-    uint32_t a[R], b[R];  // the lane's slot of chunk r is hit by sorted positions [a, b)
+    uint32_t a[R], b[R];  // the lane's slot of chunk r is hit by sorted positions [a, b) (empty: 0, 0)
     uint32_t kmin = ns, kmax = 0;
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      a[r] = 0;
-      b[r] = 0;
-      if ((vm[r] >> lane) & 1ull) {
-        a[r] = rank_in<true>(ssv, ns, top, d1[r]);
-        b[r] = rank_in<false>(ssv, ns, top, d2[r]);
-        if (a[r] < b[r]) {
-          kmin = a[r] < kmin ? a[r] : kmin;
-          kmax = b[r] > kmax ? b[r] : kmax;
-        }
+      const uint32_t s = r * kWave + lane;
+      const uint32_t ab = s < (1u << F.shift) ? runs[base + s] : 0u;
+      a[r] = ab & 0xffffu;
+      b[r] = ab >> 16;
+      if (a[r] < b[r]) {
+        kmin = a[r] < kmin ? a[r] : kmin;
+        kmax = b[r] > kmax ? b[r] : kmax;
       }
     }
     kmin = wave_min(kmin);
@@ -671,11 +668,12 @@ __global__ __launch_bounds__(kFBlock) void k_stab_write(const Node<T, IS>* __res
 #pragma unroll
     for (int r = 0; r < R; r++) {
       rv[r] = Rec{};
-      if ((vm[r] >> lane) & 1ull) {
+      if (a[r] < b[r]) {  // a slot with hits holds a point
         if constexpr (RP) {
-          rv[r].dim1_val = d1[r];
-          rv[r].dim2_val = d2[r];
-          if constexpr (has_id_v<IS>) rv[r].id = tn[r * kWave + lane].id;
+          const Node<T, IS> nd = tn[r * kWave + lane];
+          rv[r].dim1_val = nd.dim1;
+          rv[r].dim2_val = nd.dim2;
+          if constexpr (has_id_v<IS>) rv[r].id = nd.id;
         } else {
           rv[r] = ids[base + r * kWave + lane];
         }
@@ -780,8 +778,10 @@ ForestArgs forest_args(const SearchArgs& a, int cus) {
   return F;
 }
 
-// The per (query, segment) counters, then the stabbing-batch table.
+// The per (query, segment) counters, then the stabbing-batch table, then
+// per slot of the forest its run of sorted positions (two 16-bit values).
 size_t stab_prep_at(const ForestArgs& F) { return (static_cast<size_t>(F.nseg) * F.nq * 4 + 255) & ~size_t{255}; }
+size_t stab_runs_at(const ForestArgs& F) { return stab_prep_at(F) + ((sizeof(StabPrep<double>) + 255) & ~size_t{255}); }
 
 }  // namespace
 
@@ -791,7 +791,7 @@ size_t forest_dense_scratch_bytes(size_t n, size_t slots_per_tree, size_t nq, in
   a.slots_per_tree = slots_per_tree;
   a.nq = nq;
   const ForestArgs F = forest_args(a, cus);
-  return stab_prep_at(F) + sizeof(StabPrep<double>);
+  return stab_runs_at(F) + (static_cast<size_t>(F.ntrees) << F.shift) * 4;
 }
 
 Status forest_dense_search(const SearchArgs& a, void* segbuf, hipStream_t s, int plan) {
@@ -813,6 +813,7 @@ Status forest_dense_search(const SearchArgs& a, void* segbuf, hipStream_t s, int
       const auto* bounds = static_cast<const Range1<T>*>(a.tree_bounds);
       auto* prep = reinterpret_cast<StabPrep<T>*>(static_cast<unsigned char*>(segbuf) + stab_prep_at(F));
       const uint32_t* stab = &prep->all_stab;
+      auto* runs = reinterpret_cast<uint32_t*>(static_cast<unsigned char*>(segbuf) + stab_runs_at(F));
       const unsigned grid = (F.nseg + kFWaves - 1) / kFWaves;
       auto go = [&](auto rtag) -> Status {
         constexpr int R = decltype(rtag)::value;
@@ -824,7 +825,7 @@ Status forest_dense_search(const SearchArgs& a, void* segbuf, hipStream_t s, int
           // plan 2 writes from the segment offsets a plan 1 count pass kept
           hipLaunchKernelGGL((k_forest_count<T, IS, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, bounds, F, qs,
                              stab, segcnt);
-          hipLaunchKernelGGL((k_stab_count<T, IS, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, F, prep, segcnt);
+          hipLaunchKernelGGL((k_stab_count<T, IS, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, F, prep, segcnt, runs);
           // write mode without kept offsets: the per-query totals go to the cursor
           hipLaunchKernelGGL(k_forest_seg_scan, dim3(F.nq), dim3(kScanBlock), 0, s, segcnt, F.nseg, a.counts);
         }
@@ -833,13 +834,13 @@ Status forest_dense_search(const SearchArgs& a, void* segbuf, hipStream_t s, int
           hipLaunchKernelGGL((k_forest_write<T, IS, true, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, a.ids, bounds,
                              F, qs, stab, segcnt, a.offsets, a.caps, a.cap_all, a.out_total, a.out);
           hipLaunchKernelGGL((k_stab_write<T, IS, true, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, a.ids, F, prep,
-                             segcnt, a.offsets, a.caps, a.cap_all, a.out_total, a.out);
+                             runs, segcnt, a.offsets, a.caps, a.cap_all, a.out_total, a.out);
         } else {
           if constexpr (has_id_v<IS>) {
             hipLaunchKernelGGL((k_forest_write<T, IS, false, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, a.ids,
                                bounds, F, qs, stab, segcnt, a.offsets, a.caps, a.cap_all, a.out_total, a.out);
             hipLaunchKernelGGL((k_stab_write<T, IS, false, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, a.ids, F,
-                               prep, segcnt, a.offsets, a.caps, a.cap_all, a.out_total, a.out);
+                               prep, runs, segcnt, a.offsets, a.caps, a.cap_all, a.out_total, a.out);
           }
         }
         return hip_status(hipGetLastError(), "Error in launching forest search");
