@@ -527,20 +527,6 @@ __global__ __launch_bounds__(kQSlots) void k_stab_prep(const Query<T>* __restric
   }
 }
 
-// #values of sv[0..ns) below x (LT) or at most x (!LT); the same step
-// sequence in every lane.
-template <bool LT, typename T>
-__device__ inline uint32_t rank_in(const T* sv, uint32_t ns, uint32_t top, T x) {
-  uint32_t pos = 0;
-  for (uint32_t step = top; step > 0; step >>= 1) {
-    if (pos + step <= ns) {
-      const T y = sv[pos + step - 1];
-      if (LT ? (y < x) : (y <= x)) pos += step;
-    }
-  }
-  return pos;
-}
-
 __device__ inline uint32_t top_step(uint32_t ns) { return ns ? 1u << (31 - __builtin_clz(ns)) : 0u; }
 
 // Pass 1 of a stabbing batch: segcnt[q * nseg + seg] = hits of query q in
@@ -563,19 +549,43 @@ __global__ __launch_bounds__(kFBlock) void k_stab_count(const Node<T, IS>* __res
   const uint32_t seg = __builtin_amdgcn_readfirstlane(blockIdx.x * kFWaves + threadIdx.x / kWave);
   if (seg >= F.nseg) return;
   const uint32_t top = top_step(ns);
+  // the next tree's slots are loaded while the current one is worked on
+  T n1[R], n2[R];
+  uint64_t nvm[R];
+  auto load = [&](uint32_t tree) {
+    load_tree<T, IS, R>(nodes + (static_cast<uint64_t>(tree) << F.shift), F,
+                        tree + 1 == F.ntrees ? F.last_M : F.full_M, n1, n2, nvm);
+  };
+  if (seg < F.ntrees) load(seg);
   for (uint32_t tree = seg; tree < F.ntrees; tree += F.nseg) {
-    const Node<T, IS>* tn = nodes + (static_cast<uint64_t>(tree) << F.shift);
-    const uint32_t M = tree + 1 == F.ntrees ? F.last_M : F.full_M;
     T d1[R], d2[R];
     uint64_t vm[R];
-    load_tree<T, IS, R>(tn, F, M, d1, d2, vm);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      d1[r] = n1[r];
+      d2[r] = n2[r];
+      vm[r] = nvm[r];
+    }
+    if (tree + F.nseg < F.ntrees) load(tree + F.nseg);
     uint32_t* const tr = runs + (static_cast<uint64_t>(tree) << F.shift);
+    // the 2R searches of the lane in lockstep: independent LDS reads per step
+    uint32_t pa[R], pb[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) pa[r] = pb[r] = 0;
+    for (uint32_t step = top; step > 0; step >>= 1) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const uint32_t ia = pa[r] + step, ib = pb[r] + step;
+        if (ia <= ns && ssv[ia - 1] < d1[r]) pa[r] = ia;   // #values < dim1
+        if (ib <= ns && ssv[ib - 1] <= d2[r]) pb[r] = ib;  // #values <= dim2
+      }
+    }
 #pragma unroll
     for (int r = 0; r < R; r++) {
       uint32_t a = 0, b = 0;
       if ((vm[r] >> lane) & 1ull) {
-        a = rank_in<true>(ssv, ns, top, d1[r]);
-        b = rank_in<false>(ssv, ns, top, d2[r]);
+        a = pa[r];
+        b = pb[r];
         if (a < b) {
           atomicAdd(&diff[a], 1);
           atomicAdd(&diff[b], -1);
@@ -643,42 +653,51 @@ __global__ __launch_bounds__(kFBlock) void k_stab_write(const Node<T, IS>* __res
   __syncthreads();
   if (seg >= F.nseg) return;
   Rec* const o = static_cast<Rec*>(out);
-  for (uint32_t tree = seg; tree < F.ntrees; tree += F.nseg) {
+  // the next tree's runs and records are loaded while the current one is
+  // written (every slot of a tree's storage can be read)
+  uint32_t nab[R];
+  Rec nrv[R];
+  auto load = [&](uint32_t tree) {
     const uint64_t base = static_cast<uint64_t>(tree) << F.shift;
-    const Node<T, IS>* tn = nodes + base;
-    uint32_t a[R], b[R];  // the lane's slot of chunk r is hit by sorted positions [a, b) (empty: 0, 0)
-    uint32_t kmin = ns, kmax = 0;
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const uint32_t s = r * kWave + lane;
-      const uint32_t ab = s < (1u << F.shift) ? runs[base + s] : 0u;
-      a[r] = ab & 0xffffu;
-      b[r] = ab >> 16;
+      nab[r] = 0u;
+      nrv[r] = Rec{};
+      if (s < (1u << F.shift)) {
+        nab[r] = runs[base + s];
+        if constexpr (RP) {
+          const Node<T, IS> nd = nodes[base + s];
+          nrv[r].dim1_val = nd.dim1;
+          nrv[r].dim2_val = nd.dim2;
+          if constexpr (has_id_v<IS>) nrv[r].id = nd.id;
+        } else {
+          nrv[r] = ids[base + s];
+        }
+      }
+    }
+  };
+  if (seg < F.ntrees) load(seg);
+  for (uint32_t tree = seg; tree < F.ntrees; tree += F.nseg) {
+    uint32_t a[R], b[R];  // the lane's slot of chunk r is hit by sorted positions [a, b) (empty: 0, 0)
+    Rec rv[R];            // and its record
+    uint32_t kmin = ns, kmax = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      a[r] = nab[r] & 0xffffu;
+      b[r] = nab[r] >> 16;
+      rv[r] = nrv[r];
       if (a[r] < b[r]) {
         kmin = a[r] < kmin ? a[r] : kmin;
         kmax = b[r] > kmax ? b[r] : kmax;
       }
     }
+    if (tree + F.nseg < F.ntrees) load(tree + F.nseg);
     kmin = wave_min(kmin);
     kmax = wave_max(kmax);
     kmin = __builtin_amdgcn_readfirstlane(kmin);
     kmax = __builtin_amdgcn_readfirstlane(kmax);
     if (kmin >= kmax) continue;
-    Rec rv[R];  // lane l's record of chunk r: slot 64 r + l
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-      rv[r] = Rec{};
-      if (a[r] < b[r]) {  // a slot with hits holds a point
-        if constexpr (RP) {
-          const Node<T, IS> nd = tn[r * kWave + lane];
-          rv[r].dim1_val = nd.dim1;
-          rv[r].dim2_val = nd.dim2;
-          if constexpr (has_id_v<IS>) rv[r].id = nd.id;
-        } else {
-          rv[r] = ids[base + r * kWave + lane];
-        }
-      }
-    }
     for (uint32_t w0 = kmin & ~(kWave - 1u); w0 < kmax; w0 += kWave) {
       const uint32_t kl = w0 + lane;  // the lane's sorted position of this window
       uint32_t lpos = 0, llim = 0;
