@@ -473,8 +473,9 @@ __global__ __launch_bounds__(kFBlock) void k_forest_write(const Node<T, IS>* __r
 // values sorted, the queries that hit a point form one run of sorted
 // positions, [a, b) with a = #values < dim1 and b = #values <= dim2, so a
 // pass counts hits with two binary searches per point and a difference
-// array per segment, and the write pass tests a candidate against a slot
-// with two integer compares. NaN values hit nothing and sort last.
+// array per segment, and stores every slot's run; the write pass tests a
+// candidate against a slot's stored run with two integer compares. NaN
+// values hit nothing and sort last.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct StabPrep {
