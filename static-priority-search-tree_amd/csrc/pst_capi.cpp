@@ -577,10 +577,19 @@ size_t pst_gpu_arr_calc_global_mem_needed(size_t n, unsigned threads_per_block, 
   const size_t tree =
       ntrees * (St + 1) * (pst::node_size(dtype, id_dtype) + pst::dtype_size(dtype) + id_size(id_dtype)) +
       pst::forest_bounds_bytes(dtype, n, St);
-  // construction works in LDS; a search needs a result array of n points,
-  // two 4M-entry work lists and a 4M-entry copy task list; a stationary
-  // batch keeps 4 B per slot (each slot's run of sorted stabbing values)
-  const size_t search = n * pst::point_size(dtype, id_dtype) + (3ull << 22) * 16 + ntrees * (St + 1) * 4;
+  // construction works in LDS; a blocking search needs a result array of n
+  // points and the handle's walk buffers (work lists, copy tasks and the
+  // default grid's overflow stacks, sized as for a deep tree of n points,
+  // which is at least as deep as a forest tree); a stationary batch of up
+  // to 512 queries its per-(query, segment) counters, the stabbing table
+  // and 4 B per slot; all for the current device
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  const size_t search = n ? n * pst::point_size(dtype, id_dtype) + pst::search_scratch_bytes(n, dtype, cus) +
+                                pst::forest_dense_scratch_bytes(n, St, pst::kForestDenseMax, cus)
+                          : 0;
   return tree + search;
 }
 

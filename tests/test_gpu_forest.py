@@ -163,6 +163,40 @@ def test_forest_calc_global_mem_needed(P):
     assert m >= trees * 128 * 16 + n * 12
 
 
+def test_forest_calc_global_mem_needed_covers_measured(torch, P):
+    """pst_gpu_arr_calc_global_mem_needed (pst/static-pst-gpu-arr.tu:662-686)
+    against what the device reports: the forest, a 512-query stationary
+    batch's scratch (stabbing and general) and a blocking search's buffers
+    fit in it. The caller's own query and output buffers are allocated
+    before the first reading."""
+    import pst_data as D
+    n, tpb = (1 << 22) + 5, 64
+    pts = D.interval_points(n, seed=62)
+    d = P.to_device(pts)
+    iso = np.linspace(0.0, 1.0, 512).astype(np.float32)
+    st = np.zeros(512, P.query_dtype("f32"))
+    st["max_dim1"] = st["min_dim2"] = iso
+    st["kind"] = 1
+    gen = D.float_query_mix(256, 256, seed=63)
+    dst, dgen = P.to_device(st), P.to_device(gen)
+    counts = torch.zeros(512, dtype=torch.int64, device="cuda")
+    offs = torch.zeros(513, dtype=torch.int64, device="cuda")
+    out = torch.empty(1 << 20, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    f = P.StaticPSTGPUArr(d.data_ptr(), n, tpb, "f32", "u32")
+    f.batch_search(dst, 512, counts, offs, out, 1 << 20)
+    f.batch_search(dgen, 512, counts, offs, out, 1 << 20)
+    f.search(P.LEFT, 0, 0.5, 0.5)
+    torch.cuda.synchronize()
+    used = free0 - torch.cuda.mem_get_info()[0]
+    calc = P.StaticPSTGPUArr.calcGlobalMemNeeded(n, tpb, "f32", "u32")
+    trees = (n + 126) // 127
+    print(f"forest resident after batches and a search {used}, calcGlobalMemNeeded {calc}", flush=True)
+    assert trees * 128 * (16 + 4 + 4) <= used <= calc
+    f.close()
+
+
 STATIONARY_CASES = [("f32", "u32", 64, 5000), ("f32", "u32", 1, 300), ("f32", "u32", 31, 3001),
                     ("i32", "u32", 100, 20000), ("f64", None, 255, 9000), ("i64", "i64", 128, 4099),
                     ("f32", "u64", 256, 7777), ("f64", "f32", 64, 2000), ("f32", "u32", 64, 127),
