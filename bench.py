@@ -603,7 +603,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by replicating the tree or splitting points; see DESIGN.md)",
+              "(the path would shard by replicating the tree on every GPU and splitting the query batch, "
+              "or by giving each GPU the tree of 1/N of the points and concatenating per-query results)",
               file=sys.stderr)
         sys.exit(2)
 
