@@ -569,7 +569,12 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish(
     T d2[2 * kFinishTile];  // after the build: dim2 per subtree slot, then its subtree minimum
   };
   __shared__ FinishLds U;
-  __shared__ uint32_t rank[kFinishTile];
+  union RankLds {
+    uint32_t rank[kFinishTile];
+    T d1[kFinishTile];  // after the lookups: dim1 of every point, for the meds
+  };
+  __shared__ RankLds R;
+  uint32_t* rank = R.rank;
   __shared__ uint16_t slot_of[kFinishTile];  // the subtree slot (heap order) each point fills
   __shared__ uint16_t med_of[kFinishTile];   // and the point whose dim1 is its med
   __shared__ unsigned long long sres[kFinishBlock];  // one per segment of a level with <= kFinishBlock segments
@@ -593,27 +598,33 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish(
   lds_build_local<T, IS>(U.b.bufA, U.b.bufB, rank, slot_of, med_of, U.b.sst, sres, wred, static_cast<uint32_t>(m));
   // every point's node record (rank -> input index -> record; all loads in
   // flight together): subtree slot h (level u, position k) is slot
-  // ((root + 1) << u) - 1 + k of the tree
-  uint32_t ix[kPer], im[kPer];
+  // ((root + 1) << u) - 1 + k of the tree. A med is the dim1 of a point of
+  // the same subtree, exchanged through LDS.
+  uint32_t ix[kPer];
 #pragma unroll
   for (int e = 0; e < kPer; e++) {
     const uint32_t x = threadIdx.x + e * kFinishBlock;
-    if (x < m) {
-      ix[e] = perm2[rank[x]];
-      im[e] = perm2[rank[med_of[x]]];
-    }
+    if (x < m) ix[e] = perm2[rank[x]];
   }
   Point<T, IS> p[kPer];
+#pragma unroll
+  for (int e = 0; e < kPer; e++) {
+    const uint32_t x = threadIdx.x + e * kFinishBlock;
+    if (x < m) p[e] = pts[ix[e]];
+  }
+  __syncthreads();  // the build's LDS arrays and the ranks are free from here
+#pragma unroll
+  for (int e = 0; e < kPer; e++) {
+    const uint32_t x = threadIdx.x + e * kFinishBlock;
+    if (x < m) R.d1[x] = p[e].dim1_val;
+  }
+  __syncthreads();
   T med[kPer];
 #pragma unroll
   for (int e = 0; e < kPer; e++) {
     const uint32_t x = threadIdx.x + e * kFinishBlock;
-    if (x < m) {
-      p[e] = pts[ix[e]];
-      med[e] = pts[im[e]].dim1_val;
-    }
+    if (x < m) med[e] = R.d1[med_of[x]];
   }
-  __syncthreads();  // the build's LDS arrays are free from here
 #pragma unroll
   for (int e = 0; e < kPer; e++) {
     const uint32_t x = threadIdx.x + e * kFinishBlock;
