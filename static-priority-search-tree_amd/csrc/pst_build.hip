@@ -34,8 +34,9 @@ namespace {
 constexpr int kLevelBlock = 256;
 constexpr int kLevelElems = 32;
 constexpr uint32_t kLevelTile = kLevelBlock * kLevelElems;  // 8192 points
-constexpr int kFinishBlock = 256;
+constexpr int kFinishBlock = 512;      // threads per LDS subtree of the deep tree
 constexpr uint32_t kFinishTile = 2048;  // max points per LDS subtree
+constexpr int kForestBlock = 256;       // threads per forest tree
 
 template <typename I>
 struct IdStore {
@@ -456,25 +457,45 @@ __device__ void lds_build(uint32_t* bufA, uint32_t* bufB, unsigned long long* sr
   }
 }
 
+// Orders a wave's LDS accesses around it (the compiler keeps them on their
+// side; the wave's LDS instructions execute in order).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Threads of the subtree passes: from level kWaveLevel of a subtree on, the
+// segment of position k and every segment below it belong to wave
+// k >> (u - kWaveLevel), so those levels need no workgroup barrier.
+constexpr uint32_t kFinishLog = 9;
+static_assert((1u << kFinishLog) == kFinishBlock, "block size is a power of two");
+constexpr uint32_t kWaveLevel = kFinishLog - 6;
+static_assert(kWave == 64, "64-lane waves");
+
 // The subtree build of the deep tree's last levels. Every point's rank and
 // input index are looked up once up front, all lookups in flight together;
 // the levels then work on 16-bit local positions in LDS: the node of a
 // segment is the position of least rank, and the level only notes, per
 // point, the subtree slot it fills and its med point. The node records are
 // read from the input and written after the last level, all reads in
-// flight together, so that no level waits on global memory.
+// flight together, so that no level waits on global memory. The levels
+// above kWaveLevel share segments between waves and end in a workgroup
+// barrier; from it on each wave works on its own part of the subtree.
 template <typename T, typename IS>
 __device__ void lds_build_local(uint16_t* bufA, uint16_t* bufB, const uint32_t* rank, uint16_t* slot_of,
                                 uint16_t* med_of, uint16_t* sst, unsigned long long* sres, unsigned long long* wred,
                                 uint32_t m) {
   const uint32_t tid = threadIdx.x;
-  const uint32_t nthr = blockDim.x;
+  constexpr uint32_t nthr = kFinishBlock;  // the launch's block size
   const uint64_t Mp = static_cast<uint64_t>(m) + 1;
   // node of segment k of level u: point bufA[st + q]; its subtree slot
   // h = 2^u - 1 + k; med: the point at position L or L - 1 (L = floor(sz /
   // 2)), or the node itself when L = 0. Segment starts are kept per slot
   // (sst[h], sst[0] = 0 set by the caller): with the node removed, the
-  // children start at st - k and st - k + L of the next level's array.
+  // children start at st and st + L of the next level's array. Children stay
+  // within their parent's positions (the next array has gaps where nodes
+  // left), so that a wave's part of both arrays never moves.
   auto note = [&](uint32_t u, uint32_t k, uint32_t st, uint32_t sz, uint32_t q) {
     const uint32_t L = sz >> 1;
     const uint16_t x = bufA[st + q];
@@ -482,13 +503,20 @@ __device__ void lds_build_local(uint16_t* bufA, uint16_t* bufB, const uint32_t* 
     slot_of[x] = static_cast<uint16_t>(h);
     med_of[x] = L >= 1 ? bufA[st + (q < L ? L : L - 1)] : x;
     if (sz > 1) {
-      sst[2 * h + 1] = static_cast<uint16_t>(st - k);
-      sst[2 * h + 2] = static_cast<uint16_t>(st - k + L);
+      sst[2 * h + 1] = static_cast<uint16_t>(st);
+      sst[2 * h + 2] = static_cast<uint16_t>(st + L);
     }
+  };
+  // level sync: a workgroup barrier while segments span waves, else the
+  // wave's own
+  auto level_sync = [&](uint32_t u) {
+    if (u < kWaveLevel) __syncthreads();
+    else wave_sync();
   };
   for (uint32_t u = 0; (Mp >> u) != 0; u++) {
     const uint32_t nsub = 1u << u;
     if (nsub <= nthr) {
+      // segment k on threads [k G, (k + 1) G)
       const uint32_t G = nthr / nsub;
       const uint32_t k = tid / G, g = tid % G;
       const int64_t sz = level_size(Mp, u, k);
@@ -512,19 +540,24 @@ __device__ void lds_build_local(uint16_t* bufA, uint16_t* bufB, const uint32_t* 
           for (uint32_t w = 0; w < nw; w++) best = wred[w0 + w] < best ? wred[w0 + w] : best;
         }
       }
-      if (g == 0) sres[k] = best;
-      __syncthreads();
+      // the segment's result sits at its first thread's entry, which no
+      // other wave uses at any level
+      if (g == 0) sres[k * G] = best;
+      level_sync(u);
       if (sz > 0) {
-        const uint32_t q = static_cast<uint32_t>(sres[k]);
+        const uint32_t q = static_cast<uint32_t>(sres[k * G]);
         if (g == 0) note(u, k, st, static_cast<uint32_t>(sz), q);
-        const uint32_t dst0 = st - k;
+        const uint32_t dst0 = st;
         for (int64_t x = g; x < sz; x += G) {
           if (static_cast<uint32_t>(x) == q) continue;
           bufB[dst0 + x - (static_cast<uint32_t>(x) > q ? 1 : 0)] = bufA[st + x];
         }
       }
     } else {
-      for (uint32_t k = tid; k < nsub; k += nthr) {
+      // thread t takes the 2^(u - kFinishLog) segments below its segment of
+      // level kFinishLog
+      const uint32_t per = nsub >> kFinishLog;
+      for (uint32_t k = tid * per; k < (tid + 1) * per; k++) {
         const int64_t sz = level_size(Mp, u, k);
         if (sz <= 0) continue;
         const uint32_t st = sst[(1u << u) - 1 + k];
@@ -537,7 +570,7 @@ __device__ void lds_build_local(uint16_t* bufA, uint16_t* bufB, const uint32_t* 
         const uint32_t q = static_cast<uint32_t>(best);
         note(u, k, st, static_cast<uint32_t>(sz), q);
         if (sz > 1) {
-          const uint32_t dst0 = st - k;
+          const uint32_t dst0 = st;
           for (int64_t x = 0; x < sz; x++) {
             if (static_cast<uint32_t>(x) == q) continue;
             bufB[dst0 + x - (static_cast<uint32_t>(x) > q ? 1 : 0)] = bufA[st + x];
@@ -545,11 +578,12 @@ __device__ void lds_build_local(uint16_t* bufA, uint16_t* bufB, const uint32_t* 
         }
       }
     }
-    __syncthreads();
+    level_sync(u);
     uint16_t* tmp = bufA;
     bufA = bufB;
     bufB = tmp;
   }
+  __syncthreads();
 }
 
 // One workgroup per level-t0 segment: its whole subtree in LDS, then every
@@ -577,7 +611,7 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish(
   uint32_t* rank = R.rank;
   __shared__ uint16_t slot_of[kFinishTile];  // the subtree slot (heap order) each point fills
   __shared__ uint16_t med_of[kFinishTile];   // and the point whose dim1 is its med
-  __shared__ unsigned long long sres[kFinishBlock];  // one per segment of a level with <= kFinishBlock segments
+  __shared__ unsigned long long sres[kFinishBlock];  // a segment's result, at its first thread's entry
   __shared__ unsigned long long wred[kFinishBlock / kWave];
   const uint32_t j = blockIdx.x;
   const int64_t m = level_size(M, t0, j);
@@ -625,11 +659,13 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish(
     const uint32_t x = threadIdx.x + e * kFinishBlock;
     if (x < m) med[e] = R.d1[med_of[x]];
   }
+  const uint32_t mdepth = 32u - static_cast<uint32_t>(__builtin_clz(static_cast<uint32_t>(m)));
 #pragma unroll
   for (int e = 0; e < kPer; e++) {
     const uint32_t x = threadIdx.x + e * kFinishBlock;
     if (x < m) {
       const uint32_t h = slot_of[x] + 1u;
+      if ((h >> mdepth) != 0) continue;  // never: the subtree has mdepth levels
       const uint32_t u = 31u - static_cast<uint32_t>(__builtin_clz(h));
       const uint64_t slot = ((root + 1) << u) - 1 + (h - (1u << u));
       write_node(nodes, slot, p[e], med[e]);
@@ -644,28 +680,39 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish(
   if (!submin) return;
   // subtree minima, bottom level first: min(dim2(s), minima of the children
   // that exist); a slot of size ms has a left child if ms >= 2, a right one
-  // if ms >= 3
+  // if ms >= 3. Levels from kWaveLevel down: wave w takes the slots below
+  // slot w of level kWaveLevel, with no workgroup barrier between levels;
+  // the levels above them are one thread's.
   __syncthreads();
   const uint64_t Mp = static_cast<uint64_t>(m) + 1;
   const int depth = 32 - __builtin_clz(static_cast<uint32_t>(m));
-  for (int u = depth - 1; u >= 0; u--) {
-    for (uint32_t k = threadIdx.x; k < (1u << u); k += kFinishBlock) {
-      const int64_t ms = level_size(Mp, static_cast<uint32_t>(u), k);
-      if (ms < 1) continue;
-      const uint32_t sl = (1u << u) - 1 + k;
-      T v = U.d2[sl];
-      if (ms >= 2) {
-        const T c = U.d2[2 * sl + 1];
-        v = c < v ? c : v;
-      }
-      if (ms >= 3) {
-        const T c = U.d2[2 * sl + 2];
-        v = c < v ? c : v;
-      }
-      U.d2[sl] = v;
-      submin[((root + 1) << u) - 1 + k] = v;
+  auto min_slot = [&](uint32_t u, uint32_t k) {
+    const int64_t ms = level_size(Mp, u, k);
+    if (ms < 1) return;
+    const uint32_t sl = (1u << u) - 1 + k;
+    T v = U.d2[sl];
+    if (ms >= 2) {
+      const T c = U.d2[2 * sl + 1];
+      v = c < v ? c : v;
     }
-    __syncthreads();
+    if (ms >= 3) {
+      const T c = U.d2[2 * sl + 2];
+      v = c < v ? c : v;
+    }
+    U.d2[sl] = v;
+    submin[((root + 1) << u) - 1 + k] = v;
+  };
+  const uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  for (int u = depth - 1; u >= static_cast<int>(kWaveLevel); u--) {
+    const uint32_t per = 1u << (u - kWaveLevel);
+    for (uint32_t i = lane; i < per; i += kWave) min_slot(static_cast<uint32_t>(u), wave * per + i);
+    wave_sync();
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int top = depth - 1 < static_cast<int>(kWaveLevel) - 1 ? depth - 1 : static_cast<int>(kWaveLevel) - 1;
+    for (int u = top; u >= 0; u--)
+      for (uint32_t k = 0; k < (1u << u); k++) min_slot(static_cast<uint32_t>(u), k);
   }
 }
 
@@ -754,11 +801,11 @@ struct LocalPerm {
 
 // Dynamic LDS for a tree of up to N (a power of two) points.
 __host__ __device__ constexpr size_t forest_lds_bytes(uint32_t N) {
-  return static_cast<size_t>(N) * (3 * 8 + 2 * 2 + 2 * 4) + kFinishBlock * 8 + (kFinishBlock / kWave) * 8;
+  return static_cast<size_t>(N) * (3 * 8 + 2 * 2 + 2 * 4) + kForestBlock * 8 + (kForestBlock / kWave) * 8;
 }
 
 template <typename T, typename I>
-__global__ __launch_bounds__(kFinishBlock) void k_forest_build(const Point<T, I>* __restrict__ pts, uint64_t n,
+__global__ __launch_bounds__(kForestBlock) void k_forest_build(const Point<T, I>* __restrict__ pts, uint64_t n,
                                                                uint32_t St, uint32_t L, uint32_t N,
                                                                Node<T, typename IdStore<I>::type>* __restrict__ nodes,
                                                                int* __restrict__ nan_flag) {
@@ -768,8 +815,8 @@ __global__ __launch_bounds__(kFinishBlock) void k_forest_build(const Point<T, I>
   uint64_t* k2 = k1 + N;
   uint64_t* kid = k2 + N;
   unsigned long long* sres = reinterpret_cast<unsigned long long*>(kid + N);
-  unsigned long long* wred = sres + kFinishBlock;
-  uint32_t* bufA = reinterpret_cast<uint32_t*>(wred + kFinishBlock / kWave);
+  unsigned long long* wred = sres + kForestBlock;
+  uint32_t* bufA = reinterpret_cast<uint32_t*>(wred + kForestBlock / kWave);
   uint32_t* bufB = bufA + N;
   uint16_t* o1 = reinterpret_cast<uint16_t*>(bufB + N);
   uint16_t* o2 = o1 + N;
@@ -1169,7 +1216,7 @@ Status forest_build_typed(const void* pts_v, size_t n, size_t St, void* nodes_v,
   e = flag.alloc(sizeof(int));
   if (e == hipSuccess) e = hipMemsetAsync(flag.p, 0, sizeof(int), s);
   if (e != hipSuccess) return hip_status(e, "Error in allocating construction flag");
-  hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(ntrees)), dim3(kFinishBlock), lds, s,
+  hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(ntrees)), dim3(kForestBlock), lds, s,
                      static_cast<const Point<T, I>*>(pts_v), static_cast<uint64_t>(n), static_cast<uint32_t>(St), L,
                      N, static_cast<Node<T, IS>*>(nodes_v), flag.as<int>());
   e = hipGetLastError();
