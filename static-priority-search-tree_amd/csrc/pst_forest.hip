@@ -315,25 +315,43 @@ __global__ __launch_bounds__(kScanBlock) void k_forest_seg_scan(uint32_t* __rest
   const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
   const uint32_t chunk = (nseg + kWaves * kWave - 1) / (kWaves * kWave) * kWave;  // a multiple of 64
   const uint32_t lo = std::min(nseg, wave * chunk), hi = std::min(nseg, lo + chunk);
+  // kU rows of 64 entries per step, their loads issued together
+  constexpr int kU = 8;
+  auto load_rows = [&](uint32_t i0, uint32_t (&v)[kU]) {
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const uint32_t i = i0 + u * kWave + lane;
+      v[u] = i < hi ? row[i] : 0u;
+    }
+  };
   unsigned long long sum = 0;
-  for (uint32_t i = lo + lane; i < hi; i += kWave) sum += row[i];
+  for (uint32_t i0 = lo; i0 < hi; i0 += kU * kWave) {
+    uint32_t v[kU];
+    load_rows(i0, v);
+#pragma unroll
+    for (int u = 0; u < kU; u++) sum += v[u];
+  }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, kWave);
   if (lane == 0) part[wave] = sum;
   __syncthreads();
   unsigned long long run = 0;
   for (uint32_t w = 0; w < wave; w++) run += part[w];
-  for (uint32_t i0 = lo; i0 < hi; i0 += kWave) {
-    const uint32_t i = i0 + lane;
-    const unsigned long long v = i < hi ? row[i] : 0ull;
-    unsigned long long inc = v;
+  for (uint32_t i0 = lo; i0 < hi; i0 += kU * kWave) {
+    uint32_t v[kU];
+    load_rows(i0, v);
 #pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      const unsigned long long o = __shfl_up(inc, off, kWave);
-      if (static_cast<int>(lane) >= off) inc += o;
+    for (int u = 0; u < kU; u++) {
+      const uint32_t i = i0 + u * kWave + lane;
+      unsigned long long inc = v[u];
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+        const unsigned long long o = __shfl_up(inc, off, kWave);
+        if (static_cast<int>(lane) >= off) inc += o;
+      }
+      if (i < hi) row[i] = static_cast<uint32_t>(run + inc - v[u]);
+      run += __shfl(inc, kWave - 1, kWave);
     }
-    if (i < hi) row[i] = static_cast<uint32_t>(run + inc - v);
-    run += __shfl(inc, kWave - 1, kWave);
   }
   if (threadIdx.x == 0) {
     unsigned long long t = 0;
@@ -839,13 +857,15 @@ Status forest_dense_search(const SearchArgs& a, void* segbuf, hipStream_t s, int
         constexpr int R = decltype(rtag)::value;
         // whether the batch is all stabbing queries, and their sorted values;
         // of each pair of passes below, the one for the other kind of batch
-        // returns at once
-        hipLaunchKernelGGL((k_stab_prep<T>), dim3(1), dim3(kQSlots), 0, s, qs, F.nq, prep);
-        if (a.mode == 0 || plan != 2) {
-          // plan 2 writes from the segment offsets a plan 1 count pass kept
+        // returns at once. Plan 2 writes the batch a plan 1 count pass has
+        // just counted, from its kept segment offsets, and its prep.
+        const bool kept = a.mode != 0 && plan == 2;
+        if (!kept) {
+          hipLaunchKernelGGL((k_stab_prep<T>), dim3(1), dim3(kQSlots), 0, s, qs, F.nq, prep);
+          // the stabbing pass first: while it runs, the host queues the rest
+          hipLaunchKernelGGL((k_stab_count<T, IS, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, F, prep, segcnt, runs);
           hipLaunchKernelGGL((k_forest_count<T, IS, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, bounds, F, qs,
                              stab, segcnt);
-          hipLaunchKernelGGL((k_stab_count<T, IS, R>), dim3(grid), dim3(kFBlock), 0, s, nodes, F, prep, segcnt, runs);
           // write mode without kept offsets: the per-query totals go to the cursor
           hipLaunchKernelGGL(k_forest_seg_scan, dim3(F.nq), dim3(kScanBlock), 0, s, segcnt, F.nseg, a.counts);
         }
