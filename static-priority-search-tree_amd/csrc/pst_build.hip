@@ -473,9 +473,9 @@ static_assert((1u << kFinishLog) == kFinishBlock, "block size is a power of two"
 constexpr uint32_t kWaveLevel = kFinishLog - 6;
 static_assert(kWave == 64, "64-lane waves");
 
-// The subtree build of the deep tree's last levels. Every point's rank and
-// input index are looked up once up front, all lookups in flight together;
-// the levels then work on 16-bit local positions in LDS: the node of a
+// The subtree build of the deep tree's last levels. Every point's rank is
+// read once up front, all reads in flight together; the levels then work
+// on 16-bit local positions in LDS: the node of a
 // segment is the position of least rank, and the level only notes, per
 // point, the subtree slot it fills and its med point. The node records are
 // read from the input and written after the last level, all reads in
