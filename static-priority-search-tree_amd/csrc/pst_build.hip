@@ -55,11 +55,13 @@ struct IdStore<NoId> {
 // compareDim2(...) > 0.
 // ---------------------------------------------------------------------------
 // ORDER_CHECK: also sets flags[1] when some point's ID is above the next
-// point's (IDs not in index order), and flags[0] for a NaN ID.
+// point's (IDs not in index order), and flags[0] for a NaN ID. With key1
+// (32-bit keys only), the same read of point i also stores its ascending
+// dim1 key at key1[i], the keys of the dim1 sort.
 template <typename T, typename I, typename K, bool DIM2, bool DESC, bool WITH_ID, bool ORDER_CHECK = false>
 __global__ void k_keys_single(const Point<T, I>* __restrict__ pts, uint32_t n, K* __restrict__ keys,
                               uint32_t* __restrict__ vals, const uint32_t* __restrict__ r2,
-                              int* __restrict__ flags) {
+                              int* __restrict__ flags, uint32_t* __restrict__ key1 = nullptr) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint32_t i = DESC ? n - 1 - j : j;
@@ -76,6 +78,12 @@ __global__ void k_keys_single(const Point<T, I>* __restrict__ pts, uint32_t n, K
   if constexpr (ORDER_CHECK) {
     bad |= is_nan(p.id);
     if (i + 1 < n && ord_key(p.id) > ord_key(pts[i + 1].id)) flags[1] = 1;
+  }
+  if constexpr (sizeof(K) == 4) {
+    if (key1) {
+      bad |= is_nan(p.dim1_val);
+      key1[i] = static_cast<uint32_t>(ord_key(p.dim1_val));
+    }
   }
   if (bad) flags[0] = 1;
   keys[j] = DESC ? ~k : k;
@@ -1081,7 +1089,10 @@ Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, 
   // ---- key2 descending -> perm2 (v1), r2. One sort by dim2 when the IDs
   // are in index order (the sort's index tie-break is then the (id, index)
   // tie-break), else a sort by ID and then a stable one by dim2.
-  k_keys_single<T, I, K, true, true, false, kHasId><<<G, B, 0, s>>>(pts, n, key0, v0, nullptr, flags);
+  // 32-bit keys: the dim1 keys are stored by the same kernel, into v2
+  constexpr bool kKey1First = sizeof(K) == 4;
+  k_keys_single<T, I, K, true, true, false, kHasId>
+      <<<G, B, 0, s>>>(pts, n, key0, v0, nullptr, flags, kKey1First ? v2 : nullptr);
   bool ordered = true;
   if constexpr (kHasId) {
     int hflags[2] = {0, 0};
@@ -1101,8 +1112,14 @@ Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, 
   }
   const uint32_t* perm2 = v1;
   k_scatter_rank<<<G, B, 0, s>>>(perm2, n, r2);
-  // ---- key1 ascending, carrying r2 -> A (v2)
-  if (ordered) {
+  // ---- key1 ascending, carrying r2 -> A (v2; v0 when the dim1 keys came
+  // with the dim2 keys, whose sort reads them from v2 and r2 in place)
+  uint32_t* Acur = v2;
+  uint32_t* Anxt = v0;
+  if (ordered && kKey1First) {
+    PST_TRY(sortK(reinterpret_cast<const K*>(v2), key0, r2, v0), "Error in sorting by dimension 1");
+    std::swap(Acur, Anxt);
+  } else if (ordered) {
     k_keys_single<T, I, K, false, false, false><<<G, B, 0, s>>>(pts, n, key0, v0, r2, flags);
     PST_TRY(sortK(key0, key1, v0, v2), "Error in sorting by dimension 1");
   } else {
@@ -1114,8 +1131,6 @@ Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, 
     }
   }
   mark(1);
-  uint32_t* Acur = v2;
-  uint32_t* Anxt = v0;
 
   // ---- level schedule (tables computed above)
   const uint32_t t0 = BL.t0;
