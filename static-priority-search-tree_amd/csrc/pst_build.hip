@@ -1016,8 +1016,7 @@ using BuildKey = typename std::conditional<(sizeof(T) == 8 || (has_id_v<I> && si
                                            uint32_t>::type;
 
 template <typename T, typename I>
-Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, BuildTimes* times,
-                   void* submin_v, void* ids_v) {
+Status build_typed(const void* pts_v, size_t n64, BuildOut& out, hipStream_t s, BuildTimes* times) {
   using IS = typename IdStore<I>::type;
   using K = BuildKey<T, I>;
   Status st;
@@ -1031,7 +1030,6 @@ Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, 
   const uint64_t M = n64 + 1;
   const auto* pts = static_cast<const Point<T, I>*>(pts_v);
   const auto* ptsS = static_cast<const Point<T, IS>*>(pts_v);
-  auto* nodes = static_cast<Node<T, IS>*>(nodes_v);
   constexpr bool kHasId = has_id_v<I>;
 
   hipEvent_t ev[5] = {};
@@ -1131,6 +1129,17 @@ Status build_typed(const void* pts_v, size_t n64, void* nodes_v, hipStream_t s, 
     }
   }
   mark(1);
+  // the output arrays: the caller's hook allocates them while the sorts run
+  if (out.alloc) {
+    Status sa = out.alloc(out);
+    if (!sa.ok()) {
+      (void)hipStreamSynchronize(s);  // the queued passes still use the scratch
+      return sa;
+    }
+  }
+  auto* nodes = static_cast<Node<T, IS>*>(out.nodes);
+  void* submin_v = out.submin;
+  void* ids_v = out.ids;
 
   // ---- level schedule (tables computed above)
   const uint32_t t0 = BL.t0;
@@ -1283,13 +1292,20 @@ Status with_id(int idt, F&& f) {
 
 }  // namespace
 
+Status build_tree(int dtype, int id_dtype, const void* pts_d, size_t n, BuildOut& out, hipStream_t stream,
+                  BuildTimes* times) {
+  return with_t(dtype, [&](auto t) {
+    return with_id(id_dtype, [&](auto i) { return build_typed<decltype(t), decltype(i)>(pts_d, n, out, stream, times); });
+  });
+}
+
 Status build_tree(int dtype, int id_dtype, const void* pts_d, size_t n, void* nodes_d, hipStream_t stream,
                   BuildTimes* times, void* submin_d, void* ids_d) {
-  return with_t(dtype, [&](auto t) {
-    return with_id(id_dtype, [&](auto i) {
-      return build_typed<decltype(t), decltype(i)>(pts_d, n, nodes_d, stream, times, submin_d, ids_d);
-    });
-  });
+  BuildOut out;
+  out.nodes = nodes_d;
+  out.submin = submin_d;
+  out.ids = ids_d;
+  return build_tree(dtype, id_dtype, pts_d, n, out, stream, times);
 }
 
 size_t build_scratch_bytes(int dtype, int id_dtype, size_t n) {

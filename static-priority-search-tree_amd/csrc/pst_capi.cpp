@@ -349,16 +349,22 @@ int pst_gpu_create(const void* pt_arr_d, size_t n, int dtype, int id_dtype, int 
     *out = t;
     return PST_OK;
   }
-  hipError_t e = hipMalloc(&t->nodes, t->S * pst::node_size(dtype, id_dtype));
-  if (e == hipSuccess) e = hipMalloc(&t->submin, t->S * pst::dtype_size(dtype));
-  if (e == hipSuccess && id_size(id_dtype) > 0) e = hipMalloc(&t->ids, t->S * id_size(id_dtype));
-  if (e != hipSuccess) {
-    std::string m = "Error in allocating priority search tree storage array" + dev_suffix(dev_ind, num_devs) +
-                    hipGetErrorString(e);
-    pst_gpu_destroy(t);
-    return fail(PST_ERR_HIP, m);
-  }
-  Status st = pst::build_tree(dtype, id_dtype, pt_arr_d, n, t->nodes, t->stream, nullptr, t->submin, t->ids);
+  // the tree's arrays are allocated by the build once its sorts are queued,
+  // so that the allocation overlaps them
+  pst::BuildOut bo;
+  bo.alloc = [&](pst::BuildOut& o) {
+    hipError_t e = hipMalloc(&t->nodes, t->S * pst::node_size(dtype, id_dtype));
+    if (e == hipSuccess) e = hipMalloc(&t->submin, t->S * pst::dtype_size(dtype));
+    if (e == hipSuccess && id_size(id_dtype) > 0) e = hipMalloc(&t->ids, t->S * id_size(id_dtype));
+    if (e != hipSuccess)
+      return Status{PST_ERR_HIP, "Error in allocating priority search tree storage array" +
+                                     dev_suffix(dev_ind, num_devs) + hipGetErrorString(e)};
+    o.nodes = t->nodes;
+    o.submin = t->submin;
+    o.ids = t->ids;
+    return Status{};
+  };
+  Status st = pst::build_tree(dtype, id_dtype, pt_arr_d, n, bo, t->stream);
   if (!st.ok()) {
     pst_gpu_destroy(t);
     return fail(st);

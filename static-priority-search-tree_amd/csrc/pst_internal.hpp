@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <string>
 
 #include "pst_common.hpp"
@@ -42,6 +43,18 @@ struct BuildTimes {
 // without IDs) also fills what build_submin(.., slots_per_tree = 0, ..) does.
 Status build_tree(int dtype, int id_dtype, const void* pts_d, size_t n, void* nodes_d, hipStream_t stream,
                   BuildTimes* times, void* submin_d = nullptr, void* ids_d = nullptr);
+// The arrays a build writes: node records and, optionally, per-slot subtree
+// minima and IDs. With `alloc` set, the build calls it once its sorts are
+// queued (the arrays are first written after them), so that allocating
+// them overlaps the sorts; alloc sets nodes, submin and ids.
+struct BuildOut {
+  void* nodes = nullptr;
+  void* submin = nullptr;
+  void* ids = nullptr;
+  std::function<Status(BuildOut&)> alloc;
+};
+Status build_tree(int dtype, int id_dtype, const void* pts_d, size_t n, BuildOut& out, hipStream_t stream,
+                  BuildTimes* times = nullptr);
 
 // Device bytes one construction of n points allocates (freed when it ends).
 size_t build_scratch_bytes(int dtype, int id_dtype, size_t n);
