@@ -950,6 +950,28 @@ struct DevBuf {
   }
 };
 
+// Scratch in stream order on one stream: allocated before the passes that
+// use it are queued, and freed after them on the same stream (a free queued
+// before the stream's last synchronization returns the memory there, rather
+// than in a blocking free afterwards).
+struct StreamBuf {
+  void* p = nullptr;
+  hipStream_t s = nullptr;
+  ~StreamBuf() { release(); }
+  hipError_t alloc(size_t bytes, hipStream_t st) {
+    s = st;
+    return hipMallocAsync(&p, bytes ? bytes : 1, st);
+  }
+  void release() {
+    if (p) (void)hipFreeAsync(p, s);
+    p = nullptr;
+  }
+  template <typename U>
+  U* as() const {
+    return static_cast<U*>(p);
+  }
+};
+
 inline unsigned grid_for(size_t n, unsigned block) { return static_cast<unsigned>((n + block - 1) / block); }
 
 // Construction buffers of n points, carved from one allocation: two key
@@ -1046,8 +1068,8 @@ Status build_typed(const void* pts_v, size_t n64, BuildOut& out, hipStream_t s, 
     if (e != hipSuccess) return hip_status(e, what); \
   } while (0)
   const BuildLayout BL = build_layout<K>(n64);
-  DevBuf arena;
-  PST_TRY(arena.alloc(BL.total), "Error in allocating construction arrays");
+  StreamBuf arena;
+  PST_TRY(arena.alloc(BL.total, s), "Error in allocating construction arrays");
   unsigned char* base = arena.as<unsigned char>();
   K* key0 = reinterpret_cast<K*>(base + BL.key0);
   K* key1 = reinterpret_cast<K*>(base + BL.key1);
@@ -1191,6 +1213,7 @@ Status build_typed(const void* pts_v, size_t n64, BuildOut& out, hipStream_t s, 
   int hflag[3] = {0, 0, 0};
   PST_TRY(hipMemcpyAsync(hflag, flags, sizeof(hflag), hipMemcpyDeviceToHost, s),
           "Error in reading construction flag");
+  arena.release();  // after the flag copy, in stream order
   PST_TRY(hipStreamSynchronize(s), "Error in synchronising after construction");
   if (times) {
     hipEventElapsedTime(&times->sort_ms, ev[0], ev[1]);
