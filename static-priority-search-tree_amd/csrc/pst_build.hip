@@ -978,7 +978,8 @@ inline unsigned grid_for(size_t n, unsigned block) { return static_cast<unsigned
 // arrays, four 32-bit arrays (sort values, the key2 order, the level arrays
 // and the key2 ranks), flags, radix sort storage, level tables.
 struct BuildLayout {
-  size_t key0, key1, v0, v1, v2, r2, flags, tmp, starts, segmin, tileseg, tmin, total;
+  size_t key0, key1, v0, v1, v2, r2, flags, tmp, starts, segmin, tileseg, tmin;
+  size_t total_a, total_b, total;  // part A, part B, both
   size_t tile_stride;  // tileseg entries per level
   size_t tmp_bytes;
   uint32_t t0;
@@ -1015,20 +1016,25 @@ BuildLayout build_layout(size_t n) {
     o += al256(bytes ? bytes : 1);
     return at;
   };
+  // part A: what the first key kernel writes
   L.key0 = take(n * sizeof(K));
-  L.key1 = take(n * sizeof(K));
   L.v0 = take(n * 4);
-  L.v1 = take(n * 4);
   L.v2 = take(n * 4);
-  L.r2 = take(n * 4);
   L.flags = take(16);
+  L.total_a = o;
+  // part B (offsets from its own start): the rest
+  o = 0;
+  L.key1 = take(n * sizeof(K));
+  L.v1 = take(n * 4);
+  L.r2 = take(n * 4);
   L.tmp = take(tb);
   L.starts = take(nstart * 4);
   L.segmin = take(nmin * 8);
   L.tile_stride = (n + kLevelTile - 1) / kLevelTile;
   L.tileseg = take(static_cast<size_t>(L.t0) * L.tile_stride * 4);
   L.tmin = take(L.tile_stride * 2 * kSegSlots * 8);
-  L.total = o;
+  L.total_b = o;
+  L.total = L.total_a + L.total_b;
   return L;
 }
 
@@ -1068,40 +1074,17 @@ Status build_typed(const void* pts_v, size_t n64, BuildOut& out, hipStream_t s, 
     if (e != hipSuccess) return hip_status(e, what); \
   } while (0)
   const BuildLayout BL = build_layout<K>(n64);
-  StreamBuf arena;
-  PST_TRY(arena.alloc(BL.total, s), "Error in allocating construction arrays");
-  unsigned char* base = arena.as<unsigned char>();
-  K* key0 = reinterpret_cast<K*>(base + BL.key0);
-  K* key1 = reinterpret_cast<K*>(base + BL.key1);
-  uint32_t* v0 = reinterpret_cast<uint32_t*>(base + BL.v0);
-  uint32_t* v1 = reinterpret_cast<uint32_t*>(base + BL.v1);
-  uint32_t* v2 = reinterpret_cast<uint32_t*>(base + BL.v2);
-  uint32_t* r2 = reinterpret_cast<uint32_t*>(base + BL.r2);
-  int* flags = reinterpret_cast<int*>(base + BL.flags);
-  void* tmp = base + BL.tmp;
+  // scratch in two parts: what the first key kernel writes, then the rest,
+  // allocated while that kernel runs
+  StreamBuf arena_a, arena_b;
+  PST_TRY(arena_a.alloc(BL.total_a, s), "Error in allocating construction arrays");
+  unsigned char* base_a = arena_a.as<unsigned char>();
+  K* key0 = reinterpret_cast<K*>(base_a + BL.key0);
+  uint32_t* v0 = reinterpret_cast<uint32_t*>(base_a + BL.v0);
+  uint32_t* v2 = reinterpret_cast<uint32_t*>(base_a + BL.v2);
+  int* flags = reinterpret_cast<int*>(base_a + BL.flags);
   // flags[0]: NaN seen; flags[1]: IDs not in index order; flags[2]: a sort's wait ran out
   PST_TRY(hipMemsetAsync(flags, 0, 3 * sizeof(int), s), "Error in clearing construction flags");
-  // level tables, which depend on n alone: segment starts of levels 0..t0
-  // and, per level t < t0 and tile of kLevelTile points, the segment
-  // holding the tile's first point (so that no workgroup searches the
-  // starts itself)
-  {
-    const uint32_t nstart = (2u << BL.t0) - 1u;
-    uint32_t* st_d = reinterpret_cast<uint32_t*>(base + BL.starts);
-    k_level_starts<<<grid_for(nstart, 256), 256, 0, s>>>(n64 + 1, nstart, st_d);
-    if (BL.t0 > 0) {
-      const uint64_t cells = static_cast<uint64_t>(BL.t0) * BL.tile_stride;
-      k_tile_segments<<<grid_for(cells, 256), 256, 0, s>>>(st_d, static_cast<uint32_t>(n64), BL.t0,
-                                                           static_cast<uint32_t>(BL.tile_stride),
-                                                           reinterpret_cast<uint32_t*>(base + BL.tileseg));
-    }
-  }
-  // stable radix sort of (key, value) pairs: kin/vin to kout/vout
-  auto sortK = [&](const K* kin, K* kout, const uint32_t* vin, uint32_t* vout) -> hipError_t {
-    size_t tb = BL.tmp_bytes;
-    if constexpr (sizeof(K) == 4) return radix_sort_u32(tmp, tb, kin, kout, vin, vout, n, s, flags + 2);
-    else return radix_sort_u64(tmp, tb, kin, kout, vin, vout, n, s, flags + 2);
-  };
   const unsigned B = 256;
   const unsigned G = grid_for(n, B);
   int* nan_flag = flags;
@@ -1113,6 +1096,33 @@ Status build_typed(const void* pts_v, size_t n64, BuildOut& out, hipStream_t s, 
   constexpr bool kKey1First = sizeof(K) == 4;
   k_keys_single<T, I, K, true, true, false, kHasId>
       <<<G, B, 0, s>>>(pts, n, key0, v0, nullptr, flags, kKey1First ? v2 : nullptr);
+  PST_TRY(arena_b.alloc(BL.total_b, s), "Error in allocating construction arrays");
+  unsigned char* base_b = arena_b.as<unsigned char>();
+  K* key1 = reinterpret_cast<K*>(base_b + BL.key1);
+  uint32_t* v1 = reinterpret_cast<uint32_t*>(base_b + BL.v1);
+  uint32_t* r2 = reinterpret_cast<uint32_t*>(base_b + BL.r2);
+  void* tmp = base_b + BL.tmp;
+  uint32_t* starts_d = reinterpret_cast<uint32_t*>(base_b + BL.starts);
+  uint32_t* tileseg_d = reinterpret_cast<uint32_t*>(base_b + BL.tileseg);
+  // level tables, which depend on n alone: segment starts of levels 0..t0
+  // and, per level t < t0 and tile of kLevelTile points, the segment
+  // holding the tile's first point (so that no workgroup searches the
+  // starts itself)
+  {
+    const uint32_t nstart = (2u << BL.t0) - 1u;
+    k_level_starts<<<grid_for(nstart, 256), 256, 0, s>>>(n64 + 1, nstart, starts_d);
+    if (BL.t0 > 0) {
+      const uint64_t cells = static_cast<uint64_t>(BL.t0) * BL.tile_stride;
+      k_tile_segments<<<grid_for(cells, 256), 256, 0, s>>>(starts_d, static_cast<uint32_t>(n64), BL.t0,
+                                                           static_cast<uint32_t>(BL.tile_stride), tileseg_d);
+    }
+  }
+  // stable radix sort of (key, value) pairs: kin/vin to kout/vout
+  auto sortK = [&](const K* kin, K* kout, const uint32_t* vin, uint32_t* vout) -> hipError_t {
+    size_t tb = BL.tmp_bytes;
+    if constexpr (sizeof(K) == 4) return radix_sort_u32(tmp, tb, kin, kout, vin, vout, n, s, flags + 2);
+    else return radix_sort_u64(tmp, tb, kin, kout, vin, vout, n, s, flags + 2);
+  };
   bool ordered = true;
   if constexpr (kHasId) {
     int hflags[2] = {0, 0};
@@ -1165,10 +1175,8 @@ Status build_typed(const void* pts_v, size_t n64, BuildOut& out, hipStream_t s, 
 
   // ---- level schedule (tables computed above)
   const uint32_t t0 = BL.t0;
-  uint32_t* starts_d = reinterpret_cast<uint32_t*>(base + BL.starts);
-  uint32_t* tileseg_d = reinterpret_cast<uint32_t*>(base + BL.tileseg);
   const size_t nmin = t0 ? (1ull << t0) - 1 : 1;
-  unsigned long long* segmin = reinterpret_cast<unsigned long long*>(base + BL.segmin);
+  unsigned long long* segmin = reinterpret_cast<unsigned long long*>(base_b + BL.segmin);
   k_fill_u64<<<grid_for(nmin, B), B, 0, s>>>(segmin, nmin, ~0ull);
   const uint32_t* starts = starts_d;
   // level 0's minimum by its own pass; each emit pass then finds the next
@@ -1180,7 +1188,7 @@ Status build_typed(const void* pts_v, size_t n64, BuildOut& out, hipStream_t s, 
   // Levels with at most kReduceSegs segments combine the tiles' minima by a
   // reduction instead of global atomics.
   constexpr uint32_t kReduceSegs = 32;
-  unsigned long long* tmin = reinterpret_cast<unsigned long long*>(base + BL.tmin);
+  unsigned long long* tmin = reinterpret_cast<unsigned long long*>(base_b + BL.tmin);
   if (t0 > 0) {
     k_level_argmin<<<grid_for(n, kLevelTile), kLevelBlock, 0, s>>>(Acur, n, 1u, starts, tileseg_d, tmin);
     k_tile_min_reduce<<<1, kLevelBlock, 0, s>>>(tmin, kSegSlots, tileseg_d, starts, 1u, n, 0u, segmin);
@@ -1213,7 +1221,8 @@ Status build_typed(const void* pts_v, size_t n64, BuildOut& out, hipStream_t s, 
   int hflag[3] = {0, 0, 0};
   PST_TRY(hipMemcpyAsync(hflag, flags, sizeof(hflag), hipMemcpyDeviceToHost, s),
           "Error in reading construction flag");
-  arena.release();  // after the flag copy, in stream order
+  arena_b.release();  // after the flag copy, in stream order
+  arena_a.release();
   PST_TRY(hipStreamSynchronize(s), "Error in synchronising after construction");
   if (times) {
     hipEventElapsedTime(&times->sort_ms, ev[0], ev[1]);
