@@ -658,23 +658,12 @@ Status build_level_index(int dtype, int id_dtype, size_t n, const void* nodes_d,
       hipError_t e = sizeof(K) == 4
                          ? radix_sort_u32(nullptr, tmp_bytes, nullptr, nullptr, nullptr, nullptr, n, s)
                          : radix_sort_u64(nullptr, tmp_bytes, nullptr, nullptr, nullptr, nullptr, n, s);
-      // scratch in stream order on s: freed after the passes, before the
-      // last synchronization, which returns it
-      auto alloc = [&](auto** p, size_t bytes) {
-        if (e == hipSuccess) e = hipMallocAsync(reinterpret_cast<void**>(p), bytes, s);
-      };
-      alloc(&kin, n * sizeof(K));
-      alloc(&sa, n * 4);
-      alloc(&sb, n * 4);
-      alloc(&agg, (ntiles + 1) * 8ull);
-      alloc(&pre, (ntiles + 1) * 8ull);
-      alloc(&tmp, std::max<size_t>(tmp_bytes, 16));
-      auto release = [&]() {
-        for (void* p : {static_cast<void*>(kin), static_cast<void*>(sa), static_cast<void*>(sb),
-                        static_cast<void*>(agg), static_cast<void*>(pre), tmp})
-          if (p) (void)hipFreeAsync(p, s);
-        kin = nullptr, sa = sb = nullptr, agg = pre = nullptr, tmp = nullptr;
-      };
+      if (e == hipSuccess) e = hipMalloc(&kin, n * sizeof(K));
+      if (e == hipSuccess) e = hipMalloc(&sa, n * 4);
+      if (e == hipSuccess) e = hipMalloc(&sb, n * 4);
+      if (e == hipSuccess) e = hipMalloc(&agg, (ntiles + 1) * 8ull);
+      if (e == hipSuccess) e = hipMalloc(&pre, (ntiles + 1) * 8ull);
+      if (e == hipSuccess) e = hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 16));
       Status st = hip_status(e, "Error in allocating level index construction buffers");
       if (st.ok()) {
         const unsigned blocks = (n32 + kLvBlock - 1) / kLvBlock;
@@ -703,10 +692,15 @@ Status build_level_index(int dtype, int id_dtype, size_t n, const void* nodes_d,
           std::swap(sa, sb);
         }
         if (st.ok()) st = hip_status(hipGetLastError(), "Error in launching level index construction");
+        if (st.ok()) st = hip_status(hipStreamSynchronize(s), "Error in building the level index");
       }
-      release();
-      const Status sync = hip_status(hipStreamSynchronize(s), "Error in building the level index");
-      return st.ok() ? sync : st;
+      if (kin) hipFree(kin);
+      if (sa) hipFree(sa);
+      if (sb) hipFree(sb);
+      if (agg) hipFree(agg);
+      if (pre) hipFree(pre);
+      if (tmp) hipFree(tmp);
+      return st;
     });
   });
 }
