@@ -940,30 +940,10 @@ __global__ void k_export(const Node<T, IS>* __restrict__ nodes, uint64_t S,
 // ---------------------------------------------------------------------------
 struct DevBuf {
   void* p = nullptr;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
+  ~DevBuf() { release(); }
   hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
-  template <typename U>
-  U* as() const {
-    return static_cast<U*>(p);
-  }
-};
-
-// Scratch in stream order on one stream: allocated before the passes that
-// use it are queued, and freed after them on the same stream (a free queued
-// before the stream's last synchronization returns the memory there, rather
-// than in a blocking free afterwards).
-struct StreamBuf {
-  void* p = nullptr;
-  hipStream_t s = nullptr;
-  ~StreamBuf() { release(); }
-  hipError_t alloc(size_t bytes, hipStream_t st) {
-    s = st;
-    return hipMallocAsync(&p, bytes ? bytes : 1, st);
-  }
   void release() {
-    if (p) (void)hipFreeAsync(p, s);
+    if (p) (void)hipFree(p);
     p = nullptr;
   }
   template <typename U>
@@ -1076,8 +1056,8 @@ Status build_typed(const void* pts_v, size_t n64, BuildOut& out, hipStream_t s, 
   const BuildLayout BL = build_layout<K>(n64);
   // scratch in two parts: what the first key kernel writes, then the rest,
   // allocated while that kernel runs
-  StreamBuf arena_a, arena_b;
-  PST_TRY(arena_a.alloc(BL.total_a, s), "Error in allocating construction arrays");
+  DevBuf arena_a, arena_b;
+  PST_TRY(arena_a.alloc(BL.total_a), "Error in allocating construction arrays");
   unsigned char* base_a = arena_a.as<unsigned char>();
   K* key0 = reinterpret_cast<K*>(base_a + BL.key0);
   uint32_t* v0 = reinterpret_cast<uint32_t*>(base_a + BL.v0);
@@ -1096,7 +1076,7 @@ Status build_typed(const void* pts_v, size_t n64, BuildOut& out, hipStream_t s, 
   constexpr bool kKey1First = sizeof(K) == 4;
   k_keys_single<T, I, K, true, true, false, kHasId>
       <<<G, B, 0, s>>>(pts, n, key0, v0, nullptr, flags, kKey1First ? v2 : nullptr);
-  PST_TRY(arena_b.alloc(BL.total_b, s), "Error in allocating construction arrays");
+  PST_TRY(arena_b.alloc(BL.total_b), "Error in allocating construction arrays");
   unsigned char* base_b = arena_b.as<unsigned char>();
   K* key1 = reinterpret_cast<K*>(base_b + BL.key1);
   uint32_t* v1 = reinterpret_cast<uint32_t*>(base_b + BL.v1);
@@ -1221,9 +1201,9 @@ Status build_typed(const void* pts_v, size_t n64, BuildOut& out, hipStream_t s, 
   int hflag[3] = {0, 0, 0};
   PST_TRY(hipMemcpyAsync(hflag, flags, sizeof(hflag), hipMemcpyDeviceToHost, s),
           "Error in reading construction flag");
-  arena_b.release();  // after the flag copy, in stream order
-  arena_a.release();
   PST_TRY(hipStreamSynchronize(s), "Error in synchronising after construction");
+  arena_b.release();
+  arena_a.release();
   if (times) {
     hipEventElapsedTime(&times->sort_ms, ev[0], ev[1]);
     hipEventElapsedTime(&times->levels_ms, ev[1], ev[2]);
